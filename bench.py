@@ -1,0 +1,165 @@
+"""Benchmark: H psi band-applications/s on the Si64 H-psi microbenchmark.
+
+Workload (one GPU): Si64 simple-cubic supercell (2x2x2 conventional diamond cells,
+a = 20.5262 bohr), GTH LDA Si, ecutwfc 15 Ha, 72^3 dense FFT grid (Ns_ override),
+Gamma point, Ngw = 23871 plane waves, 160 bands, Nbeta = 320 projectors.  psi is complex
+i.i.d. normal from a seed, Loewdin-orthonormalised; V_loc(r) = V_Ps_loc + V_H + V_xc of
+the Gaussian atomic-superposition density.  Inputs are resident in HBM before timing.
+
+A step is one op_H over all 160 bands (a band-application is one column through op_H).
+`value` is the mixed-precision path (FP32 FFT + FP32-MFMA projectors); the FP64 path is
+timed beside it, and dE/atom between the two (sum_b <psi_b|H|psi_b> / Natoms) is reported.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pwdft.jl_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+A_SI64 = 20.5262
+NBANDS = 160
+PEAK_HBM_GBS = 8000.0
+
+
+def build_system(mixed=False):
+    import pwdft_hip as ph
+    frac = ph.diamond_supercell(1.0, 2)
+    atoms = ph.Atoms(["Si"] * len(frac), frac, ph.gen_lattice_sc(A_SI64))
+    H = ph.Hamiltonian(atoms, ["Si-q4.lda.gth"], 15.0, Ns_=(72, 72, 72), Nstates=NBANDS, mixed=mixed,
+                       band_batch=NBANDS)
+    H.update(H.guess_rhoe())
+    return H
+
+
+def make_psi(H, seed=20260101):
+    import pwdft_hip as ph
+    rng = np.random.default_rng(seed)
+    n = H.pw.Ngw[0]
+    a = rng.standard_normal((n, NBANDS)) + 1j * rng.standard_normal((n, NBANDS))
+    X = ph.DeviceMatrix.from_host(H.ctx, a)
+    H.ik = 0
+    ph.ortho_sqrt_(H, X)
+    return X
+
+
+def cpu_baseline(seconds=12.0):
+    """Time the oracle (CPU restatement, reference loop structure) on a bounded sample of
+    the same workload, single thread; runs in a child process with 1 BLAS thread."""
+    code = r"""
+import os, sys, time, json
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import numpy as np
+from oracle.setup import Atoms, PsPot_GTH, gen_lattice_sc
+from oracle.energy import Hamiltonian
+from oracle.density import guess_rhoe
+from oracle import ops
+import pwdft_hip.basis as B
+frac = B.diamond_supercell(1.0, 2)
+atoms = Atoms(["Si"]*len(frac), frac, gen_lattice_sc(%r))
+H = Hamiltonian(atoms, [PsPot_GTH(os.path.join(%r, "Si-q4.lda.gth"))], 15.0, Ns=(72,72,72), Nstates=%d)
+H.update(guess_rhoe(H))
+rng = np.random.default_rng(1)
+n = H.pw.Ngw[0]
+done = 0; t0 = time.perf_counter()
+while time.perf_counter() - t0 < %r:
+    psi = rng.standard_normal((n, 4)) + 1j*rng.standard_normal((n, 4))
+    ops.op_H(H, psi); done += 4
+dt = time.perf_counter() - t0
+print(json.dumps({"bands": done, "seconds": dt}))
+""" % (ROOT, os.path.join(ROOT, "pwdft.jl_amd"), A_SI64,
+       os.path.join(ROOT, "pwdft.jl_amd", "pwdft_hip", "pseudopotentials"), NBANDS, seconds)
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return {"value": None, "unit": "band-applications/s", "cores": 1, "kind": "port",
+                "sample": "failed: " + r.stderr[-300:]}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": d["bands"] / d["seconds"], "unit": "band-applications/s", "cores": 1, "kind": "port",
+            "sample": f"oracle op_H (numpy FFT + GTH rank-1 nonlocal loop) on {d['bands']} bands of the "
+                      f"same Si64 system in {d['seconds']:.1f} s, 1 thread"}
+
+
+def time_op_H(H, X, Y, steps, warmup, mixed):
+    from pwdft_hip._lib import PW_MIXED, PW_TERM_ALL
+    c = H.ctx
+    flags = PW_TERM_ALL | (PW_MIXED if mixed else 0)
+    for _ in range(warmup):
+        c.call("pw_op_H_dev", 0, 0, NBANDS, X.p, X.rows, Y.p, Y.rows, flags)
+    c.call("pw_synchronize")
+    c.call("pw_event_record", 0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.call("pw_op_H_dev", 0, 0, NBANDS, X.p, X.rows, Y.p, Y.rows, flags)
+    c.call("pw_event_record", 1)
+    c.call("pw_synchronize")
+    wall = time.perf_counter() - t0
+    import ctypes
+    ms = ctypes.c_float()
+    c.call("pw_event_elapsed_ms", 0, 1, ctypes.byref(ms))
+    return ms.value / 1e3, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
+        sys.exit(2)
+    import pwdft_hip as ph
+    print("# building Si64 system", file=sys.stderr, flush=True)
+    H = build_system()
+    X = make_psi(H)
+    Y = ph.DeviceMatrix(H.ctx, X.rows, NBANDS)
+    print(f"# Ngw={H.pw.Ngw[0]} Nbeta={H.pspotNL.NbetaNL} Ns={H.pw.Ns}", file=sys.stderr, flush=True)
+    # accuracy of the mixed path: band energies <psi|H|psi>
+    res = {}
+    from pwdft_hip.solvers import _Dense
+    for mixed in (False, True):
+        sec, wall = time_op_H(H, X, Y, args.steps, args.warmup, mixed)
+        D = _Dense(H, False)
+        e = np.real(np.diag(D.gram(X.p, Y.p, NBANDS, NBANDS, X.rows)))
+        res["mixed" if mixed else "fp64"] = {"sec": sec, "wall": wall, "eband": e}
+        print(f"# {'mixed' if mixed else 'fp64'}: {sec / args.steps * 1e3:.3f} ms/op_H", file=sys.stderr, flush=True)
+    Natoms = H.atoms.Natoms
+    dE = abs(np.sum(res["mixed"]["eband"]) - np.sum(res["fp64"]["eband"])) * 2.0 / Natoms
+    t_mix = res["mixed"]["sec"] / args.steps
+    t_64 = res["fp64"]["sec"] / args.steps
+    value = NBANDS / t_mix
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "Hpsi band-applications/s (mixed precision; FP64 beside it)",
+        "value": value,
+        "unit": "band-applications/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mix * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-fft/f32-mfma (mixed) vs f64",
+        "data": "synthetic (seeded random orthonormal psi, atomic-superposition V_loc)",
+        "config": {"workload": "Si64 sc a=20.5262, GTH LDA, ecutwfc 15 Ha, 72^3 grid, Gamma, 160 bands",
+                   "Ngw": int(H.pw.Ngw[0]), "Nbeta": int(H.pspotNL.NbetaNL), "bands_per_step": NBANDS},
+        "fp64": {"value": NBANDS / t_64, "ms_per_step": t_64 * 1e3},
+        "mixed_over_fp64": t_64 / t_mix,
+        "dE_per_atom_mixed_vs_fp64": dE,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

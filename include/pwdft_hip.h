@@ -1,0 +1,145 @@
+/*
+ * pwdft_hip.h -- C ABI of libpwdft_hip.so, the MI355X (gfx950) implementation of
+ * PWDFT.jl's Kohn-Sham Hamiltonian application, density accumulation and the dense
+ * products of its block eigensolvers.
+ *
+ * Conventions (identical to the reference's Julia data):
+ *  - complex numbers are interleaved (re, im) double pairs, i.e. Julia ComplexF64 /
+ *    C double _Complex / hipDoubleComplex;
+ *  - wave functions are column-major Ngw x Nstates with an explicit leading dimension;
+ *  - real-space grid arrays are flat column-major (x fastest), ip = i + n1*(j + n2*k);
+ *  - index maps are passed as int64 with an explicit base (1 for Julia's 1-based maps);
+ *  - k-point and spin indices are 0-based here (the Julia shim subtracts one).
+ *
+ * Every entry point returns PW_OK (0) or a negative PW_ERR_* code; the message of the
+ * last failure is available from pw_last_error().  No C++ exception crosses the ABI.
+ * One context owns one HIP stream and its device buffers; a context is not thread
+ * safe (the reference's Hamiltonian cursor Ham.ik/Ham.ispin is not re-entrant either).
+ * Entry points taking host pointers are synchronous with respect to those buffers.
+ * Entry points with the _dev suffix take device pointers (from pw_malloc) and are
+ * asynchronous on the context stream; pw_synchronize() waits for them.
+ */
+#ifndef PWDFT_HIP_H
+#define PWDFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pw_ctx pw_ctx;
+
+enum {
+  PW_OK = 0,
+  PW_ERR_ARG = -1,          /* invalid argument (ArgumentError / @assert in the reference) */
+  PW_ERR_HIP = -2,          /* HIP runtime failure */
+  PW_ERR_STATE = -3,        /* grid / k-point / potential not set up */
+  PW_ERR_NOT_POSDEF = -4,   /* reserved for Cholesky failures (PosDefException) */
+  PW_ERR_NOMEM = -5         /* device allocation failed */
+};
+
+/* op_H term mask and options (flags argument) */
+enum {
+  PW_TERM_K = 1u << 0,      /* op_K            (op_K.jl:44-75)          */
+  PW_TERM_VLOC = 1u << 1,   /* op_V_loc        (op_V_loc.jl:33-92)      */
+  PW_TERM_NLOC = 1u << 2,   /* op_V_Ps_nloc    (op_V_Ps_nloc.jl:39-78)  */
+  PW_TERM_ALL = 7u,         /* op_H            (op_H.jl:4-27)           */
+  PW_ACCUMULATE = 1u << 4,  /* add into the output (the reference's `!` forms of op_K/op_V_loc/op_V_Ps_nloc) */
+  PW_MIXED = 1u << 5        /* FP32 FFT and FP32-MFMA projector products (mixed precision) */
+};
+
+/* ---- context ------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  No reference counterpart (the reference
+ * keeps this state inside Hamiltonian, Hamiltonian.jl:3-21). */
+int pw_ctx_create(pw_ctx** ctx, int device);
+int pw_ctx_destroy(pw_ctx* ctx);
+const char* pw_last_error(pw_ctx* ctx);
+/* Bands processed per fused FFT launch (intermediates live in HBM/L2 per batch). */
+int pw_set_band_batch(pw_ctx* ctx, int nb);
+
+/* ---- set-up: the state op_H reads from Hamiltonian ------------------------------- */
+
+/* FFT grid of PWGrid (PWGrid.jl:31-47: Ns, CellVolume). */
+int pw_set_grid(pw_ctx* ctx, int n1, int n2, int n3, double cell_volume);
+
+/* Wave-function sphere of k-point ik: pw.gvecw.idx_gw2r[ik] (PWGrid.jl:314) and the
+ * G+k vectors (3 x ngw, column-major) of pw.gvec.G[:, idx_gw2g[ik]] .+ k (op_K.jl:69). */
+int pw_set_kpoint(pw_ctx* ctx, int ik, int ngw, const int64_t* idx_gw2r, int index_base,
+                  const double* gk);
+
+/* Total local potential Ham.potentials.Total[:, ispin] (Potentials.jl:7-14), N values. */
+int pw_set_vloc(pw_ctx* ctx, int ispin, const double* V);
+
+/* Nonlocal GTH data: D (nbeta x nbeta, column-major, block diagonal, D[ib,jb] = h[l,i,j]
+ * of PsPotNL prj2beta blocks, PsPotNL.jl:147-195) and betaNL[ik] (ngw x nbeta,
+ * PsPotNL.jl:16-77).  pw_set_dij must precede pw_set_projectors. */
+int pw_set_dij(pw_ctx* ctx, int nbeta, const double* D);
+int pw_set_projectors(pw_ctx* ctx, int ik, int nbeta, const void* betaNL, int64_t ld);
+
+/* ---- Hamiltonian application ----------------------------------------------------- */
+
+/* Hpsi = (terms of) H psi for nb bands at (ik, ispin): op_H (op_H.jl:4-27) with the
+ * term mask in flags; PW_ACCUMULATE adds into Hpsi as op_K!/op_V_loc!/op_V_Ps_nloc! do
+ * (op_K.jl:44-75, op_V_loc.jl:33-92, op_V_Ps_nloc.jl:39-78).  Host pointers. */
+int pw_op_H(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldpsi, void* Hpsi,
+            int64_t ldh, unsigned flags);
+/* Same on device pointers. */
+int pw_op_H_dev(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldpsi, void* Hpsi,
+                int64_t ldh, unsigned flags);
+
+/* Kinetic preconditioner Kprec (Kprec.jl:10-28): out = in / (1 + |G+k|^2), device pointers. */
+int pw_kprec_dev(pw_ctx* ctx, int ik, int nb, const void* in, int64_t ldin, void* out, int64_t ldout);
+
+/* betaNL_psi = (betaNL^H psi) as an nbeta x nb column-major complex matrix: the transpose
+ * of calc_betaNL_psi (PsPotNL.jl:242-254), device pointers. */
+int pw_betaNL_psi_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, void* out,
+                      int64_t ldout, unsigned flags);
+
+/* ---- density --------------------------------------------------------------------- */
+
+/* calc_rhoe! (calc_rhoe.jl:9-80) split into: zero the device accumulator, add the bands
+ * of one k-point with weights w[ist] = wk[ik] * Focc[ist, ik], and read rho (N doubles). */
+int pw_rhoe_zero(pw_ctx* ctx);
+int pw_rhoe_accum(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, const double* w,
+                  unsigned flags);
+int pw_rhoe_accum_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, const double* w,
+                      unsigned flags);
+int pw_rhoe_get(pw_ctx* ctx, double* rho);
+
+/* ---- dense products of diag_LOBPCG! / diag_davidson! ------------------------------ */
+
+/* C = alpha * op(A) * B + beta * C with op(A) = A (transa 0, A m x k) or A^H (transa 1,
+ * A k x m); complex, column-major, device pointers.  These are the X'HX, Q'HQ, Q'Q, Q*U,
+ * HQ*U and W*U products of diag_LOBPCG.jl:85-203 and diag_davidson.jl:122-150.
+ * flags & PW_MIXED computes in FP32 MFMA with FP32 accumulation. */
+int pw_zgemm_dev(pw_ctx* ctx, int transa, int m, int n, int k, const double* alpha, const void* A,
+                 int64_t lda, const void* B, int64_t ldb, const double* beta, void* C, int64_t ldc,
+                 unsigned flags);
+
+/* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */
+
+int pw_malloc(pw_ctx* ctx, size_t bytes, void** dptr);
+int pw_free(pw_ctx* ctx, void* dptr);
+int pw_memcpy_h2d(pw_ctx* ctx, void* dst, const void* src, size_t bytes);
+int pw_memcpy_d2h(pw_ctx* ctx, void* dst, const void* src, size_t bytes);
+int pw_memcpy_d2d(pw_ctx* ctx, void* dst, const void* src, size_t bytes);
+int pw_synchronize(pw_ctx* ctx);
+
+/* Strided 2-D copy of complex columns between device buffers: dst[:, j] = src[:, j]
+ * for j < ncols, nrows complex each. */
+int pw_copy_cols_dev(pw_ctx* ctx, int nrows, int ncols, const void* src, int64_t lds, void* dst,
+                     int64_t ldd);
+
+/* ---- timing ---------------------------------------------------------------------- */
+/* Events on the context stream (the stream every kernel of the context runs on). */
+int pw_event_record(pw_ctx* ctx, int slot);
+int pw_event_elapsed_ms(pw_ctx* ctx, int slot0, int slot1, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PWDFT_HIP_H */

@@ -1,0 +1,177 @@
+"""Oracle: eigensolvers restated from diag_LOBPCG.jl, diag_davidson.jl, ortho_sqrt.jl,
+gen_wavefunc.jl.  Test infrastructure only.
+
+The solvers take `opH(X)` and `prec(R)` callables so the same restatement runs on the
+oracle Hamiltonian (ops.op_H / ops.Kprec)."""
+import numpy as np
+import scipy.linalg as sla
+
+EPS = np.finfo(float).eps
+
+
+def herm_u(A):
+    """Julia's Hermitian(A) (uplo = :U): upper triangle mirrored, real diagonal."""
+    U = np.triu(A, 1)
+    return U + U.conj().T + np.diag(np.real(np.diag(A)))
+
+
+def ortho_sqrt(psi):
+    """ortho_sqrt.jl:1-4: psi * inv(sqrt(psi'psi)) (Loewdin)."""
+    O = herm_u(psi.conj().T @ psi)
+    w, U = np.linalg.eigh(O)
+    return psi @ (U @ np.diag(1.0 / np.sqrt(w)) @ U.conj().T)
+
+
+def rand_wavefunc(rng, Nbasis, Nstates):
+    """gen_wavefunc.jl:90-92 with the project RNG: ortho_sqrt(U[0,1) + i U[0,1))."""
+    X = rng.random((Nbasis, Nstates)) + 1j * rng.random((Nbasis, Nstates))
+    return ortho_sqrt(X)
+
+
+def _rdiv_chol(W, C):
+    """W / cholesky(Hermitian(C)).U  (diag_LOBPCG.jl:172-174) -> (W U^{-1}, U)."""
+    U = sla.cholesky(herm_u(C), lower=False)
+    return sla.solve_triangular(U, W.T, trans="T", lower=False).T, U
+
+
+def _rdiv_U(W, U):
+    return sla.solve_triangular(U, W.T, trans="T", lower=False).T
+
+
+def diag_LOBPCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
+    """diag_LOBPCG.jl:51-225 for one k-point.  X (Nbasis x Nstates, orthonormal) is
+    overwritten in place; returns (evals, iterations, number of opH columns applied)."""
+    Nbasis, Nstates = X.shape
+    if Nstates_conv == 0:
+        Nstates_conv = Nstates
+    ncols = Nstates
+    HX = opH(X)
+    P = np.zeros_like(X)
+    HP = np.zeros_like(X)
+    evals_old = np.zeros(Nstates)
+    SMALL = 10.0 * EPS
+    it_done = 0
+    for it in range(1, NiterMax + 1):
+        it_done = it
+        XHX = herm_u(X.conj().T @ HX)
+        evals = np.linalg.eigvalsh(XHX)
+        devals = np.abs(evals - evals_old)
+        evals_old = evals.copy()
+        R = HX - X @ XHX
+        nconv = int(np.count_nonzero(devals < tol))
+        ilock = np.nonzero(devals <= SMALL)[0]
+        iact = np.nonzero(devals > SMALL)[0]
+        Nlock, Nact = len(ilock), len(iact)
+        if nconv >= Nstates_conv:
+            break
+        W = prec(R)
+        if Nlock > 0:
+            # lock branch, diag_LOBPCG.jl:125-164 (the `ww = ww - ...` update at :130-133
+            # rebinds a local in the reference and changes nothing; it is omitted)
+            Xlock = X[:, ilock].copy()
+            X[:, iact] = X[:, iact] - Xlock @ (Xlock.conj().T @ X[:, iact])
+            W[:, iact] = W[:, iact] - Xlock @ (Xlock.conj().T @ W[:, iact])
+            HX[:, iact] = opH(X[:, iact])
+            HW = opH(W[:, iact])
+            ncols += 2 * Nact
+            HX[:, iact] = HX[:, iact] - Xlock @ (Xlock.conj().T @ HX[:, iact])
+            HW = HW - Xlock @ (Xlock.conj().T @ HW)
+            Wa, U = _rdiv_chol(W[:, iact], W[:, iact].conj().T @ W[:, iact])
+            W[:, iact] = Wa
+            HW = _rdiv_U(HW, U)
+            Q = np.hstack([X[:, iact], W[:, iact]])
+            HQ = np.hstack([HX[:, iact], HW])
+            T = herm_u(Q.conj().T @ HQ)
+            G = herm_u(Q.conj().T @ Q)
+            sd, S = sla.eigh(T, G, lower=False)
+            Xact = Q @ S[:, :Nact]
+            X[:, :] = np.hstack([Xlock, Xact])
+            T = herm_u(X.conj().T @ opH(X))
+            ncols += Nstates
+            evT, evecT = np.linalg.eigh(T)
+            X[:, :] = X @ evecT
+            HX = opH(X)
+            ncols += Nstates
+        else:
+            HW = opH(W)
+            ncols += Nstates
+            W, U = _rdiv_chol(W, W.conj().T @ W)
+            HW = _rdiv_U(HW, U)
+            if it > 1:
+                Q = np.hstack([X, W, P])
+                HQ = np.hstack([HX, HW, HP])
+            else:
+                Q = np.hstack([X, W])
+                HQ = np.hstack([HX, HW])
+            T = herm_u(Q.conj().T @ HQ)
+            G = herm_u(Q.conj().T @ Q)
+            sd, S = sla.eigh(T, G, lower=False)
+            Um = S[:, :Nstates]
+            X[:, :] = Q @ Um
+            HX = HQ @ Um
+            if it > 1:
+                s2 = slice(Nstates, 2 * Nstates)
+                s3 = slice(2 * Nstates, 3 * Nstates)
+                P = W @ Um[s2, :] + P @ Um[s3, :]
+                HP = HW @ Um[s2, :] + HP @ Um[s3, :]
+                P, U = _rdiv_chol(P, P.conj().T @ P)
+                HP = _rdiv_U(HP, U)
+            else:
+                P = W.copy()
+                HP = HW.copy()
+    S = X.conj().T @ HX
+    S = herm_u(0.5 * (S + S.conj().T))
+    evals, Qm = np.linalg.eigh(S)
+    X[:, :] = X @ Qm
+    return evals, it_done, ncols
+
+
+def diag_davidson(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
+    """diag_davidson.jl:50-191 for one k-point (block Davidson on [X R]).  X is
+    overwritten in place; returns (evals, iterations, number of opH columns applied)."""
+    Nbasis, Nstates = X.shape
+    if Nstates_conv == 0:
+        Nstates_conv = Nstates
+    HX = opH(X)
+    ncols = Nstates
+    evals = np.real(np.einsum("gi,gi->i", X.conj(), HX))
+    R = evals[None, :] * X - HX
+    res = np.sqrt(np.real(np.einsum("gi,gi->i", R.conj(), R)))
+    s1 = slice(0, Nstates)
+    s2 = slice(Nstates, 2 * Nstates)
+    evals_old = evals.copy()
+    Hred = np.zeros((2 * Nstates, 2 * Nstates), dtype=complex)
+    Sred = np.zeros((2 * Nstates, 2 * Nstates), dtype=complex)
+    it_done = 0
+    for it in range(1, NiterMax + 1):
+        it_done = it
+        res_norm = np.where(EPS < res, 1.0 / np.where(res > 0, res, 1.0), 1.0)
+        R = R * res_norm[None, :]
+        R = prec(R)
+        HR = opH(R)
+        ncols += Nstates
+        if it == 1:
+            Hred[s1, s1] = X.conj().T @ HX
+        else:
+            Hred[s1, s1] = np.diag(evals.astype(complex))
+        Hred[s1, s2] = X.conj().T @ HR
+        Hred[s2, s2] = R.conj().T @ HR
+        Hred[s2, s1] = Hred[s1, s2].conj().T
+        Sred[s1, s1] = np.eye(Nstates)
+        Sred[s1, s2] = X.conj().T @ R
+        Sred[s2, s2] = R.conj().T @ R
+        Sred[s2, s1] = Sred[s1, s2].conj().T
+        Hred = 0.5 * (Hred + Hred.conj().T)
+        Sred = 0.5 * (Sred + Sred.conj().T)
+        lam, Xr = sla.eigh(herm_u(Hred), herm_u(Sred), lower=False)
+        evals = lam[:Nstates].copy()
+        devals = np.abs(evals - evals_old)
+        nconv = int(np.count_nonzero(devals < tol))
+        X[:, :] = X @ Xr[s1, s1] + R @ Xr[s2, s1]
+        HX = HX @ Xr[s1, s1] + HR @ Xr[s2, s1]
+        R = evals[None, :] * X - HX
+        res = np.sqrt(np.real(np.einsum("gi,gi->i", R.conj(), R)))
+        if nconv >= Nstates_conv:
+            break
+        evals_old = evals.copy()
+    return evals, it_done, ncols

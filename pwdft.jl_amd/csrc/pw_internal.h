@@ -1,0 +1,127 @@
+// Internal definitions shared by the HIP translation units of libpwdft_hip.
+// Target: gfx950 (MI355X) only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pwdft_hip.h"
+
+namespace pw {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__host__ __device__ inline cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__host__ __device__ inline cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__host__ __device__ inline cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+__host__ __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> scal(T s, cplx<T> a) { return mk(s * a.x, s * a.y); }
+// multiply by +i (s > 0) or -i (s < 0)
+template <int S, typename T>
+__host__ __device__ inline cplx<T> mul_i(cplx<T> a) {
+  return S > 0 ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+template <typename TO, typename TI>
+__host__ __device__ inline cplx<TO> cvt(cplx<TI> a) { return mk((TO)a.x, (TO)a.y); }
+
+// One FFT dimension: length n = prod(radix[0..npass)).
+struct FftDim {
+  int n;
+  int npass;
+  int radix[8];
+};
+
+// Sphere bookkeeping of one k-point.  Sphere entries are sorted by (column, z);
+// a column is one (x, y) pair that holds at least one entry.
+struct KData {
+  int ngw = 0;
+  int ncol = 0;
+  int nxact = 0;
+  int nbeta = 0;
+  int* d_ent_igw = nullptr;   // [ngw]   row of psi for each entry
+  int* d_ent_pos = nullptr;   // [ngw]   column * n3 + z
+  int* d_col_ptr = nullptr;   // [ncol+1]
+  int* d_col_ij = nullptr;    // [ncol]  i + n1 * j (offset inside a z-plane)
+  int* d_xact = nullptr;      // [nxact] distinct x of the columns
+  double* d_gk2 = nullptr;    // [ngw]   |G+k|^2
+  double* d_gk = nullptr;     // [3*ngw] G+k (x,y,z per entry, column-major 3 x ngw)
+  void* d_beta64 = nullptr;   // [ngw * nbeta] complex double, column-major
+  void* d_betaD64 = nullptr;  // beta * D
+  void* d_beta32 = nullptr;   // complex float copies for the mixed path
+  void* d_betaD32 = nullptr;
+  std::vector<int> h_xact;
+};
+
+}  // namespace pw
+
+struct pw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n1 = 0, n2 = 0, n3 = 0;
+  long N = 0;
+  double omega = 0.0;
+  pw::FftDim dim[3];
+  // twiddle tables exp(+2 pi i t / n) per dimension, double and float
+  void* d_tw64[3] = {nullptr, nullptr, nullptr};
+  void* d_tw32[3] = {nullptr, nullptr, nullptr};
+  std::vector<pw::KData> kd;
+  int nspin = 0;
+  std::vector<double*> d_v64;  // V_loc / N per spin
+  std::vector<float*> d_v32;
+  int nbeta = 0;
+  std::vector<double> h_D;     // nbeta x nbeta column-major
+  // workspaces (grow on demand)
+  void* d_work = nullptr;  size_t work_bytes = 0;     // FFT column/plane intermediates
+  void* d_gpart = nullptr; size_t gpart_bytes = 0;    // GEMM split-K partials
+  void* d_bnl = nullptr;   size_t bnl_bytes = 0;      // beta^H psi
+  void* d_rpart = nullptr; size_t rpart_bytes = 0;    // rho partial planes
+  double* d_rho = nullptr;                            // accumulated density [N]
+  void* d_tmp = nullptr;   size_t tmp_bytes = 0;      // staging for host-pointer entry points
+  void* d_wts = nullptr;   size_t wts_bytes = 0;      // band weights of calc_rhoe
+  int band_batch = 64;
+  std::string err;
+};
+
+namespace pw {
+
+int set_error(pw_ctx* c, int code, const std::string& msg);
+int hip_check(pw_ctx* c, hipError_t e, const char* what);
+int ensure(pw_ctx* c, void** buf, size_t* have, size_t need);
+
+// FFT drivers (pw_fft.hip)
+int vloc_apply(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
+               cplx<double>* hpsi, long ldh, unsigned flags);
+int kinetic_apply(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,
+                  long ldh, unsigned flags);
+int rhoe_accum(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, const double* d_w,
+               unsigned flags);
+int kprec_apply(pw_ctx* c, int ik, int nb, const cplx<double>* in, long ldin, cplx<double>* out,
+                long ldout);
+
+// GEMM drivers (pw_gemm.hip)
+// C (M x N) = alpha * op(A) * B + beta * C ; op(A) = A (transa = 0, A is M x K) or
+// A^H (transa = 1, A is K x M).  All pointers are device pointers.
+int zgemm(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<double>* A,
+          long lda, const cplx<double>* B, long ldb, cplx<double> beta, cplx<double>* C, long ldc,
+          int prec);
+int zgemm_f32in(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<float>* A,
+                long lda, const cplx<double>* B, long ldb, cplx<double> beta, cplx<double>* C,
+                long ldc, int prec);
+
+}  // namespace pw

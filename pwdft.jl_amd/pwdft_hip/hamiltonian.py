@@ -1,0 +1,272 @@
+"""Hamiltonian state and the operator API of PWDFT.jl, executed by libpwdft_hip.
+
+Mirrors the reference names (Julia `f!` -> Python `f_`):
+  op_H(Ham, psi)            op_H.jl:4-27          op_H_(Ham, psi, Hpsi)   op_H.jl:53-63
+  op_K(Ham, psi)            op_K.jl:44-75         op_K_(Ham, psi, Hpsi)   (accumulates)
+  op_V_loc(Ham, psi)        op_V_loc.jl:173-184   op_V_loc_(...)          op_V_loc.jl:33-92
+  op_V_Ps_nloc(Ham, psi)    op_V_Ps_nloc.jl:26-30 op_V_Ps_nloc_(...)      op_V_Ps_nloc.jl:39-78
+  calc_rhoe(Ham, psiks)     calc_rhoe.jl:1-7      calc_rhoe_(Ham, psiks, Rhoe)
+  Kprec(Ham, psi)           Kprec.jl:10-28
+psi is a numpy (Ngw x Nst) complex128 array (host entry points) or a DeviceMatrix
+(device entry points); a list of them is a BlochWavefunc indexed ik + ispin*Nkpt.
+Ham.ik / Ham.ispin are the reference's mutable cursor (Hamiltonian.jl:16-17).
+
+op_H_ skips op_Vexx when there is no EXX object: the reference's op_H! calls op_Vexx!
+unconditionally and fails for every Hamiltonian it builds (op_H.jl:61, Hamiltonian.jl:293).
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC,
+                   Context, DeviceMatrix, ptr)
+from .basis import PsPot_GTH, PsPotNL, PWGrid, V_Ps_loc, kpoint_grid
+from .potentials import calc_E_NN, epsxc_vxc_vwn, guess_rhoe, poisson_solve
+
+
+class Electrons:
+    """Electrons.jl:121-270 for Nspin_wf = 1: Nstates, Nstates_occ and initial Focc."""
+
+    def __init__(self, Nelectrons, Nkpt, Nstates=None, extra_states=None, use_smearing=False):
+        Ne = float(Nelectrons)
+        nmin = int(round(Ne / 2))
+        if nmin * 2 < Ne:
+            nmin += 1
+        if Nstates is None and use_smearing and extra_states is None:
+            Nstates = max(math.ceil(1.2 * Ne / 2), math.ceil(Ne / 2) + 4)
+            nempty = Nstates - math.ceil(Ne / 2)
+        elif Nstates is None:
+            nempty = extra_states or 0
+            Nstates = nmin + nempty
+        else:
+            if Nstates < nmin:
+                raise ValueError(f"Given Nstates is not enough: Nstates = {Nstates}, NstatesMin = {nmin}")
+            nempty = Nstates - nmin
+        self.Nelectrons = Ne
+        self.Nstates = int(Nstates)
+        self.Nstates_occ = int(Nstates - nempty)
+        self.Focc = np.zeros((self.Nstates, Nkpt))
+        self.Focc[: self.Nstates_occ - 1] = 2.0
+        self.Focc[self.Nstates_occ - 1] = 1.0 if int(round(Ne)) % 2 == 1 else 2.0
+        self.ebands = np.zeros((self.Nstates, Nkpt))
+        self.E_fermi = 0.0
+        self.kT = 0.0
+        self.use_smearing = use_smearing
+
+
+class Potentials:
+    """Potentials.jl:7-14 (Nspin = 1)."""
+
+    def __init__(self, Npoints):
+        self.Ps_loc = np.zeros(Npoints)
+        self.Hartree = np.zeros(Npoints)
+        self.XC = np.zeros((Npoints, 1))
+        self.Total = np.zeros((Npoints, 1))
+
+
+class Hamiltonian:
+    """Hamiltonian(atoms, pspfiles, ecutwfc; kw...) of Hamiltonian.jl:337-405 (GTH, dual 4,
+    Nspin 1).  Builds the host set-up data and uploads what op_H reads to the device."""
+
+    def __init__(self, atoms, pspfiles, ecutwfc, meshk=(1, 1, 1), shiftk=(0, 0, 0), Ns_=None,
+                 kpoints=None, rots=None, extra_states=None, Nstates=None, use_smearing=False,
+                 smearing_kT=0.0, xcfunc="VWN", device=0, band_batch=64, mixed=False):
+        if len(pspfiles) != atoms.Nspecies:
+            raise ValueError(f"Length of pspfiles is not equal to Nspecies={atoms.Nspecies}")
+        self.pspots = [p if isinstance(p, PsPot_GTH) else PsPot_GTH(p) for p in pspfiles]
+        for isp, psp in enumerate(self.pspots):
+            if psp.atsymb.lower() != atoms.SpeciesSymbols[isp].lower():
+                raise ValueError("Atomic symbols and pspot atsymb does not match")
+        if xcfunc not in ("VWN", "LDA"):
+            raise NotImplementedError("only the LDA (Slater + VWN) functional is implemented")
+        self.atoms = atoms
+        self.xcfunc = xcfunc
+        if kpoints is None:
+            kpoints = kpoint_grid(atoms.LatVecs, meshk, shiftk, rots=rots)
+        self.pw = PWGrid(ecutwfc, atoms.LatVecs, kpoints=kpoints, Ns=Ns_)
+        self.pspotNL = PsPotNL(atoms, self.pw, self.pspots)
+        self.Zvals = np.array([p.zval for p in self.pspots])
+        Ne = float(np.sum(self.Zvals[atoms.atm2species]))
+        self.electrons = Electrons(Ne, self.pw.Nkpt, Nstates=Nstates, extra_states=extra_states,
+                                   use_smearing=use_smearing)
+        self.electrons.kT = smearing_kT
+        self.potentials = Potentials(self.pw.Npoints)
+        self.potentials.Ps_loc = V_Ps_loc(atoms, self.pw, self.pspots)
+        self.potentials.Total[:, 0] = self.potentials.Ps_loc
+        self.E_NN = calc_E_NN(atoms, self.Zvals)
+        self.ik = 0
+        self.ispin = 0
+        self.mixed = mixed
+        self.ctx = Context(device)
+        self.ctx.call("pw_set_band_batch", int(band_batch))
+        self._upload_static()
+        self._upload_vloc()
+
+    # ---- device state ---------------------------------------------------------------
+    def _upload_static(self):
+        pw, c = self.pw, self.ctx
+        n1, n2, n3 = pw.Ns
+        c.call("pw_set_grid", n1, n2, n3, float(pw.CellVolume))
+        nb = self.pspotNL.NbetaNL
+        D = np.asfortranarray(self.pspotNL.D, dtype=float)
+        c.call("pw_set_dij", nb, ptr(D))
+        for ik in range(pw.Nkpt):
+            idx = np.ascontiguousarray(pw.idx_gw2r[ik], dtype=np.int64)
+            gk = np.ascontiguousarray(pw.gk(ik).T, dtype=float)  # ngw x 3 row-major == 3 x ngw col-major
+            c.call("pw_set_kpoint", ik, int(pw.Ngw[ik]), ptr(idx), 0, ptr(gk))
+            if nb > 0:
+                b = np.asfortranarray(self.pspotNL.betaNL[ik])
+                c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
+
+    def _upload_vloc(self):
+        V = np.ascontiguousarray(self.potentials.Total[:, 0])
+        self.ctx.call("pw_set_vloc", 0, ptr(V))
+
+    def set_Vloc(self, V):
+        """Replace Ham.potentials.Total[:, 1] and upload it."""
+        self.potentials.Total[:, 0] = V
+        self._upload_vloc()
+
+    def update(self, rho):
+        """update!(Ham, psiks, Rhoe) for Nspin = 1 (Hamiltonian.jl:510-545)."""
+        rho = np.asarray(rho, dtype=float).reshape(-1)
+        self.potentials.Hartree = poisson_solve(self.pw, rho)
+        _, vxc = epsxc_vxc_vwn(rho)
+        self.potentials.XC[:, 0] = vxc
+        self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + vxc
+        self._upload_vloc()
+
+    def guess_rhoe(self):
+        return guess_rhoe(self.atoms, self.pw, self.pspots, self.electrons.Nelectrons)
+
+    @property
+    def flags_prec(self):
+        return PW_MIXED if self.mixed else 0
+
+
+# ---- operator API -------------------------------------------------------------------
+
+def _apply(Ham, psi, terms, out=None, accumulate=False):
+    flags = terms | Ham.flags_prec | (PW_ACCUMULATE if accumulate else 0)
+    ik, ispin = Ham.ik, Ham.ispin
+    ngw = Ham.pw.Ngw[ik]
+    if isinstance(psi, DeviceMatrix):
+        if psi.rows != ngw:
+            raise ValueError("psi rows differ from Ngw[ik]")
+        if out is None:
+            out = DeviceMatrix(Ham.ctx, psi.rows, psi.cols)
+        Ham.ctx.call("pw_op_H_dev", ik, ispin, psi.cols, psi.p, psi.rows, out.p, out.rows, flags)
+        return out
+    a = np.asfortranarray(psi, dtype=np.complex128)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1, order="F")
+    if a.shape[0] != ngw:
+        raise ValueError(f"psi has {a.shape[0]} rows, Ngw[{ik}] = {ngw}")
+    if out is None:
+        out = np.zeros_like(a)
+    elif not (out.flags.f_contiguous and out.dtype == np.complex128):
+        raise ValueError("Hpsi must be a Fortran-ordered complex128 array")
+    Ham.ctx.call("pw_op_H", ik, ispin, a.shape[1], ptr(a), ngw, ptr(out), ngw, flags)
+    return out
+
+
+def _per_k(Ham, psiks, fn):
+    out = []
+    for ispin in range(1):
+        for ik in range(Ham.pw.Nkpt):
+            Ham.ik, Ham.ispin = ik, ispin
+            out.append(fn(psiks[ik + ispin * Ham.pw.Nkpt]))
+    return out
+
+
+def op_H(Ham, psi):
+    """op_H.jl:4-27 (allocating): K + V_loc + V_Ps_nloc (+ op_Vtau = 0 for LDA)."""
+    if isinstance(psi, list):
+        return _per_k(Ham, psi, lambda p: op_H(Ham, p))
+    return _apply(Ham, psi, PW_TERM_ALL)
+
+
+def op_H_(Ham, psi, Hpsi):
+    """op_H! (op_H.jl:53-63): Hpsi is overwritten with H psi."""
+    _apply(Ham, psi, PW_TERM_ALL, out=Hpsi, accumulate=False)
+
+
+def op_K(Ham, psi):
+    if isinstance(psi, list):
+        return _per_k(Ham, psi, lambda p: op_K(Ham, p))
+    return _apply(Ham, psi, PW_TERM_K)
+
+
+def op_K_(Ham, psi, Hpsi):
+    """op_K! accumulates into Hpsi (op_K.jl:55-75)."""
+    _apply(Ham, psi, PW_TERM_K, out=Hpsi, accumulate=True)
+
+
+def op_V_loc(Ham, psi):
+    if isinstance(psi, list):
+        return _per_k(Ham, psi, lambda p: op_V_loc(Ham, p))
+    return _apply(Ham, psi, PW_TERM_VLOC)
+
+
+def op_V_loc_(Ham, psi, Hpsi):
+    """op_V_loc! accumulates into Hpsi (op_V_loc.jl:33-92)."""
+    _apply(Ham, psi, PW_TERM_VLOC, out=Hpsi, accumulate=True)
+
+
+def op_V_Ps_nloc(Ham, psi):
+    if isinstance(psi, list):
+        return _per_k(Ham, psi, lambda p: op_V_Ps_nloc(Ham, p))
+    return _apply(Ham, psi, PW_TERM_NLOC)
+
+
+def op_V_Ps_nloc_(Ham, psi, Hpsi):
+    """op_V_Ps_nloc! accumulates into Hpsi (op_V_Ps_nloc.jl:39-78)."""
+    _apply(Ham, psi, PW_TERM_NLOC, out=Hpsi, accumulate=True)
+
+
+def Kprec(Ham, psi, out=None):
+    """Kprec.jl:10-28: psi / (1 + |G+k|^2) (device)."""
+    if out is None:
+        out = DeviceMatrix(Ham.ctx, psi.rows, psi.cols)
+    Ham.ctx.call("pw_kprec_dev", Ham.ik, psi.cols, psi.p, psi.rows, out.p, out.rows)
+    return out
+
+
+def calc_betaNL_psi(Ham, psi):
+    """PsPotNL.jl:242-254: conj(psi' * betaNL[ik]) -> Nstates x Nbeta (host array)."""
+    nb = Ham.pspotNL.NbetaNL
+    dev = psi if isinstance(psi, DeviceMatrix) else DeviceMatrix.from_host(Ham.ctx, psi)
+    out = DeviceMatrix(Ham.ctx, nb, dev.cols)
+    Ham.ctx.call("pw_betaNL_psi_dev", Ham.ik, dev.cols, dev.p, dev.rows, out.p, nb, Ham.flags_prec)
+    return out.download().T.copy()
+
+
+def calc_rhoe_(Ham, psiks, Rhoe, Focc=None):
+    """calc_rhoe! (calc_rhoe.jl:9-80, Nspin = 1, no symmetrisation): Rhoe[:, 0] is
+    overwritten with sum_k wk sum_ist Focc |psi(r)|^2 (psi(r) = N/sqrt(Omega) IFFT psi)."""
+    c = Ham.ctx
+    Focc = Ham.electrons.Focc if Focc is None else Focc
+    c.call("pw_rhoe_zero")
+    for ik in range(Ham.pw.Nkpt):
+        psi = psiks[ik]
+        w = np.ascontiguousarray(Ham.pw.wk[ik] * Focc[:, ik], dtype=float)
+        if isinstance(psi, DeviceMatrix):
+            c.call("pw_rhoe_accum_dev", ik, psi.cols, psi.p, psi.rows, ptr(w), Ham.flags_prec)
+        else:
+            a = np.asfortranarray(psi, dtype=np.complex128)
+            c.call("pw_rhoe_accum", ik, a.shape[1], ptr(a), a.shape[0], ptr(w), Ham.flags_prec)
+    r = np.empty(Ham.pw.Npoints)
+    c.call("pw_rhoe_get", ptr(r))
+    Rhoe = np.asarray(Rhoe)
+    if Rhoe.ndim == 2:
+        Rhoe[:, 0] = r
+    else:
+        Rhoe[:] = r
+    return Rhoe
+
+
+def calc_rhoe(Ham, psiks, Focc=None):
+    """calc_rhoe.jl:1-7 (allocating): Npoints x 1 array."""
+    Rhoe = np.zeros((Ham.pw.Npoints, 1))
+    return calc_rhoe_(Ham, psiks, Rhoe, Focc=Focc)

@@ -1,0 +1,115 @@
+"""Potential update and ion-ion terms used around the hot path: guess_rhoe
+(guess_rhoe.jl:1-38), Poisson_solve! (Poisson_solve.jl:22-39), Slater + VWN LDA exchange-
+correlation (XC_funcs/XC_x_slater.jl, XC_funcs/XC_c_vwn.jl), Ewald E_NN (calc_E_NN.jl:60-172).
+Host-side, once per SCF iteration (O(N_grid)), in FP64."""
+import math
+
+import numpy as np
+from scipy.special import erfc
+
+from .basis import ZATOMS, strfact
+
+
+def _ifft(pw, f):
+    return np.fft.ifftn(np.asarray(f).reshape(pw.Ns, order="F")).reshape(-1, order="F")
+
+
+def _fft(pw, f):
+    return np.fft.fftn(np.asarray(f, dtype=complex).reshape(pw.Ns, order="F")).reshape(-1, order="F")
+
+
+_ATMLEN = [  # guess_rhoe.jl:230-330: (core-electron limit, table by nval, default)
+    (0.5, [0.6, 0.4, 0.3, 0.25], 0.2),
+    (2.5, [1.8, 1.4, 1.0, 0.7, 0.6, 0.5, 0.4, 0.35], 0.3),
+    (10.5, [2.0, 1.6, 1.25, 1.1, 1.0, 0.9, 0.8, 0.7, 0.7, 0.7], 0.6),
+    (12.5, [1.9, 1.5, 1.15, 1.0, 0.9, 0.8, 0.7, 0.6, 0.6, 0.6], 0.5),
+    (18.5, [2.0, 1.8, 1.5, 1.2, 1.0, 0.9, 0.85, 0.8, 0.75, 0.7, 0.65, 0.65], 0.6),
+    (28.5, [1.5, 1.25, 1.15, 1.05, 1.00, 0.95, 0.95, 0.9, 0.9, 0.85, 0.85, 0.80, 0.8, 0.75], 0.7),
+    (36.5, [2.0, 2.00, 1.60, 1.40, 1.25, 1.10, 1.00, 0.95, 0.90, 0.85, 0.80, 0.75], 0.7),
+    (1e9, [2.0, 2.00, 1.55, 1.25, 1.15, 1.10, 1.05, 1.0, 0.95, 0.9, 0.85, 0.85], 0.8),
+]
+
+
+def atmlength(zion, znucl):
+    nval = int(round(zion))
+    if nval == 0:
+        return 0.0
+    for lim, tab, dflt in _ATMLEN:
+        if znucl - zion < lim:
+            return tab[nval - 1] if nval <= len(tab) else dflt
+
+
+def guess_rhoe(atoms, pw, pspots, Nelectrons):
+    """guess_rhoe.jl:1-38 (species structure factor added once per atom, as there)."""
+    Sf = strfact(atoms.positions, atoms.atm2species, atoms.Nspecies, pw.G)
+    rG = np.zeros(pw.Npoints, dtype=complex)
+    for ia in range(atoms.Natoms):
+        isp = atoms.atm2species[ia]
+        l = atmlength(pspots[isp].zval, ZATOMS[atoms.SpeciesSymbols[isp]])
+        rG[pw.idx_g2r] += pspots[isp].zval * Sf[:, isp] * np.exp(-pw.G2 * l ** 2)
+    rho = np.real(_ifft(pw, rG))
+    return rho * Nelectrons / (rho.sum() * pw.CellVolume / pw.Npoints)
+
+
+def poisson_solve(pw, rho):
+    """Poisson_solve.jl:22-39: 4 pi rho(G)/G^2 on the rho sphere, G = 0 zeroed; grid
+    entries outside the sphere are left unscaled, as the reference does."""
+    c = _fft(pw, rho)
+    c[0] = 0.0
+    c[pw.idx_g2r[1:]] *= 4 * np.pi / pw.G2[1:]
+    return np.real(_ifft(pw, c))
+
+
+def epsxc_vxc_vwn(rho):
+    """Slater exchange + VWN5 correlation (spin-unpolarised); zero below rho = 1e-10,
+    the density threshold the reference gives Libxc (LDA_VWN.jl:175-186)."""
+    rho = np.asarray(rho, dtype=float)
+    ok = rho > 1e-10
+    rs = 0.6203504908994 / np.where(ok, rho, 1.0) ** (1.0 / 3.0)
+    fa = -0.687247939924714 * 2.0 / 3.0
+    ex, vx = fa / rs, 4.0 / 3.0 * fa / rs
+    a, b, c, x0 = 0.0310907, 3.72744, 12.9352, -0.10498
+    q = math.sqrt(4.0 * c - b * b)
+    f1, f2, f3 = 2.0 * b / q, b * x0 / (x0 * x0 + b * x0 + c), 2.0 * (2.0 * x0 + b) / q
+    r12 = np.sqrt(rs)
+    fx = rs + b * r12 + c
+    qx = np.arctan(q / (2.0 * r12 + b))
+    ec = a * (np.log(rs / fx) + f1 * qx - f2 * (np.log((r12 - x0) ** 2 / fx) + f3 * qx))
+    tx = 2.0 * r12 + b
+    tt = tx * tx + q * q
+    vc = ec - r12 * a / 6.0 * (2.0 / r12 - tx / fx - 4.0 * b / tt - f2 * (2.0 / (r12 - x0) - tx / fx - 4.0 * (2.0 * x0 + b) / tt))
+    return np.where(ok, ex + ec, 0.0), np.where(ok, vx + vc, 0.0)
+
+
+def calc_E_NN(atoms, Zvals):
+    """calc_E_NN.jl:60-172 (Ewald, eta from gcut = 2 and ebsl = 1e-8)."""
+    L = atoms.LatVecs
+    Om = abs(np.linalg.det(L))
+    Rec = 2 * np.pi * np.linalg.inv(L.T)
+    gexp = -math.log(1e-8)
+    eta = math.sqrt(4.0 / gexp) / 2
+    Z = np.asarray(Zvals, dtype=float)[atoms.atm2species]
+    tau = atoms.positions
+    E = -2 * eta * np.sum(Z ** 2) / math.sqrt(np.pi) - np.pi * np.sum(Z) ** 2 / (Om * eta ** 2)
+
+    def lattice_sum(vecs, cut):
+        mm = [int(round(cut / np.linalg.norm(vecs[:, i]) + 1.5)) for i in range(3)]
+        I = np.stack(np.meshgrid(*[np.arange(-m, m + 1) for m in mm], indexing="ij")).reshape(3, -1)
+        return vecs @ I, np.abs(I).sum(axis=0) != 0
+
+    T, nzT = lattice_sum(L, math.sqrt(0.5 * gexp) / eta)
+    for ia in range(atoms.Natoms):
+        for ja in range(atoms.Natoms):
+            d = (tau[:, ia] - tau[:, ja])[:, None] - T
+            r = np.sqrt((d ** 2).sum(axis=0))
+            use = nzT if ia == ja else np.ones_like(nzT)
+            E += Z[ia] * Z[ja] * np.sum(erfc(eta * r[use]) / r[use])
+    Gv, nzG = lattice_sum(Rec, 2.0)
+    Gv = Gv[:, nzG]
+    G2 = (Gv ** 2).sum(axis=0)
+    x = 4 * np.pi / Om * np.exp(-0.25 * G2 / eta ** 2) / G2
+    for ia in range(atoms.Natoms):
+        for ja in range(atoms.Natoms):
+            d = tau[:, ia] - tau[:, ja]
+            E += Z[ia] * Z[ja] * np.sum(x * np.cos(d @ Gv))
+    return 0.5 * E

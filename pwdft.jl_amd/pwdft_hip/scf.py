@@ -1,0 +1,193 @@
+"""Energies, occupations and the SCF driver: calc_energies (calc_energies.jl:457-492),
+Energies sum (Energies.jl:30-34), calc_Focc / find_E_fermi / calc_entropy
+(occupations.jl:6-245) and KS_solve_SCF! (KS_solve_SCF.jl:24-373, mix_method "simple").
+
+The wave functions stay on the device for the whole SCF: LOBPCG/Davidson, calc_rhoe and
+the psi-dependent energy terms run through libpwdft_hip; the O(N_grid) potential update,
+density mixing and the FP64 energy sums of grid quantities run on the host."""
+import time
+
+import numpy as np
+
+from ._lib import DeviceMatrix, PW_TERM_K, ptr
+from .hamiltonian import calc_betaNL_psi, calc_rhoe_
+from .potentials import epsxc_vxc_vwn
+from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc, _Dense
+
+EPS = np.finfo(float).eps
+
+
+class Energies:
+    """Energies.jl:6-34."""
+
+    def __init__(self):
+        self.Kinetic = self.Ps_loc = self.Ps_nloc = self.Hartree = self.XC = self.NN = self.mTS = 0.0
+
+    def total(self):
+        return self.Kinetic + self.Ps_loc + self.Ps_nloc + self.Hartree + self.XC + self.NN + self.mTS
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k in ("Kinetic", "Ps_loc", "Ps_nloc", "Hartree", "XC", "NN", "mTS")}
+        d["Total"] = self.total()
+        return d
+
+
+def calc_E_kin(Ham, psiks, Focc):
+    """calc_energies.jl:342-375: 0.5 sum_k wk sum_ist Focc <psi|G+k|^2|psi>, with the
+    kinetic image K psi and the diagonal of psi' (K psi) formed on the device."""
+    E = 0.0
+    D = _Dense(Ham, False)
+    for ik, psi in enumerate(psiks):
+        Ham.ik = ik
+        Kp = DeviceMatrix(Ham.ctx, psi.rows, psi.cols)
+        Ham.ctx.call("pw_op_H_dev", ik, 0, psi.cols, psi.p, psi.rows, Kp.p, Kp.rows, PW_TERM_K)
+        d = np.real(np.diag(D.gram(psi.p, Kp.p, psi.cols, psi.cols, psi.rows)))
+        E += Ham.pw.wk[ik] * np.sum(Focc[:, ik] * d)
+        Kp.free()
+    return E  # (the 0.5 is inside op_K)
+
+
+def calc_E_Ps_nloc(Ham, psiks, Focc):
+    """calc_energies.jl:7-58 (GTH): sum wk Focc h_ij Re(conj(<b_i|psi>) <b_j|psi>)."""
+    nl = Ham.pspotNL
+    if nl.NbetaNL == 0:
+        return 0.0
+    E = 0.0
+    for ik, psi in enumerate(psiks):
+        Ham.ik = ik
+        bp = calc_betaNL_psi(Ham, psi)          # Nst x Nbeta
+        enl = np.real(np.einsum("si,ij,sj->s", bp.conj(), nl.D, bp))
+        E += Ham.pw.wk[ik] * np.sum(Focc[:, ik] * enl)
+    return E
+
+
+def calc_E_local(Ham, rho):
+    """calc_energies.jl:245-333."""
+    pw = Ham.pw
+    dV = pw.CellVolume / pw.Npoints
+    epsxc, _ = epsxc_vxc_vwn(rho)
+    return (np.dot(Ham.potentials.Ps_loc, rho) * dV, 0.5 * np.dot(Ham.potentials.Hartree, rho) * dV,
+            np.sum(epsxc * rho) * dV)
+
+
+def calc_energies(Ham, psiks, rho, Focc=None, mTS=0.0):
+    Focc = Ham.electrons.Focc if Focc is None else Focc
+    e = Energies()
+    e.Kinetic = calc_E_kin(Ham, psiks, Focc)
+    e.Ps_nloc = calc_E_Ps_nloc(Ham, psiks, Focc)
+    e.Ps_loc, e.Hartree, e.XC = calc_E_local(Ham, rho)
+    e.NN = Ham.E_NN
+    e.mTS = mTS
+    return e
+
+
+def wgauss(x):
+    """occupations.jl:71-84."""
+    x = np.asarray(x, dtype=float)
+    with np.errstate(over="ignore"):
+        v = 1.0 / (1.0 + np.exp(-x))
+    return np.where(x < -200.0, 0.0, np.where(x > 200.0, 1.0, v))
+
+
+def find_E_fermi(Nelectrons, wk, kT, evals, NiterMax=300):
+    """occupations.jl:126-183 (bisection, SMALL = 1e-10)."""
+    sumkg = lambda e: float(np.sum(2 * wk * np.sum(wgauss((e - evals) / kT), axis=0)))
+    Elw = evals[0].min() - 2 * kT
+    Eup = evals[-1].max() + 2 * kT
+    if sumkg(Eup) - Nelectrons < -EPS or sumkg(Elw) - Nelectrons > EPS:
+        raise RuntimeError("Bounds for E_fermi is not found")
+    Ef = Ef_old = 0.5 * (Eup + Elw)
+    for _ in range(NiterMax):
+        s = sumkg(Ef)
+        if abs(s - Nelectrons) < 1e-10:
+            return Ef
+        if s - Nelectrons < -1e-10:
+            Elw = Ef
+        else:
+            Eup = Ef
+        Ef = 0.5 * (Eup + Elw)
+        if abs(Ef - Ef_old) < 1e-10:
+            return Ef
+        Ef_old = Ef
+    return Ef
+
+
+def calc_Focc(Nelectrons, wk, kT, evals):
+    """occupations.jl:6-32 (Nspin = 1)."""
+    Ef = find_E_fermi(Nelectrons, wk, kT, evals)
+    return 2.0 * wgauss((Ef - evals) / kT), Ef
+
+
+def calc_entropy(wk, kT, evals, Ef):
+    """occupations.jl:222-245 (w1gauss :196-206)."""
+    x = (Ef - evals) / kT
+    with np.errstate(all="ignore"):
+        f = 1.0 / (1.0 + np.exp(-x))
+        w1 = np.where(np.abs(x) <= 36.0, f * np.log(f) + (1 - f) * np.log(1 - f), 0.0)
+    return float(np.sum(2 * wk[None, :] * kT * w1))
+
+
+def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6,
+                  ethr_evals_last=1e-5, update_psi="LOBPCG", use_smearing=False, kT=1e-3,
+                  seed=1234, verbose=False, mixed_until=None):
+    """KS_solve_SCF! (KS_solve_SCF.jl:24-373), mix_method = "simple", Nspin = 1.
+
+    `mixed_until`: when Ham.mixed is set, switch the Hamiltonian back to FP64 once the
+    energy change falls below this threshold (FP64 refinement of the final iterations)."""
+    pw = Ham.pw
+    el = Ham.electrons
+    if psiks is None:
+        psiks = rand_BlochWavefunc(Ham, seed=seed)
+    rho = Ham.guess_rhoe()
+    Ham.update(rho)
+    Focc = el.Focc.copy()
+    Etot_old = 0.0
+    nconv = 0
+    ethr = 0.1
+    evals = np.zeros((el.Nstates, pw.Nkpt))
+    mTS = 0.0
+    history = []
+    ncols = 0
+    t0 = time.perf_counter()
+    solver = diag_LOBPCG_ if update_psi == "LOBPCG" else diag_davidson_
+    e = None
+    for it in range(1, NiterMax + 1):
+        if it == 1:
+            ethr = 0.1
+        elif it == 2:
+            ethr = 0.01
+        else:
+            ethr = max(ethr / 5.0, ethr_evals_last)
+        for ik in range(pw.Nkpt):
+            Ham.ik, Ham.ispin = ik, 0
+            ev, st = solver(Ham, psiks[ik], tol=ethr, Nstates_conv=el.Nstates_occ)
+            evals[:, ik] = ev
+            ncols += st["ncols"]
+        if use_smearing:
+            Focc, Ef = calc_Focc(el.Nelectrons, pw.wk, kT, evals)
+            mTS = calc_entropy(pw.wk, kT, evals, Ef)
+            el.E_fermi = Ef
+            el.Focc = Focc.copy()
+        rho_new = calc_rhoe_(Ham, psiks, np.zeros(pw.Npoints), Focc=Focc)
+        rho = betamix * rho_new + (1 - betamix) * rho
+        rho = np.maximum(rho, EPS)
+        Ham.update(rho)
+        e = calc_energies(Ham, psiks, rho, Focc, mTS)
+        Etot = e.total()
+        diffE = abs(Etot - Etot_old)
+        history.append(Etot)
+        if verbose:
+            print(f"SCF: {it:5d} {Etot:18.10f} {diffE:12.5e}{'  (mixed)' if Ham.mixed else ''}", flush=True)
+        if Ham.mixed and mixed_until is not None and diffE < mixed_until:
+            Ham.mixed = False
+            nconv = 0
+            Etot_old = Etot
+            continue
+        nconv = nconv + 1 if diffE < etot_conv_thr else 0
+        if nconv >= 2:
+            break
+        Etot_old = Etot
+    el.ebands = evals
+    Ham.energies = e
+    return {"energies": e.as_dict(), "evals": evals, "psiks": psiks, "rho": rho, "history": history,
+            "ncols": ncols, "seconds": time.perf_counter() - t0, "iterations": len(history)}

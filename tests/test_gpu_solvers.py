@@ -1,0 +1,81 @@
+"""GPU eigensolvers and SCF through the C ABI, against closed forms, dense
+diagonalisation and the reference's recorded total energy."""
+import numpy as np
+import pytest
+
+from _systems import A_SI, oracle_ham, product_ham, rand_psi, si_fcc_structure
+
+pytestmark = pytest.mark.gpu
+
+# test/LOG_runtest_v.1.1.1:1181 -- Si fcc GTH-LDA, 15 Ha, 3x3x3 MP, LOBPCG + simple mixing
+# (beta = 0.5); converged to |dE| = 2.6e-7, so the SCF end point is pinned to ~1e-6.
+ETOT_LOG = -7.9110091083
+
+
+def _si(meshk=(1, 1, 1), **kw):
+    import pwdft_hip as ph
+    sym, frac = si_fcc_structure()
+    return product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=meshk,
+                       extra_states=4, **kw)
+
+
+@pytest.mark.parametrize("solver", ["LOBPCG", "davidson"])
+def test_free_electron(solver):
+    """V = 0 and no projectors: eigenvalues are the sorted 1/2|G+k|^2
+    (timing/time_diag_LOBPCG.jl:5-24)."""
+    import pwdft_hip as ph
+    H = _si(meshk=(2, 2, 2))
+    H.set_Vloc(np.zeros(H.pw.Npoints))
+    H.ctx.call("pw_set_dij", 0, None)
+    for ik in range(H.pw.Nkpt):
+        H.ctx.call("pw_set_projectors", ik, 0, None, 0)
+    psiks = ph.rand_BlochWavefunc(H, seed=5)
+    fn = ph.diag_LOBPCG if solver == "LOBPCG" else ph.diag_davidson
+    print(f"case free electron {solver}", flush=True)
+    evals, _ = fn(H, psiks, tol=1e-12, NiterMax=300)
+    for ik in range(H.pw.Nkpt):
+        exact = np.sort(0.5 * H.pw.gk2(ik))[:8]
+        assert np.abs(evals[:, ik] - exact).max() < 1e-8, (ik, evals[:, ik], exact)
+
+
+@pytest.mark.parametrize("solver", ["LOBPCG", "davidson"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_eigs_vs_dense(solver, mixed):
+    """Lowest 8 eigenvalues of the Si fcc Hamiltonian at one k-point against the dense
+    eigendecomposition of the oracle's explicit H matrix."""
+    import pwdft_hip as ph
+    from oracle import ops as O
+    from oracle.setup import gen_lattice_fcc
+    sym, frac = si_fcc_structure()
+    oH = oracle_ham(sym, frac, gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(2, 2, 2), extra_states=4)
+    H = _si(meshk=(2, 2, 2))
+    rho = H.guess_rhoe()
+    H.update(rho)
+    oH.update(rho)
+    ik = 1
+    H.ik = oH.ik = ik
+    n = H.pw.Ngw[ik]
+    Hd = O.op_H(oH, np.eye(n, dtype=complex))
+    exact = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))[:8]
+    X = ph.DeviceMatrix.from_host(H.ctx, rand_psi(np.random.default_rng(1), n, 8))
+    ph.ortho_sqrt_(H, X)
+    H.mixed = mixed
+    fn = ph.diag_LOBPCG_ if solver == "LOBPCG" else ph.diag_davidson_
+    print(f"case eigs {solver} mixed={mixed}", flush=True)
+    try:
+        ev, st = fn(H, X, tol=1e-11 if not mixed else 1e-6, NiterMax=300)
+    finally:
+        H.mixed = False
+    tol = 1e-8 if not mixed else 1e-4
+    assert np.abs(ev - exact).max() < tol, (ev, exact)
+
+
+def test_scf_si_fcc_matches_reference_log():
+    """KS_solve_SCF! with LOBPCG and simple mixing (beta 0.5) on the 3x3x3 grid (14 k-points
+    after time reversal) reproduces the reference's recorded total energy."""
+    import pwdft_hip as ph
+    H = _si(meshk=(3, 3, 3))
+    print("case scf si fcc fp64", flush=True)
+    out = ph.KS_solve_SCF_(H, betamix=0.5, etot_conv_thr=1e-6, verbose=True)
+    assert abs(out["energies"]["Total"] - ETOT_LOG) < 1e-6, out["energies"]
+    assert abs(out["energies"]["NN"] - (-8.397927400714764)) < 1e-10
