@@ -86,13 +86,18 @@ print(json.dumps({"bands": done, "seconds": dt}))
                       f"same Si64 system in {d['seconds']:.1f} s, 1 thread"}
 
 
-def time_op_H(H, X, Y, steps, warmup, mixed):
+def time_op_H(H, X, Y, steps, warmup, mixed, profile=False):
+    """Time `steps` op_H applications (all bands) between two events on the library's
+    stream; with profile=True the library also brackets every kernel launch with events.
+    Returns (GPU seconds, wall seconds, per-kernel profile or None)."""
     from pwdft_hip._lib import PW_MIXED, PW_TERM_ALL
     c = H.ctx
     flags = PW_TERM_ALL | (PW_MIXED if mixed else 0)
     for _ in range(warmup):
         c.call("pw_op_H_dev", 0, 0, NBANDS, X.p, X.rows, Y.p, Y.rows, flags)
     c.call("pw_synchronize")
+    if profile:
+        c.profile(True)
     c.call("pw_event_record", 0)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -103,7 +108,44 @@ def time_op_H(H, X, Y, steps, warmup, mixed):
     import ctypes
     ms = ctypes.c_float()
     c.call("pw_event_elapsed_ms", 0, 1, ctypes.byref(ms))
-    return ms.value / 1e3, wall
+    prof = None
+    if profile:
+        prof = c.profile_read()
+        c.profile(False)
+    return ms.value / 1e3, wall, prof
+
+
+def roofline(prof, H, mixed):
+    """Roofline object for the kernel with the largest total time in one op_H step."""
+    name = max(prof, key=lambda k: prof[k][0])
+    ms, n, nbytes = prof[name]
+    avg = ms / 1e3 / max(n, 1)
+    per = nbytes / max(n, 1)
+    ngw, nb = H.pw.Ngw[0], H.pspotNL.NbetaNL
+    if name.startswith("gemm"):
+        flops = 8.0 * ngw * nb * NBANDS
+        peak = 157.3 if mixed else 78.6
+        achieved = flops / avg / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "avg_launch_us": avg * 1e6,
+                "algorithmic": flops}
+    achieved = per / avg / 1e9
+    tr = _traffic(name, mixed)
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": tr, "avg_launch_us": avg * 1e6,
+            "algorithmic": per}
+
+
+def _traffic(name, mixed):
+    """HBM bytes per launch of `name` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    f = os.path.join(ROOT, "profiles", "traffic_si64.json")
+    try:
+        with open(f) as fh:
+            d = json.load(fh)
+        return d["kernels"][("mixed/" if mixed else "fp64/") + name]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -112,6 +154,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
@@ -122,15 +166,27 @@ def main():
     X = make_psi(H)
     Y = ph.DeviceMatrix(H.ctx, X.rows, NBANDS)
     print(f"# Ngw={H.pw.Ngw[0]} Nbeta={H.pspotNL.NbetaNL} Ns={H.pw.Ns}", file=sys.stderr, flush=True)
+    if args.calib_copy:
+        a = ph.DeviceMatrix(H.ctx, 16 * 1024 * 1024, 1)
+        b = ph.DeviceMatrix(H.ctx, 16 * 1024 * 1024, 1)
+        H.ctx.call("pw_memcpy_d2d", b.p, a.p, a.nbytes)
+        H.ctx.call("pw_synchronize")
+        a.free()
+        b.free()
     # accuracy of the mixed path: band energies <psi|H|psi>
     res = {}
     from pwdft_hip.solvers import _Dense
     for mixed in (False, True):
-        sec, wall = time_op_H(H, X, Y, args.steps, args.warmup, mixed)
+        sec, wall, prof = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=True)
         D = _Dense(H, False)
         e = np.real(np.diag(D.gram(X.p, Y.p, NBANDS, NBANDS, X.rows)))
-        res["mixed" if mixed else "fp64"] = {"sec": sec, "wall": wall, "eband": e}
-        print(f"# {'mixed' if mixed else 'fp64'}: {sec / args.steps * 1e3:.3f} ms/op_H", file=sys.stderr, flush=True)
+        res["mixed" if mixed else "fp64"] = {"sec": sec, "wall": wall, "eband": e, "prof": prof}
+        tag = "mixed" if mixed else "fp64"
+        print(f"# {tag}: {sec / args.steps * 1e3:.3f} ms/op_H", file=sys.stderr, flush=True)
+        for k, (ms, n, b) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+            if n:
+                print(f"#   {tag} {k:11s} {ms / n * 1e3:9.1f} us/launch x{n // args.steps} per step, "
+                      f"{b / n / (ms / n / 1e3) / 1e9:8.1f} GB/s algorithmic", file=sys.stderr, flush=True)
     Natoms = H.atoms.Natoms
     dE = abs(np.sum(res["mixed"]["eband"]) - np.sum(res["fp64"]["eband"])) * 2.0 / Natoms
     t_mix = res["mixed"]["sec"] / args.steps
@@ -155,7 +211,10 @@ def main():
         "fp64": {"value": NBANDS / t_64, "ms_per_step": t_64 * 1e3},
         "mixed_over_fp64": t_64 / t_mix,
         "dE_per_atom_mixed_vs_fp64": dE,
-        "roofline": None,
+        "roofline": roofline(res["mixed"]["prof"], H, True),
+        "roofline_fp64": roofline(res["fp64"]["prof"], H, False),
+        "kernels_us_per_step": {t: {k: v[0] * 1e3 / args.steps for k, v in res[t]["prof"].items() if v[1]}
+                                for t in ("fp64", "mixed")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

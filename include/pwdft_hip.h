@@ -134,6 +134,20 @@ int pw_copy_cols_dev(pw_ctx* ctx, int nrows, int ncols, const void* src, int64_t
                      int64_t ldd);
 
 /* ---- timing ---------------------------------------------------------------------- */
+/* Per-kernel timing slots (HIP events around each launch on the context stream, plus the
+ * kernel's algorithmic HBM bytes per launch).  No reference counterpart. */
+#define PW_PROF_SLOTS 8
+enum {
+  PW_PROF_COL_IFFT = 0,    /* sphere scatter + inverse z-FFT of the columns        */
+  PW_PROF_PLANE_VLOC = 1,  /* y/x FFTs and the V(r) product on z-planes            */
+  PW_PROF_COL_FFT = 2,     /* forward z-FFT + sphere gather + kinetic              */
+  PW_PROF_GEMM_CN = 3,     /* A^H B products (betaNL^H psi, Gram/overlap)          */
+  PW_PROF_GEMM_NN = 4,     /* A B products (beta (D B), Q U)                       */
+  PW_PROF_PLANE_RHOE = 5,  /* y/x inverse FFTs and |psi|^2 accumulation            */
+  PW_PROF_OTHER = 6
+};
+int pw_profile_enable(pw_ctx* ctx, int on);     /* on != 0 clears the counters */
+int pw_profile_read(pw_ctx* ctx, int slot, double* total_ms, long* count, double* bytes);
 /* Events on the context stream (the stream every kernel of the context runs on). */
 int pw_event_record(pw_ctx* ctx, int slot);
 int pw_event_elapsed_ms(pw_ctx* ctx, int slot0, int slot1, float* ms);

@@ -169,6 +169,49 @@ static int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>*
 
 static int stage(pw_ctx* c, size_t bytes) { return ensure(c, &c->d_tmp, &c->tmp_bytes, bytes); }
 
+static hipEvent_t pool_event(pw_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+ProfScope::ProfScope(pw_ctx* ctx, int s, double bytes) : c(ctx), slot(s) {
+  if (!c->profile) return;
+  hipEvent_t e0 = pool_event(c);
+  e1 = pool_event(c);
+  if (!e0 || !e1) {
+    e1 = nullptr;
+    return;
+  }
+  hipEventRecord(e0, c->stream);
+  c->ev_used.push_back({slot, {e0, e1}});
+  c->prof_bytes[slot] += bytes;
+}
+
+ProfScope::~ProfScope() {
+  if (e1) hipEventRecord(e1, c->stream);
+}
+
+static void drain_events(pw_ctx* c) {
+  if (c->ev_used.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& u : c->ev_used) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, u.second.first, u.second.second) == hipSuccess) {
+      c->prof_ms[u.first] += ms;
+      c->prof_count[u.first] += 1;
+    }
+    c->ev_pool.push_back(u.second.first);
+    c->ev_pool.push_back(u.second.second);
+  }
+  c->ev_used.clear();
+}
+
 }  // namespace pw
 
 using namespace pw;
@@ -212,6 +255,8 @@ int pw_ctx_destroy(pw_ctx* c) {
   hipFree(c->d_rho);
   hipFree(c->d_tmp);
   hipFree(c->d_wts);
+  drain_events(c);
+  for (auto e : c->ev_pool) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
   delete c;
   return PW_OK;
@@ -302,11 +347,13 @@ int pw_set_kpoint(pw_ctx* c, int ik, int ngw, const int64_t* idx, int base, cons
     return a.col != b.col ? a.col < b.col : a.z < b.z;
   });
   std::vector<int> e_igw(ngw), e_pos(ngw), col_ptr(ncol + 1, 0);
+  const int cs = odd_stride(n3), ps = odd_stride(n1);
   for (int e = 0; e < ngw; ++e) {
     e_igw[e] = ent[e].igw;
-    e_pos[e] = ent[e].col * n3 + ent[e].z;
+    e_pos[e] = ent[e].col * cs + ent[e].z;
     col_ptr[ent[e].col + 1]++;
   }
+  for (int cc = 0; cc < ncol; ++cc) col_ij[cc] = (col_ij[cc] % n1) + ps * (col_ij[cc] / n1);
   for (int cc = 0; cc < ncol; ++cc) col_ptr[cc + 1] += col_ptr[cc];
   std::sort(xs.begin(), xs.end());
   xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
@@ -558,6 +605,31 @@ int pw_copy_cols_dev(pw_ctx* c, int nrows, int ncols, const void* src, int64_t l
                      reinterpret_cast<const cplx<double>*>(src), lds, reinterpret_cast<cplx<double>*>(dst), ldd,
                      nrows, ncols);
   return hip_check(c, hipGetLastError(), "copy_cols");
+}
+
+int pw_profile_enable(pw_ctx* c, int on) {
+  CTX_CHECK(c);
+  drain_events(c);
+  c->profile = on != 0;
+  if (on) {
+    for (int i = 0; i < PW_PROF_SLOTS; ++i) {
+      c->prof_ms[i] = 0.0;
+      c->prof_count[i] = 0;
+      c->prof_bytes[i] = 0.0;
+    }
+  }
+  return PW_OK;
+}
+
+int pw_profile_read(pw_ctx* c, int slot, double* total_ms, long* count, double* bytes) {
+  CTX_CHECK(c);
+  if (slot < 0 || slot >= PW_PROF_SLOTS || !total_ms || !count || !bytes)
+    return set_error(c, PW_ERR_ARG, "pw_profile_read: bad arguments");
+  drain_events(c);
+  *total_ms = c->prof_ms[slot];
+  *count = c->prof_count[slot];
+  *bytes = c->prof_bytes[slot];
+  return PW_OK;
 }
 
 static hipEvent_t g_events[16];

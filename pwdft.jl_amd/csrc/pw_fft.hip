@@ -24,8 +24,19 @@
 
 namespace pw {
 
-constexpr int MAXE = 36;  // elements a thread may hold across one pass barrier
-constexpr int FFT_THREADS = 256;
+// Launch shapes: column kernels run 512 threads holding up to COL_ME elements each across
+// a pass barrier; plane kernels run 1024 threads (16 waves on one CU, whose LDS the plane
+// fills) holding up to PLANE_ME elements each.
+#ifndef PW_COL_NT
+#define PW_COL_NT 512
+#endif
+#ifndef PW_PLANE_NT
+#define PW_PLANE_NT 1024
+#endif
+constexpr int COL_NT = PW_COL_NT;
+constexpr int COL_ME = 12;
+constexpr int PLANE_NT = PW_PLANE_NT;
+constexpr int PLANE_ME = 12;
 
 // ----------------------------------------------------------------------------- butterflies
 
@@ -143,10 +154,10 @@ struct DFT<T, S, 9> {
 // One Stockham pass over `nlines` lines of length n held in LDS.  Element r of line L is
 // at lds[base(L) + r*es], base(L) = lbase ? lbase[L] : L*lstride.  Ns = product of the
 // radices already applied.  tw[t] = exp(+2 pi i t / n).
-template <typename T, int S, int R>
+template <typename T, int S, int R, int ME>
 __device__ void fft_pass(cplx<T>* lds, int nlines, const int* lbase, int lstride, int es, int n,
                          int Ns, const cplx<T>* __restrict__ tw) {
-  constexpr int MB = MAXE / R;
+  constexpr int MB = (ME / R) > 0 ? (ME / R) : 1;
   const int nthr = blockDim.x, tid = threadIdx.x;
   const int nbl = n / R;
   const int nbf = nlines * nbl;
@@ -183,84 +194,99 @@ __device__ void fft_pass(cplx<T>* lds, int nlines, const int* lbase, int lstride
   __syncthreads();
 }
 
-template <typename T, int S>
+template <typename T, int S, int ME>
 __device__ void fft_lines(cplx<T>* lds, int nlines, const int* lbase, int lstride, int es,
                           const FftDim& d, const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int p = 0; p < d.npass; ++p) {
     const int R = d.radix[p];
     switch (R) {
-      case 2: fft_pass<T, S, 2>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 3: fft_pass<T, S, 3>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 4: fft_pass<T, S, 4>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 5: fft_pass<T, S, 5>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 8: fft_pass<T, S, 8>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      default: fft_pass<T, S, 9>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 2: fft_pass<T, S, 2, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 3: fft_pass<T, S, 3, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 4: fft_pass<T, S, 4, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 5: fft_pass<T, S, 5, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 8: fft_pass<T, S, 8, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      default: fft_pass<T, S, 9, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
     }
     Ns *= R;
   }
+}
+
+// copy a twiddle table into LDS (caller synchronises)
+template <typename T>
+__device__ inline void stage_tw(cplx<T>* dst, const cplx<T>* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 // ----------------------------------------------------------------------------- kernels
 
 // 1. scatter + inverse z-FFT of the columns [c0, c0+C) of band (b0 + blockIdx.y).
 template <typename T>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(COL_NT)
 k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __restrict__ W,
            const int* __restrict__ ent_igw, const int* __restrict__ ent_pos,
            const int* __restrict__ col_ptr, int ncol, int C, FftDim dz,
            const cplx<T>* __restrict__ twz) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int n3 = dz.n;
+  const int n3 = dz.n, cs = odd_stride(n3);
+  cplx<T>* tw = lds + C * cs;
   const int bl = blockIdx.y;
   const int c0 = blockIdx.x * C;
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
-  for (int i = tid; i < cn * n3; i += nthr) lds[i] = mk((T)0, (T)0);
+  for (int i = tid; i < cn * cs; i += nthr) lds[i] = mk((T)0, (T)0);
+  stage_tw(tw, twz, n3);
   __syncthreads();
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
   const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
-  const int off = c0 * n3;
+  const int off = c0 * cs;
   for (int e = e0 + tid; e < e1; e += nthr) lds[ent_pos[e] - off] = cvt<T>(p[ent_igw[e]]);
   __syncthreads();
-  fft_lines<T, +1>(lds, cn, nullptr, n3, 1, dz, twz);
+  fft_lines<T, +1, COL_ME>(lds, cn, nullptr, cs, 1, dz, tw);
   cplx<T>* w = W + (long)bl * n3 * ncol;
   for (int i = tid; i < cn * n3; i += nthr) {
     const int cl = i % cn, z = i / cn;
-    w[(long)z * ncol + c0 + cl] = lds[cl * n3 + z];
+    w[(long)z * ncol + c0 + cl] = lds[cl * cs + z];
   }
 }
 
 // 2. per (z-plane, band): y/x inverse FFT, * V/N, x/y forward FFT.  In place on W.
 template <typename T>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(PLANE_NT)
 k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __restrict__ col_ij,
              int ncol, const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
              const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, int n3) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int n1 = dx.n, n2 = dy.n, np = n1 * n2;
+  const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
+  cplx<T>* tx = lds + ps * n2;
+  cplx<T>* ty = tx + n1;
   const int z = blockIdx.x, bl = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  for (int i = tid; i < np; i += nthr) lds[i] = mk((T)0, (T)0);
+  for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
+  stage_tw(tx, twx, n1);
+  stage_tw(ty, twy, n2);
   __syncthreads();
   cplx<T>* w = W + ((long)bl * n3 + z) * ncol;
   for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
   __syncthreads();
-  fft_lines<T, +1>(lds, nxact, xact, 0, n1, dy, twy);   // y lines at active x
-  fft_lines<T, +1>(lds, n2, nullptr, n1, 1, dx, twx);   // x rows
+  fft_lines<T, +1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);   // y lines at active x
+  fft_lines<T, +1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);   // x rows
   const T* v = V + (long)z * np;
-  for (int i = tid; i < np; i += nthr) lds[i] = scal(v[i], lds[i]);
+  for (int i = tid; i < np; i += nthr) {
+    const int j = i / n1, x = i - j * n1;
+    lds[x + ps * j] = scal(v[i], lds[x + ps * j]);
+  }
   __syncthreads();
-  fft_lines<T, -1>(lds, n2, nullptr, n1, 1, dx, twx);
-  fft_lines<T, -1>(lds, nxact, xact, 0, n1, dy, twy);
+  fft_lines<T, -1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);
+  fft_lines<T, -1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);
   for (int c = tid; c < ncol; c += nthr) w[c] = lds[col_ij[c]];
 }
 
 // 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi.
 template <typename T>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(COL_NT)
 k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, long ldpsi,
           cplx<double>* __restrict__ hpsi, long ldh, int b0, const int* __restrict__ ent_igw,
           const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol, int C,
@@ -268,7 +294,8 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
           int acc) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int n3 = dz.n;
+  const int n3 = dz.n, cs = odd_stride(n3);
+  cplx<T>* tw = lds + C * cs;
   const int bl = blockIdx.y;
   const int c0 = blockIdx.x * C;
   const int cn = min(C, ncol - c0);
@@ -276,12 +303,13 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
   const cplx<T>* w = W + (long)bl * n3 * ncol;
   for (int i = tid; i < cn * n3; i += nthr) {
     const int cl = i % cn, z = i / cn;
-    lds[cl * n3 + z] = w[(long)z * ncol + c0 + cl];
+    lds[cl * cs + z] = w[(long)z * ncol + c0 + cl];
   }
+  stage_tw(tw, twz, n3);
   __syncthreads();
-  fft_lines<T, -1>(lds, cn, nullptr, n3, 1, dz, twz);
+  fft_lines<T, -1, COL_ME>(lds, cn, nullptr, cs, 1, dz, tw);
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
-  const int off = c0 * n3;
+  const int off = c0 * cs;
   const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
   cplx<double>* h = hpsi + (long)(b0 + bl) * ldh;
   for (int e = e0 + tid; e < e1; e += nthr) {
@@ -294,48 +322,43 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
 }
 
 // rho planes: per (z-plane, band group): y/x inverse FFT of each band and
-// sum_b w_b |psi_b(r)|^2 in registers; one partial plane per group.
-constexpr int MAXP = 32;
+// sum_b w_b |psi_b(r)|^2 accumulated in an LDS plane of doubles; one partial plane per group.
 template <typename T>
-__global__ void __launch_bounds__(FFT_THREADS)
+__global__ void __launch_bounds__(PLANE_NT)
 k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int nb, int bpg,
              double* __restrict__ rpart, long N, const int* __restrict__ col_ij, int ncol,
              const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
              const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, int n3) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int n1 = dx.n, n2 = dy.n, np = n1 * n2;
+  const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
+  cplx<T>* tx = lds + ps * n2;
+  cplx<T>* ty = tx + n1;
+  double* acc = reinterpret_cast<double*>(ty + n2);
   const int z = blockIdx.x, g = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  double acc[MAXP];
-#pragma unroll
-  for (int s = 0; s < MAXP; ++s) acc[s] = 0.0;
+  stage_tw(tx, twx, n1);
+  stage_tw(ty, twy, n2);
+  for (int i = tid; i < np; i += nthr) acc[i] = 0.0;
   const int bend = min(nb, (g + 1) * bpg);
   for (int b = g * bpg; b < bend; ++b) {
-    for (int i = tid; i < np; i += nthr) lds[i] = mk((T)0, (T)0);
+    for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
     __syncthreads();
     const cplx<T>* w = W + ((long)b * n3 + z) * ncol;
     for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
     __syncthreads();
-    fft_lines<T, +1>(lds, nxact, xact, 0, n1, dy, twy);
-    fft_lines<T, +1>(lds, n2, nullptr, n1, 1, dx, twx);
+    fft_lines<T, +1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);
+    fft_lines<T, +1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);
     const double wb = wts[b];
-#pragma unroll
-    for (int s = 0; s < MAXP; ++s) {
-      const int i = tid + s * nthr;
-      if (i < np) {
-        const double re = (double)lds[i].x, im = (double)lds[i].y;
-        acc[s] += wb * (re * re + im * im);
-      }
+    for (int i = tid; i < np; i += nthr) {
+      const int j = i / n1, x = i - j * n1;
+      const double re = (double)lds[x + ps * j].x, im = (double)lds[x + ps * j].y;
+      acc[i] += wb * (re * re + im * im);
     }
     __syncthreads();
   }
   double* out = rpart + (long)g * N + (long)z * np;
-#pragma unroll
-  for (int s = 0; s < MAXP; ++s) {
-    const int i = tid + s * nthr;
-    if (i < np) out[i] = acc[s];
-  }
+  for (int i = tid; i < np; i += nthr) out[i] = acc[i];
 }
 
 __global__ void k_rho_reduce(double* __restrict__ rho, const double* __restrict__ rpart, int ng,
@@ -369,27 +392,34 @@ __global__ void k_kprec(const cplx<double>* __restrict__ in, long ldin, cplx<dou
 
 // ----------------------------------------------------------------------------- host drivers
 
+static int per_thread(int me, int R) { return (me / R) > 0 ? (me / R) : 1; }
+
 static int cols_per_block(const pw_ctx* c, size_t esz) {
   // keep the column tile at <= 48 KiB of LDS and within one pass's register budget:
-  // C * (n3 / R) butterflies must fit (MAXE / R) per thread for every radix R.
+  // C * (n3 / R) butterflies must fit per_thread(COL_ME, R) per thread for every radix R.
   const FftDim& d = c->dim[2];
-  int C = (int)((48 * 1024) / (esz * d.n));
+  int C = (int)((48 * 1024) / (esz * odd_stride(d.n)));
   for (int p = 0; p < d.npass; ++p) {
     const int R = d.radix[p];
-    const int cap = (MAXE / R) * FFT_THREADS / (d.n / R);
+    const int cap = per_thread(COL_ME, R) * COL_NT / (d.n / R);
     if (C > cap) C = cap;
   }
   if (C < 1) C = 1;
   return C;
 }
 
-static bool pass_fits(const FftDim& d, int nlines) {
+static bool pass_fits(const FftDim& d, int nlines, int nt, int me) {
   for (int p = 0; p < d.npass; ++p) {
     const int R = d.radix[p];
-    if ((long)nlines * (d.n / R) > (long)(MAXE / R) * FFT_THREADS) return false;
+    if ((long)nlines * (d.n / R) > (long)per_thread(me, R) * nt) return false;
   }
   return true;
 }
+
+static size_t plane_lds(const pw_ctx* c, size_t esz) {
+  return ((size_t)odd_stride(c->n1) * c->n2 + c->n1 + c->n2) * esz;
+}
+static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
 template <typename T>
 static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
@@ -398,10 +428,11 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
   const int nchunk = (k.ncol + C - 1) / C;
-  const size_t plane_bytes = (size_t)c->n1 * c->n2 * esz;
+  const size_t plane_bytes = plane_lds(c, esz);
   if (plane_bytes > 160 * 1024)
     return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
-  if (!pass_fits(c->dim[2], C) || !pass_fits(c->dim[0], c->n2) || !pass_fits(c->dim[1], k.nxact))
+  if (!pass_fits(c->dim[2], C, COL_NT, COL_ME) || !pass_fits(c->dim[0], c->n2, PLANE_NT, PLANE_ME) ||
+      !pass_fits(c->dim[1], k.nxact, PLANE_NT, PLANE_ME))
     return set_error(c, PW_ERR_ARG, "FFT plane too large for one workgroup");
   const int bb = std::max(1, std::min(nb, c->band_batch));
   const size_t wbytes = (size_t)bb * c->n3 * k.ncol * esz;
@@ -416,14 +447,26 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
   for (int b0 = 0; b0 < nb; b0 += bb) {
     const int nbb = std::min(bb, nb - b0);
     dim3 g1(nchunk, nbb);
-    hipLaunchKernelGGL(k_col_ifft<T>, g1, dim3(FFT_THREADS), C * c->n3 * esz, c->stream, psi, ldpsi, b0,
-                       W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2], tw[2]);
+    // algorithmic HBM bytes: psi in (16 B/coef) + index maps, work array out
+    const double wb = (double)nbb * c->n3 * k.ncol * esz;
+    const double psib = (double)nbb * k.ngw * 16.0;
+    {
+      ProfScope ps(c, PW_PROF_COL_IFFT, psib + wb + k.ngw * 8.0);
+      hipLaunchKernelGGL(k_col_ifft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, psi, ldpsi, b0,
+                         W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2], tw[2]);
+    }
     dim3 g2(c->n3, nbb);
-    hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(FFT_THREADS), plane_bytes, c->stream, W, V, k.d_col_ij,
-                       k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
-    hipLaunchKernelGGL(k_col_fft<T>, g1, dim3(FFT_THREADS), C * c->n3 * esz, c->stream, W, psi, ldpsi,
-                       hpsi, ldh, b0, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
-                       tw[2], k.d_gk2, kin, acc);
+    {
+      ProfScope ps(c, PW_PROF_PLANE_VLOC, 2.0 * wb + (double)c->N * sizeof(T));
+      hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PLANE_NT), plane_bytes, c->stream, W, V, k.d_col_ij,
+                         k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+    }
+    {
+      ProfScope ps(c, PW_PROF_COL_FFT, wb + psib * ((kin ? 1 : 0) + (acc ? 1 : 0) + 1) + k.ngw * 16.0);
+      hipLaunchKernelGGL(k_col_fft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, W, psi, ldpsi,
+                         hpsi, ldh, b0, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
+                         tw[2], k.d_gk2, kin, acc);
+    }
   }
   return hip_check(c, hipGetLastError(), "vloc kernels");
 }
@@ -463,10 +506,11 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const int C = cols_per_block(c, esz);
   const int nchunk = (k.ncol + C - 1) / C;
   const int np = c->n1 * c->n2;
-  const size_t plane_bytes = (size_t)np * esz;
-  if (plane_bytes > 160 * 1024 || np > MAXP * FFT_THREADS)
+  const size_t plane_bytes = plane_lds(c, esz) + (size_t)np * sizeof(double);
+  if (plane_bytes > 160 * 1024)
     return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
-  if (!pass_fits(c->dim[2], C) || !pass_fits(c->dim[0], c->n2) || !pass_fits(c->dim[1], k.nxact))
+  if (!pass_fits(c->dim[2], C, COL_NT, COL_ME) || !pass_fits(c->dim[0], c->n2, PLANE_NT, PLANE_ME) ||
+      !pass_fits(c->dim[1], k.nxact, PLANE_NT, PLANE_ME))
     return set_error(c, PW_ERR_ARG, "FFT plane too large for one workgroup");
   const int bb = std::max(1, std::min(nb, c->band_batch));
   int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)bb * c->n3 * k.ncol * esz);
@@ -483,10 +527,11 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
     const int ng = std::max(1, std::min(nbb, ngroup));
     const int bpg = (nbb + ng - 1) / ng;
     const int ngu = (nbb + bpg - 1) / bpg;
-    hipLaunchKernelGGL(k_col_ifft<T>, dim3(nchunk, nbb), dim3(FFT_THREADS), C * c->n3 * esz, c->stream,
+    hipLaunchKernelGGL(k_col_ifft<T>, dim3(nchunk, nbb), dim3(COL_NT), col_lds(c, C, esz), c->stream,
                        psi, ldpsi, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
                        tw[2]);
-    hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(FFT_THREADS), plane_bytes, c->stream, W,
+    ProfScope ps(c, PW_PROF_PLANE_RHOE, (double)nbb * c->n3 * k.ncol * esz + (double)ngu * c->N * 8.0);
+    hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PLANE_NT), plane_bytes, c->stream, W,
                        d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,
                        k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     hipLaunchKernelGGL(k_rho_reduce, dim3(1024), dim3(256), 0, c->stream, c->d_rho,

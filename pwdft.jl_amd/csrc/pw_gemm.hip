@@ -185,6 +185,10 @@ static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alph
   }
   dim3 grid(tm, tn, nz);
   const int direct = nz == 1 ? 1 : 0;
+  // algorithmic bytes: A and B read once, C written (and read when beta != 0)
+  const double gbytes = (double)M * K * sizeof(cplx<TA>) + (double)K * N * sizeof(cplx<TB>) +
+                        (double)M * N * 16.0 * (use_beta ? 2 : 1);
+  ProfScope ps(c, transa ? PW_PROF_GEMM_CN : PW_PROF_GEMM_NN, gbytes);
   if (transa)
     hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 1>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, B, ldb,
                        kchunk, C, ldc, alpha, beta, direct, use_beta, part);

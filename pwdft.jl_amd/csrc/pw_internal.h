@@ -47,6 +47,9 @@ struct FftDim {
   int radix[8];
 };
 
+// LDS strides: odd line lengths keep strided butterfly reads off a single bank group.
+__host__ __device__ inline int odd_stride(int n) { return (n % 2 == 0) ? n + 1 : n; }
+
 // Sphere bookkeeping of one k-point.  Sphere entries are sorted by (column, z);
 // a column is one (x, y) pair that holds at least one entry.
 struct KData {
@@ -55,9 +58,9 @@ struct KData {
   int nxact = 0;
   int nbeta = 0;
   int* d_ent_igw = nullptr;   // [ngw]   row of psi for each entry
-  int* d_ent_pos = nullptr;   // [ngw]   column * n3 + z
+  int* d_ent_pos = nullptr;   // [ngw]   column * cstride + z (LDS slot in the column tile)
   int* d_col_ptr = nullptr;   // [ncol+1]
-  int* d_col_ij = nullptr;    // [ncol]  i + n1 * j (offset inside a z-plane)
+  int* d_col_ij = nullptr;    // [ncol]  i + pstride * j (LDS slot inside a padded z-plane)
   int* d_xact = nullptr;      // [nxact] distinct x of the columns
   double* d_gk2 = nullptr;    // [ngw]   |G+k|^2
   double* d_gk = nullptr;     // [3*ngw] G+k (x,y,z per entry, column-major 3 x ngw)
@@ -96,9 +99,26 @@ struct pw_ctx {
   void* d_wts = nullptr;   size_t wts_bytes = 0;      // band weights of calc_rhoe
   int band_batch = 64;
   std::string err;
+  // per-kernel timing (pw_profile_*): event pairs recorded around launches on `stream`
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;  // (slot, (start, stop))
+  double prof_ms[PW_PROF_SLOTS] = {0};
+  long prof_count[PW_PROF_SLOTS] = {0};
+  double prof_bytes[PW_PROF_SLOTS] = {0};
 };
 
 namespace pw {
+
+// Brackets one kernel launch with events when profiling is on; `bytes` is the kernel's
+// algorithmic HBM traffic for that launch.
+struct ProfScope {
+  pw_ctx* c;
+  int slot;
+  hipEvent_t e1 = nullptr;
+  ProfScope(pw_ctx* ctx, int s, double bytes);
+  ~ProfScope();
+};
 
 int set_error(pw_ctx* c, int code, const std::string& msg);
 int hip_check(pw_ctx* c, hipError_t e, const char* what);

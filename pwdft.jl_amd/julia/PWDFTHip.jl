@@ -1,0 +1,160 @@
+"""
+    PWDFTHip
+
+Julia binding of libpwdft_hip.so (include/pwdft_hip.h): the drop-in for PWDFT.jl's
+Hamiltonian application, density accumulation and eigensolver products on MI355X.
+
+Loading this module after `using PWDFT` adds methods for `Hamiltonian{PsPot_GTH}` that
+route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! through the C ABI.  The
+reference's own KS_solve_SCF!, diag_LOBPCG! and diag_davidson! then run unchanged on top
+of them (they call op_H).  A HIP context is attached to each Hamiltonian the first time it
+is used, keyed by object identity; `sync_potential!(Ham)` re-uploads V_loc after
+`update!` (call it from a wrapper of update!, or use `KS_solve_SCF_hip!`).
+
+Index conventions: Julia's 1-based idx_gw2r is passed with index_base = 1; Ham.ik and
+Ham.ispin are converted to 0-based at the call.
+"""
+module PWDFTHip
+
+using PWDFT
+import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
+              calc_rhoe!, Hamiltonian, PsPot_GTH, BlochWavefunc
+
+const libpw = joinpath(@__DIR__, "..", "pwdft_hip", "libpwdft_hip.so")
+
+const PW_TERM_K = UInt32(1)
+const PW_TERM_VLOC = UInt32(2)
+const PW_TERM_NLOC = UInt32(4)
+const PW_TERM_ALL = UInt32(7)
+const PW_ACCUMULATE = UInt32(16)
+const PW_MIXED = UInt32(32)
+
+mutable struct HipCtx
+    h::Ptr{Cvoid}
+    mixed::Bool
+end
+
+const CONTEXTS = IdDict{Any,HipCtx}()
+
+function check(ctx::HipCtx, rc::Cint)
+    if rc != 0
+        msg = unsafe_string(ccall((:pw_last_error, libpw), Cstring, (Ptr{Cvoid},), ctx.h))
+        rc == -4 && throw(PWDFT.LinearAlgebra.PosDefException(0))
+        error("libpwdft_hip error $rc: $msg")
+    end
+    return nothing
+end
+
+"""Attach (once) a HIP context holding the grid, k-point spheres, projectors and V_loc."""
+function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
+    haskey(CONTEXTS, Ham) && return CONTEXTS[Ham]
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pw_ctx_create, libpw), Cint, (Ref{Ptr{Cvoid}}, Cint), r, device)
+    rc == 0 || error("pw_ctx_create failed ($rc)")
+    ctx = HipCtx(r[], false)
+    pw = Ham.pw
+    Ns = pw.Ns
+    check(ctx, ccall((:pw_set_grid, libpw), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble),
+                     ctx.h, Ns[1], Ns[2], Ns[3], pw.CellVolume))
+    # D matrix from prj2beta blocks (PsPotNL.jl:147-195)
+    nb = Ham.pspotNL.NbetaNL
+    D = zeros(Float64, nb, nb)
+    atoms = Ham.atoms
+    for ia in 1:atoms.Natoms
+        psp = Ham.pspots[atoms.atm2species[ia]]
+        for l in 0:psp.lmax, m in -l:l, i in 1:psp.Nproj_l[l+1], j in 1:psp.Nproj_l[l+1]
+            ib = Ham.pspotNL.prj2beta[i, ia, l+1, m+psp.lmax+1]
+            jb = Ham.pspotNL.prj2beta[j, ia, l+1, m+psp.lmax+1]
+            D[ib, jb] = psp.h[l+1, i, j]
+        end
+    end
+    check(ctx, ccall((:pw_set_dij, libpw), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), ctx.h, nb, D))
+    G = pw.gvec.G
+    for ik in 1:pw.gvecw.kpoints.Nkpt
+        idx = pw.gvecw.idx_gw2r[ik]
+        gk = G[:, pw.gvecw.idx_gw2g[ik]] .+ pw.gvecw.kpoints.k[:, ik]
+        check(ctx, ccall((:pw_set_kpoint, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Ptr{Int64}, Cint, Ptr{Float64}),
+                         ctx.h, ik - 1, length(idx), idx, 1, gk))
+        if nb > 0
+            β = Ham.pspotNL.betaNL[ik]
+            check(ctx, ccall((:pw_set_projectors, libpw), Cint,
+                             (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Int64),
+                             ctx.h, ik - 1, nb, β, size(β, 1)))
+        end
+    end
+    CONTEXTS[Ham] = ctx
+    sync_potential!(Ham)
+    return ctx
+end
+
+"""Upload Ham.potentials.Total (call after update!)."""
+function sync_potential!(Ham::Hamiltonian{PsPot_GTH})
+    ctx = CONTEXTS[Ham]
+    V = Ham.potentials.Total
+    for ispin in 1:size(V, 2)
+        check(ctx, ccall((:pw_set_vloc, libpw), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+                         ctx.h, ispin - 1, view(V, :, ispin)))
+    end
+end
+
+function apply!(Ham::Hamiltonian{PsPot_GTH}, psi::AbstractMatrix{ComplexF64},
+                Hpsi::AbstractMatrix{ComplexF64}, flags::UInt32)
+    ctx = context(Ham)
+    flags |= ctx.mixed ? PW_MIXED : UInt32(0)
+    check(ctx, ccall((:pw_op_H, libpw), Cint,
+                     (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Int64, Cuint),
+                     ctx.h, Ham.ik - 1, Ham.ispin - 1, size(psi, 2), psi, stride(psi, 2),
+                     Hpsi, stride(Hpsi, 2), flags))
+    return Hpsi
+end
+
+# op_H.jl:4-27 and :53-63 (EXX is skipped when Ham.exx === nothing, see op_H.jl:61)
+op_H(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_ALL)
+op_H!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+    (apply!(Ham, psi, Hpsi, PW_TERM_ALL); nothing)
+op_K(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_K)
+op_K!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+    (apply!(Ham, psi, Hpsi, PW_TERM_K | PW_ACCUMULATE); nothing)
+op_V_loc(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_VLOC)
+op_V_loc!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+    (apply!(Ham, psi, Hpsi, PW_TERM_VLOC | PW_ACCUMULATE); nothing)
+op_V_Ps_nloc(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_NLOC)
+op_V_Ps_nloc!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+    (apply!(Ham, psi, Hpsi, PW_TERM_NLOC | PW_ACCUMULATE); nothing)
+
+# calc_rhoe.jl:9-135 (norm-conserving, Nspin = 1; symmetrisation stays in PWDFT)
+function calc_rhoe!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64})
+    ctx = context(Ham)
+    pw = Ham.pw
+    wk = pw.gvecw.kpoints.wk
+    Focc = Ham.electrons.Focc
+    check(ctx, ccall((:pw_rhoe_zero, libpw), Cint, (Ptr{Cvoid},), ctx.h))
+    for ik in 1:pw.gvecw.kpoints.Nkpt
+        psi = psiks[ik]
+        w = wk[ik] .* Focc[:, ik]
+        check(ctx, ccall((:pw_rhoe_accum, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Int64, Ptr{Float64}, Cuint),
+                         ctx.h, ik - 1, size(psi, 2), psi, size(psi, 1), w,
+                         ctx.mixed ? PW_MIXED : UInt32(0)))
+    end
+    check(ctx, ccall((:pw_rhoe_get, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, view(Rhoe, :, 1)))
+    if Ham.sym_info.Nsyms > 1
+        PWDFT.symmetrize_rhoe!(Ham.pw, Ham.sym_info, Ham.rhoe_symmetrizer, Rhoe)
+    end
+    return
+end
+
+"""KS_solve_SCF! with the potential kept in sync on the device after every update!."""
+function KS_solve_SCF_hip!(Ham::Hamiltonian{PsPot_GTH}; mixed::Bool=false, kwargs...)
+    context(Ham).mixed = mixed
+    orig = PWDFT.update!
+    # PWDFT calls update!(Ham, psiks, Rhoe); re-upload V_loc right after it
+    @eval PWDFT function update!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64})
+        invoke($orig, Tuple{Hamiltonian, BlochWavefunc, Matrix{Float64}}, Ham, psiks, Rhoe)
+        $(sync_potential!)(Ham)
+    end
+    PWDFT.KS_solve_SCF!(Ham; kwargs...)
+end
+
+end # module

@@ -20,6 +20,8 @@ PW_TERM_ALL = 7
 PW_ACCUMULATE = 16
 PW_MIXED = 32
 
+PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
+
 _c = ctypes
 _vp = _c.c_void_p
 _i = _c.c_int
@@ -54,6 +56,8 @@ _PROTOS = {
     "pw_synchronize": [_vp],
     "pw_copy_cols_dev": [_vp, _i, _i, _vp, _i64, _vp, _i64],
     "pw_event_record": [_vp, _i],
+    "pw_profile_enable": [_vp, _i],
+    "pw_profile_read": [_vp, _i, _c.POINTER(_c.c_double), _c.POINTER(_c.c_long), _c.POINTER(_c.c_double)],
     "pw_event_elapsed_ms": [_vp, _i, _i, _c.POINTER(_c.c_float)],
 }
 
@@ -116,6 +120,18 @@ class Context:
 
     def call(self, name, *args):
         self.check(getattr(self.lib, name)(self.h, *args))
+
+    def profile(self, on):
+        self.call("pw_profile_enable", 1 if on else 0)
+
+    def profile_read(self):
+        """{slot name: (total_ms, launches, algorithmic bytes)} since profile(True)."""
+        out = {}
+        for name, slot in PROF_SLOTS.items():
+            ms, n, b = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+            self.call("pw_profile_read", slot, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
+            out[name] = (ms.value, n.value, b.value)
+        return out
 
     def close(self):
         if self.h is not None and self.h.value:

@@ -1,0 +1,105 @@
+"""CPU-only checks of the product's host side: its set-up data equals the oracle's, and the
+C-ABI library loads and exports every entry point include/pwdft_hip.h declares (no compute
+call is made: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from _systems import A_SI, ROOT, si_fcc_structure
+
+import pwdft_hip.basis as B
+import pwdft_hip.potentials as P
+from oracle import setup as S
+from oracle.density import Poisson_solve, calc_epsxc_Vxc_VWN, guess_rhoe as o_guess
+from oracle.energy import calc_E_NN
+
+
+def _pair(ecut=15.0, meshk=(2, 2, 2), shift=(0, 0, 0)):
+    sym, frac = si_fcc_structure()
+    k, wk = B.kpoint_grid(B.gen_lattice_fcc(A_SI), meshk, shift)
+    ko, wko = S.gen_kpoint_grid(S.gen_lattice_fcc(A_SI), meshk, shift)
+    assert np.array_equal(k, ko) and np.array_equal(wk, wko)
+    pw = B.PWGrid(ecut, B.gen_lattice_fcc(A_SI), kpoints=(k, wk))
+    opw = S.PWGrid(ecut, S.gen_lattice_fcc(A_SI), kpoints=(ko, wko))
+    return sym, frac, pw, opw
+
+
+def test_pwgrid_maps_identical():
+    _, _, pw, opw = _pair()
+    assert pw.Ns == opw.Ns
+    assert np.array_equal(pw.idx_g2r, opw.idx_g2r)
+    assert np.array_equal(pw.G, opw.G) and np.array_equal(pw.G2_shells, opw.G2_shells)
+    for ik in range(pw.Nkpt):
+        assert np.array_equal(pw.idx_gw2r[ik], opw.idx_gw2r[ik])
+        assert np.array_equal(pw.gk2(ik), opw.gk2(ik))
+
+
+def test_kpoints_shifted_al_mesh():
+    k, wk = B.kpoint_grid(B.gen_lattice_fcc(7.6), (4, 4, 4), (1, 1, 1))
+    ko, wko = S.gen_kpoint_grid(S.gen_lattice_fcc(7.6), (4, 4, 4), (1, 1, 1))
+    assert np.array_equal(k, ko) and np.allclose(wk, wko)
+    assert k.shape[1] == 32   # 64 shifted points pair up under time reversal
+
+
+def test_projectors_and_local_potential_identical():
+    sym, frac, pw, opw = _pair()
+    at = B.Atoms(sym, frac, B.gen_lattice_fcc(A_SI))
+    oat = S.Atoms(sym, frac, S.gen_lattice_fcc(A_SI))
+    psp = [B.PsPot_GTH("Si-q4.lda.gth")]
+    opsp = [S.PsPot_GTH(os.path.join(ROOT, "pwdft.jl_amd", "pwdft_hip", "pseudopotentials", "Si-q4.lda.gth"))]
+    assert np.array_equal(psp[0].h, opsp[0].h) and np.array_equal(psp[0].c, opsp[0].c)
+    nl, onl = B.PsPotNL(at, pw, psp), S.PsPotNL(oat, opw, opsp)
+    assert np.array_equal(nl.D, onl.D)
+    for ik in range(pw.Nkpt):
+        assert np.abs(nl.betaNL[ik] - onl.betaNL[ik]).max() < 1e-13
+    v, ov = B.V_Ps_loc(at, pw, psp), S.calc_V_Ps_loc(oat, opw, opsp)
+    assert np.abs(v - ov).max() < 1e-12
+
+    class OH:  # the oracle's guess_rhoe reads these fields
+        pass
+    oh = OH()
+    oh.atoms, oh.pw, oh.pspots, oh.Nelectrons = oat, opw, opsp, 8.0
+    r, orr = P.guess_rhoe(at, pw, psp, 8.0), o_guess(oh)
+    assert np.abs(r - orr).max() < 1e-14
+    assert np.abs(P.poisson_solve(pw, r) - Poisson_solve(opw, orr)).max() < 1e-12
+    e1, v1 = P.epsxc_vxc_vwn(r)
+    e2, v2 = calc_epsxc_Vxc_VWN(r)
+    assert np.abs(e1 - e2).max() < 1e-15 and np.abs(v1 - v2).max() < 1e-15
+    assert abs(P.calc_E_NN(at, [4.0]) - calc_E_NN(oat.LatVecs, oat, np.array([4.0]))) < 1e-12
+
+
+def test_electrons_counts():
+    from pwdft_hip.hamiltonian import Electrons
+    e = Electrons(8.0, 4, extra_states=4)
+    assert (e.Nstates, e.Nstates_occ) == (8, 4) and e.Focc[:4].sum() == 32
+    a = Electrons(3.0, 2, Nstates=8)
+    assert (a.Nstates, a.Nstates_occ) == (8, 2) and a.Focc[1, 0] == 1.0
+    with pytest.raises(ValueError):
+        Electrons(8.0, 1, Nstates=3)
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "pwdft_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pw_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pwdft_hip import _lib
+    path = _lib.LIB_PATH
+    assert os.path.exists(path), "run __graft_entry__.build() first"
+    lib = ctypes.CDLL(path)
+    syms = _header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(_lib.EXPORTED) == syms
+
+
+def test_null_context_is_an_error_not_a_crash():
+    from pwdft_hip import _lib
+    lib = _lib.load()
+    assert lib.pw_set_grid(None, 8, 8, 8, 1.0) == -1
+    assert lib.pw_last_error(None) == b"null context"

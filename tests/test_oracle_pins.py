@@ -1,0 +1,228 @@
+"""Pins of the CPU oracle against the reference's own known answers, closed forms, brute
+force on tiny inputs and invariants.  A plausible mistake in the oracle (dropped term,
+wrong sign, transposed operand, wrong normalisation) fails at least one of these."""
+import math
+
+import numpy as np
+import pytest
+
+from _systems import A_SI, PSP, oracle_ham, si_fcc_structure
+
+from oracle import ops as O
+from oracle import setup as S
+from oracle.density import calc_epsxc_Vxc_VWN, calc_rhoe, Poisson_solve
+from oracle.energy import calc_E_NN, Hamiltonian, KS_solve_SCF
+from oracle.solvers import diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
+
+
+def test_vwn_table():
+    """test/test_xc.jl:3-4 (Libxc VWN, tolerance 1e-5)."""
+    eps, v = calc_epsxc_Vxc_VWN(np.array([0.1, 0.2, 0.3, 0.4, 0.5]))
+    assert np.allclose(eps, [-0.396206, -0.490557, -0.556226, -0.608272, -0.652089], atol=1e-5)
+    assert np.allclose(v, [-0.51789, -0.64225, -0.728923, -0.797671, -0.85558], atol=1e-5)
+
+
+def test_vxc_is_derivative_of_rho_epsxc():
+    """V_xc = d(rho eps_xc)/d rho (central differences)."""
+    rho = np.linspace(0.01, 0.9, 17)
+    h = 1e-6
+    ep, _ = calc_epsxc_Vxc_VWN(rho + h)
+    em, _ = calc_epsxc_Vxc_VWN(rho - h)
+    _, v = calc_epsxc_Vxc_VWN(rho)
+    fd = ((rho + h) * ep - (rho - h) * em) / (2 * h)
+    assert np.abs(fd - v).max() < 1e-7
+
+
+def test_ewald_si_fcc():
+    """test/test_energies.jl:19: E_NN = -8.397927400714764 for Si fcc, a = 10.2631."""
+    sym, frac = si_fcc_structure()
+    atoms = S.Atoms(sym, frac, S.gen_lattice_fcc(A_SI))
+    assert abs(calc_E_NN(atoms.LatVecs, atoms, np.array([4.0])) - (-8.397927400714764)) < 1e-10
+
+
+def test_grid_rule_and_sphere_sizes():
+    """PWGrid.jl:84-95 sizing rule and the plane-wave counts of the workloads."""
+    sym, frac = si_fcc_structure()
+    pw = S.PWGrid(15.0, S.gen_lattice_fcc(A_SI))
+    assert pw.Ns == (27, 27, 27) and pw.Ng == 5985
+    L64 = S.gen_lattice_sc(20.5262)
+    assert S.PWGrid(15.0, L64).Ns == (75, 75, 75)            # rule; the workload overrides to 72^3
+    pw64 = S.PWGrid(15.0, L64, Ns=(72, 72, 72))
+    assert pw64.Ngw[0] == 23871 and pw64.Ng == 191735
+    # brute force: every sphere vector satisfies the cutoff, nothing is missed
+    n1, n2, n3 = pw.Ns
+    cnt = 0
+    for k in range(n3):
+        for j in range(n2):
+            for i in range(n1):
+                m = np.array([S.mm_to_nn(i, n1), S.mm_to_nn(j, n2), S.mm_to_nn(k, n3)], dtype=float)
+                g = pw.RecVecs @ m
+                cnt += 0.5 * g @ g <= pw.ecutrho
+    assert cnt == pw.Ng
+    assert np.all(np.diff(pw.G2) >= 0)  # sorted by |G|^2
+
+
+def test_good_fft_order():
+    """fft_support.jl: only factors 2, 3, 5."""
+    assert [S.good_fft_order(n) for n in (73, 74, 75, 109, 121)] == [75, 75, 75, 120, 125]
+
+
+def test_kpoint_grid_time_reversal():
+    """3x3x3 unshifted MP grid, time reversal only: 1 + 26/2 = 14 points, weights sum to 1."""
+    k, wk = S.gen_kpoint_grid(S.gen_lattice_fcc(A_SI), (3, 3, 3), (0, 0, 0))
+    assert k.shape[1] == 14 and abs(wk.sum() - 1) < 1e-15
+    assert abs(wk[0] - 1 / 27) < 1e-15 and np.allclose(wk[1:], 2 / 27)
+
+
+def test_fft_conventions_against_direct_dft():
+    """wrappers_fft.jl: G_to_R = (1/N) sum_G f(G) exp(+2 pi i g.r), R_to_G unnormalised exp(-)."""
+    class PW:  # tiny grid
+        Ns = (4, 3, 5)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(60) + 1j * rng.standard_normal(60)
+    n1, n2, n3 = PW.Ns
+    idx = np.array([(i, j, k) for k in range(n3) for j in range(n2) for i in range(n1)])
+    ph = np.exp(2j * np.pi * (np.outer(idx[:, 0], idx[:, 0]) / n1 + np.outer(idx[:, 1], idx[:, 1]) / n2
+                              + np.outer(idx[:, 2], idx[:, 2]) / n3))
+    assert np.allclose(S.G_to_R(PW, f), ph @ f / 60)
+    assert np.allclose(S.R_to_G(PW, f), ph.conj() @ f)
+
+
+@pytest.fixture(scope="module")
+def si_small():
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 6.0, meshk=(2, 2, 2), extra_states=4)
+    rng = np.random.default_rng(1)
+    H.V_loc[:, 0] = H.V_Ps_loc + 0.2 * rng.standard_normal(H.pw.Npoints)
+    return H
+
+
+def test_op_V_loc_constant_potential(si_small):
+    H = si_small
+    H.ik = 1
+    psi = np.random.default_rng(2).standard_normal((H.pw.Ngw[1], 3)) + 0j
+    assert np.allclose(O.op_V_loc(H, psi, V=np.full(H.pw.Npoints, 0.7)), 0.7 * psi, atol=1e-13)
+
+
+def test_op_V_loc_plane_wave_cosine(si_small):
+    """psi = single plane wave G1, V(r) = 2 cos(G2.r): V psi has weight 1 at G1 +- G2."""
+    H = si_small
+    H.ik = 0
+    pw = H.pw
+    idx = pw.idx_gw2g[0]
+    n1, n2, n3 = pw.Ns
+    ip = np.arange(pw.Npoints)
+    r = np.stack([ip % n1 / n1, (ip // n1) % n2 / n2, ip // (n1 * n2) / n3])
+    m2 = np.array([1, 0, -1])
+    V = 2 * np.cos(2 * np.pi * (m2 @ r))
+    g1 = 5
+    psi = np.zeros((pw.Ngw[0], 1), dtype=complex)
+    psi[g1] = 1.0
+    out = O.op_V_loc(H, psi, V=V)[:, 0]
+    mill = pw.miller[:, idx]
+    target = {tuple(mill[:, g1] + m2), tuple(mill[:, g1] - m2)}
+    for g in range(pw.Ngw[0]):
+        want = 1.0 if tuple(mill[:, g]) in target else 0.0
+        assert abs(out[g] - want) < 1e-12
+
+
+def test_op_H_hermitian_and_nonlocal_dense(si_small):
+    H = si_small
+    H.ik = 2
+    rng = np.random.default_rng(3)
+    n = H.pw.Ngw[2]
+    a = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    b = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    assert np.allclose(a.conj().T @ O.op_H(H, b), (O.op_H(H, a)).conj().T @ b, atol=1e-10)
+    beta = H.pspotNL.betaNL[2]
+    dense = beta @ H.pspotNL.D @ beta.conj().T
+    assert np.allclose(O.op_V_Ps_nloc(H, a), dense @ a, atol=1e-12)
+    assert np.allclose(O.op_K(H, a), 0.5 * H.pw.gk2(2)[:, None] * a)
+
+
+def test_betaNL_norm_real_vs_reciprocal(si_small):
+    """check_betaNL_norm (PsPotNL.jl:199-239): with beta(r) = N * G_to_R(beta_G), Parseval
+    gives integral |beta(r)|^2 dr = Omega * sum_G |beta_G|^2."""
+    H = si_small
+    pw = H.pw
+    beta = H.pspotNL.betaNL[0]
+    for ib in range(beta.shape[1]):
+        c = np.zeros(pw.Npoints, dtype=complex)
+        c[pw.idx_gw2r[0]] = beta[:, ib]
+        r = S.G_to_R(pw, c) * pw.Npoints
+        lhs = np.sum(np.abs(r) ** 2) * pw.CellVolume / pw.Npoints
+        assert abs(lhs - pw.CellVolume * np.sum(np.abs(beta[:, ib]) ** 2)) < 1e-10 * lhs
+
+
+def test_ylm_orthonormal():
+    """Real harmonics of Ylm_real.jl are orthonormal on the sphere (Lebedev-free check:
+    dense random sampling of the unit sphere, Monte Carlo with fixed seed)."""
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((3, 400000))
+    v /= np.linalg.norm(v, axis=0)
+    Y = [S.Ylm_real(l, m, v) for l in range(4) for m in range(-l, l + 1)]
+    G = np.array([[np.mean(a * b) * 4 * np.pi for b in Y] for a in Y])
+    assert np.abs(G - np.eye(len(Y))).max() < 0.02
+
+
+def test_calc_rhoe_integrates_to_occupation(si_small):
+    H = si_small
+    rng = np.random.default_rng(4)
+    psiks = [rand_wavefunc(rng, n, H.Nstates) for n in H.pw.Ngw]
+    rho = calc_rhoe(H, psiks)
+    assert rho.min() >= 0
+    assert abs(rho.sum() * H.pw.CellVolume / H.pw.Npoints - H.Nelectrons) < 1e-10
+
+
+def test_poisson_plane_wave(si_small):
+    """Poisson: rho = cos(G.r) -> V_H = 4 pi / |G|^2 cos(G.r)."""
+    H = si_small
+    pw = H.pw
+    g = pw.G[:, 3]
+    n1, n2, n3 = pw.Ns
+    ip = np.arange(pw.Npoints)
+    frac = np.stack([ip % n1 / n1, (ip // n1) % n2 / n2, ip // (n1 * n2) / n3])
+    r = pw.LatVecs @ frac
+    rho = np.cos(g @ r)
+    assert np.abs(Poisson_solve(pw, rho) - 4 * np.pi / (g @ g) * rho).max() < 1e-10
+
+
+@pytest.mark.parametrize("solver", [diag_LOBPCG, diag_davidson])
+def test_free_electron(solver):
+    """timing/time_diag_LOBPCG.jl:5-24: V = 0, no projectors -> sorted 1/2|G+k|^2."""
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 8.0, meshk=(2, 2, 2), extra_states=4)
+    H.V_loc[:] = 0.0
+    H.pspotNL.NbetaNL = 0
+    rng = np.random.default_rng(6)
+    for ik in range(H.pw.Nkpt):
+        H.ik = ik
+        X = rand_wavefunc(rng, H.pw.Ngw[ik], 8)
+        ev, _, _ = solver(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-12, NiterMax=300)
+        assert np.abs(ev - np.sort(0.5 * H.pw.gk2(ik))[:8]).max() < 1e-9
+
+
+def test_lobpcg_vs_dense(si_small):
+    H = si_small
+    H.ik = 1
+    n = H.pw.Ngw[1]
+    Hd = O.op_H(H, np.eye(n, dtype=complex))
+    exact = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))[:8]
+    X = rand_wavefunc(np.random.default_rng(7), n, 8)
+    ev, _, _ = diag_LOBPCG(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-12, NiterMax=300)
+    assert np.abs(ev - exact).max() < 1e-9
+    assert np.allclose(X.conj().T @ X, np.eye(8), atol=1e-10)
+
+
+@pytest.mark.slow
+def test_scf_si_fcc_matches_reference_log():
+    """test/LOG_runtest_v.1.1.1:1163-1181: Si fcc, LOBPCG, simple mixing beta = 0.5:
+    Etot -7.9110091083 (converged to |dE| 2.6e-7); E_Ps_nloc 1.5811898971; today's Ps_loc is
+    the log's Ps_loc + PspCore = -2.4707538756 (PsPot_GTH.jl:184-186)."""
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(3, 3, 3), extra_states=4)
+    e, evals, psiks, rho, hist = KS_solve_SCF(H, betamix=0.5)
+    assert abs(e["Total"] - (-7.9110091083)) < 1e-6
+    assert abs(e["Ps_nloc"] - 1.5811898971) < 1e-6
+    assert abs(e["Ps_loc"] - (-2.1761282488 - 0.2946256268)) < 1e-6
+    assert abs(e["Total"] - (-7.9110087934)) < 1e-5   # ABINIT, test/validations_old/Si_fcc.jl:15
