@@ -277,9 +277,13 @@ int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
   if (n1 < 2 || n2 < 2 || n3 < 2 || !(omega > 0.0)) return set_error(c, PW_ERR_ARG, "pw_set_grid: bad sizes");
   const int ns[3] = {n1, n2, n3};
   FftDim dims[3];
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3; ++d) {
     if (!factorize(ns[d], dims[d]))
       return set_error(c, PW_ERR_ARG, "pw_set_grid: size " + std::to_string(ns[d]) + " is not a product of 2,3,5");
+    plan_lanes(dims[d], 12);
+    if (dims[d].gpw < 1)
+      return set_error(c, PW_ERR_ARG, "pw_set_grid: size " + std::to_string(ns[d]) + " too long for one wavefront");
+  }
   for (auto& k : c->kd) free_k(k);
   c->kd.clear();
   c->n1 = n1;

@@ -31,12 +31,21 @@ namespace pw {
 #define PW_COL_NT 512
 #endif
 #ifndef PW_PLANE_NT
-#define PW_PLANE_NT 1024
+#define PW_PLANE_NT 512
 #endif
+#ifndef PW_PLANE_NT64
+#define PW_PLANE_NT64 1024
+#endif
+// Launch shapes: workgroups of COL_NT / PLANE_NT threads; a lane holds at most FFT_ME
+// elements of a line across one radix pass (FftDim::L lanes per line).
 constexpr int COL_NT = PW_COL_NT;
-constexpr int COL_ME = 12;
-constexpr int PLANE_NT = PW_PLANE_NT;
-constexpr int PLANE_ME = 12;
+constexpr int PLANE_NT = PW_PLANE_NT;  // FP32 planes (~41 KiB of LDS at 72^2): several workgroups per CU
+constexpr int PLANE_NT64 = PW_PLANE_NT64;  // FP64 planes fill one CU's LDS: one wide workgroup
+template <typename T>
+struct PlaneNT {
+  static constexpr int value = sizeof(T) == 8 ? PLANE_NT64 : PLANE_NT;
+};
+constexpr int FFT_ME = 12;
 
 // ----------------------------------------------------------------------------- butterflies
 
@@ -151,28 +160,45 @@ struct DFT<T, S, 9> {
   }
 };
 
-// One Stockham pass over `nlines` lines of length n held in LDS.  Element r of line L is
-// at lds[base(L) + r*es], base(L) = lbase ? lbase[L] : L*lstride.  Ns = product of the
-// radices already applied.  tw[t] = exp(+2 pi i t / n).
+// Wave-local Stockham FFT.  One line (length n, element r at lds[base + r*es]) is owned by
+// a group of d.L lanes of one wavefront; a wavefront holds d.gpw groups.  Lane q of a group
+// takes butterflies q, q + L, ... of each radix pass, so no workgroup barrier is needed
+// between passes: the LDS operations of one wavefront complete in issue order, and the
+// wavefront fence keeps the compiler from moving the next pass's reads above this pass's
+// writes.  In-place: every lane reads all of its butterflies before any lane writes.
+// Ns = product of the radices already applied; tw[t] = exp(+2 pi i t / n).
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS pointers keep their address space across the (non-inlined) pass so that accesses
+// compile to ds_read/ds_write rather than flat instructions.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LDS_AS __attribute__((address_space(3)))
+#else
+#define LDS_AS
+#endif
+
 template <typename T, int S, int R, int ME>
-__device__ void fft_pass(cplx<T>* lds, int nlines, const int* lbase, int lstride, int es, int n,
-                         int Ns, const cplx<T>* __restrict__ tw) {
+__device__ __attribute__((noinline)) void wave_pass(LDS_AS cplx<T>* lds, int base, int es, int n, int Ns,
+                                                    const LDS_AS cplx<T>* tw, int L, int q, bool active,
+                                                    const T* __restrict__ vpre, bool usev) {
   constexpr int MB = (ME / R) > 0 ? (ME / R) : 1;
-  const int nthr = blockDim.x, tid = threadIdx.x;
   const int nbl = n / R;
-  const int nbf = nlines * nbl;
   const int twstep = n / (Ns * R);
   cplx<T> v[MB][R];
   int dst[MB];
 #pragma unroll
   for (int s = 0; s < MB; ++s) {
-    const int b = tid + s * nthr;
+    const int j = q + s * L;
     dst[s] = -1;
-    if (b < nbf) {
-      const int line = b / nbl, j = b - line * nbl;
-      const int base = lbase ? lbase[line] : line * lstride;
+    if (active && j < nbl) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[s][r] = lds[base + (j + r * nbl) * es];
+      for (int r = 0; r < R; ++r) {
+        v[s][r] = lds[base + (j + r * nbl) * es];
+        if (usev) v[s][r] = scal(vpre[j + r * nbl], v[s][r]);
+      }
       const int jm = j % Ns;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -183,7 +209,7 @@ __device__ void fft_pass(cplx<T>* lds, int nlines, const int* lbase, int lstride
       dst[s] = base + ((j - jm) * R + jm) * es;
     }
   }
-  __syncthreads();
+  wave_fence();
 #pragma unroll
   for (int s = 0; s < MB; ++s) {
     if (dst[s] >= 0) {
@@ -191,30 +217,120 @@ __device__ void fft_pass(cplx<T>* lds, int nlines, const int* lbase, int lstride
       for (int r = 0; r < R; ++r) lds[dst[s] + r * Ns * es] = v[s][r];
     }
   }
-  __syncthreads();
+  wave_fence();
 }
 
+// Compile-time specialised line FFT: the same Stockham passes as wave_pass with n, the
+// radices, Ns and the lane count known to the compiler, so index arithmetic folds to
+// constants and the first pass (all twiddles = 1) does no twiddle multiply.
+template <typename T, int S, int N, int NS, int R, int L>
+__device__ inline void spass(LDS_AS cplx<T>* lds, int base, int es, const LDS_AS cplx<T>* tw, int q,
+                             bool active, const T* __restrict__ vpre, bool usev) {
+  constexpr int NBL = N / R;
+  constexpr int MB = (NBL + L - 1) / L;
+  constexpr int TWS = N / (NS * R);
+  cplx<T> v[MB][R];
+  int dst[MB];
+#pragma unroll
+  for (int s = 0; s < MB; ++s) {
+    const int j = q + s * L;
+    dst[s] = -1;
+    if (active && (MB * L == NBL || j < NBL)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        v[s][r] = lds[base + (j + r * NBL) * es];
+        if (usev) v[s][r] = scal(vpre[j + r * NBL], v[s][r]);
+      }
+      const int jm = NS == 1 ? 0 : j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cplx<T> w = tw[jm * r * TWS];
+          v[s][r] = S > 0 ? cmul(v[s][r], w) : cmulc(v[s][r], w);
+        }
+      }
+      DFT<T, S, R>::run(v[s]);
+      dst[s] = base + ((j - jm) * R + jm) * es;
+    }
+  }
+  wave_fence();
+#pragma unroll
+  for (int s = 0; s < MB; ++s) {
+    if (dst[s] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dst[s] + r * NS * es] = v[s][r];
+    }
+  }
+  wave_fence();
+}
+
+template <typename T, int S, int N, int R0, int R1, int R2>
+__device__ __attribute__((noinline)) void line_static(LDS_AS cplx<T>* lds, int base, int es,
+                                                      const LDS_AS cplx<T>* tw, int q, bool active,
+                                                      const T* __restrict__ vpre, bool usev) {
+  constexpr int L = lanes_for(N, R0, R1, R2, 12);
+  spass<T, S, N, 1, R0, L>(lds, base, es, tw, q, active, vpre, usev);
+  spass<T, S, N, R0, R1, L>(lds, base, es, tw, q, active, vpre, false);
+  if constexpr (R2 > 1) spass<T, S, N, R0 * R1, R2, L>(lds, base, es, tw, q, active, vpre, false);
+}
+
+// One line through all radix passes.  vpre (length n, or null) multiplies the input
+// element-wise before the first pass.
 template <typename T, int S, int ME>
-__device__ void fft_lines(cplx<T>* lds, int nlines, const int* lbase, int lstride, int es,
-                          const FftDim& d, const cplx<T>* __restrict__ tw) {
+__device__ inline void wave_line_fft(cplx<T>* lds, int base, int es, const FftDim& d,
+                                     const cplx<T>* __restrict__ tw, int q, bool active,
+                                     const T* vpre) {
+  {
+    LDS_AS cplx<T>* l3 = (LDS_AS cplx<T>*)lds;
+    const LDS_AS cplx<T>* t3 = (const LDS_AS cplx<T>*)tw;
+    const bool uv = vpre != nullptr;
+    switch (d.plan) {
+      case 72: line_static<T, S, 72, 9, 8, 1>(l3, base, es, t3, q, active, vpre, uv); return;
+      case 27: line_static<T, S, 27, 9, 3, 1>(l3, base, es, t3, q, active, vpre, uv); return;
+      case 45: line_static<T, S, 45, 9, 5, 1>(l3, base, es, t3, q, active, vpre, uv); return;
+      case 20: line_static<T, S, 20, 5, 4, 1>(l3, base, es, t3, q, active, vpre, uv); return;
+      case 108: line_static<T, S, 108, 9, 4, 3>(l3, base, es, t3, q, active, vpre, uv); return;
+      default: break;
+    }
+  }
   int Ns = 1;
   for (int p = 0; p < d.npass; ++p) {
     const int R = d.radix[p];
+    // global V row for the first pass only; the flag travels separately from the pointer
+    const bool usev = (p == 0) && (vpre != nullptr);
+    const T* vp = vpre;
+    LDS_AS cplx<T>* l3 = (LDS_AS cplx<T>*)lds;
+    const LDS_AS cplx<T>* t3 = (const LDS_AS cplx<T>*)tw;
     switch (R) {
-      case 2: fft_pass<T, S, 2, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 3: fft_pass<T, S, 3, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 4: fft_pass<T, S, 4, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 5: fft_pass<T, S, 5, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      case 8: fft_pass<T, S, 8, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
-      default: fft_pass<T, S, 9, ME>(lds, nlines, lbase, lstride, es, d.n, Ns, tw); break;
+      case 2: wave_pass<T, S, 2, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
+      case 3: wave_pass<T, S, 3, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
+      case 4: wave_pass<T, S, 4, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
+      case 5: wave_pass<T, S, 5, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
+      case 8: wave_pass<T, S, 8, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
+      default: wave_pass<T, S, 9, ME>(l3, base, es, d.n, Ns, t3, d.L, q, active, vp, usev); break;
     }
     Ns *= R;
   }
 }
 
-// copy a twiddle table into LDS (caller synchronises)
+// All `nlines` lines of a workgroup, distributed over its wavefronts; line t starts at
+// lds[lbase ? lbase[t] : t * lstride].  Wavefronts run their lines independently.
+template <typename T, int S, int ME>
+__device__ inline void wave_lines(cplx<T>* lds, int nlines, const int* lbase, int lstride, int es,
+                                  const FftDim& d, const cplx<T>* __restrict__ tw) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int g = lane / d.L, q = lane - g * d.L;
+  for (int t0 = wave * d.gpw; t0 < nlines; t0 += nwave * d.gpw) {
+    const int t = t0 + g;
+    const bool active = g < d.gpw && t < nlines;
+    const int base = active ? (lbase ? lbase[t] : t * lstride) : 0;
+    wave_line_fft<T, S, ME>(lds, base, es, d, tw, q, active, nullptr);
+  }
+}
+
+// copy a table into LDS (caller synchronises)
 template <typename T>
-__device__ inline void stage_tw(cplx<T>* dst, const cplx<T>* __restrict__ src, int n) {
+__device__ inline void stage(T* dst, const T* __restrict__ src, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
@@ -236,14 +352,15 @@ k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < cn * cs; i += nthr) lds[i] = mk((T)0, (T)0);
-  stage_tw(tw, twz, n3);
+  stage(tw, twz, n3);
   __syncthreads();
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
   const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
   const int off = c0 * cs;
   for (int e = e0 + tid; e < e1; e += nthr) lds[ent_pos[e] - off] = cvt<T>(p[ent_igw[e]]);
   __syncthreads();
-  fft_lines<T, +1, COL_ME>(lds, cn, nullptr, cs, 1, dz, tw);
+  wave_lines<T, +1, FFT_ME>(lds, cn, nullptr, cs, 1, dz, tw);
+  __syncthreads();
   cplx<T>* w = W + (long)bl * n3 * ncol;
   for (int i = tid; i < cn * n3; i += nthr) {
     const int cl = i % cn, z = i / cn;
@@ -251,9 +368,11 @@ k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __
   }
 }
 
-// 2. per (z-plane, band): y/x inverse FFT, * V/N, x/y forward FFT.  In place on W.
+// 2. per (z-plane, band): inverse y-FFT on the active x lines, then per x-row one wavefront
+// does inverse x-FFT, * V/N (read from L2 inside the first forward pass) and forward
+// x-FFT, then forward y-FFT on the active x lines.  In place on W.
 template <typename T>
-__global__ void __launch_bounds__(PLANE_NT)
+__global__ void __launch_bounds__(PlaneNT<T>::value)
 k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __restrict__ col_ij,
              int ncol, const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
              const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, int n3) {
@@ -262,25 +381,34 @@ k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __rest
   const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
   cplx<T>* tx = lds + ps * n2;
   cplx<T>* ty = tx + n1;
+  int* xl = reinterpret_cast<int*>(ty + n2);
   const int z = blockIdx.x, bl = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
-  stage_tw(tx, twx, n1);
-  stage_tw(ty, twy, n2);
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
   __syncthreads();
   cplx<T>* w = W + ((long)bl * n3 + z) * ncol;
   for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
   __syncthreads();
-  fft_lines<T, +1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);   // y lines at active x
-  fft_lines<T, +1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);   // x rows
-  const T* v = V + (long)z * np;
-  for (int i = tid; i < np; i += nthr) {
-    const int j = i / n1, x = i - j * n1;
-    lds[x + ps * j] = scal(v[i], lds[x + ps * j]);
+  wave_lines<T, +1, FFT_ME>(lds, nxact, xl, 0, ps, dy, ty);
+  __syncthreads();
+  {
+    const T* v = V + (long)z * np;
+    const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
+    const int g = lane / dx.L, q = lane - g * dx.L;
+    for (int t0 = wave * dx.gpw; t0 < n2; t0 += nwave * dx.gpw) {
+      const int row = t0 + g;
+      const bool active = g < dx.gpw && row < n2;
+      const int base = active ? row * ps : 0;
+      wave_line_fft<T, +1, FFT_ME>(lds, base, 1, dx, tx, q, active, nullptr);
+      wave_line_fft<T, -1, FFT_ME>(lds, base, 1, dx, tx, q, active, active ? v + (long)row * n1 : nullptr);
+    }
   }
   __syncthreads();
-  fft_lines<T, -1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);
-  fft_lines<T, -1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);
+  wave_lines<T, -1, FFT_ME>(lds, nxact, xl, 0, ps, dy, ty);
+  __syncthreads();
   for (int c = tid; c < ncol; c += nthr) w[c] = lds[col_ij[c]];
 }
 
@@ -305,9 +433,10 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
     const int cl = i % cn, z = i / cn;
     lds[cl * cs + z] = w[(long)z * ncol + c0 + cl];
   }
-  stage_tw(tw, twz, n3);
+  stage(tw, twz, n3);
   __syncthreads();
-  fft_lines<T, -1, COL_ME>(lds, cn, nullptr, cs, 1, dz, tw);
+  wave_lines<T, -1, FFT_ME>(lds, cn, nullptr, cs, 1, dz, tw);
+  __syncthreads();
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
   const int off = c0 * cs;
   const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
@@ -324,7 +453,7 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
 // rho planes: per (z-plane, band group): y/x inverse FFT of each band and
 // sum_b w_b |psi_b(r)|^2 accumulated in an LDS plane of doubles; one partial plane per group.
 template <typename T>
-__global__ void __launch_bounds__(PLANE_NT)
+__global__ void __launch_bounds__(PlaneNT<T>::value)
 k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int nb, int bpg,
              double* __restrict__ rpart, long N, const int* __restrict__ col_ij, int ncol,
              const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
@@ -335,10 +464,12 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
   cplx<T>* tx = lds + ps * n2;
   cplx<T>* ty = tx + n1;
   double* acc = reinterpret_cast<double*>(ty + n2);
+  int* xl = reinterpret_cast<int*>(acc + np);
   const int z = blockIdx.x, g = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  stage_tw(tx, twx, n1);
-  stage_tw(ty, twy, n2);
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
   for (int i = tid; i < np; i += nthr) acc[i] = 0.0;
   const int bend = min(nb, (g + 1) * bpg);
   for (int b = g * bpg; b < bend; ++b) {
@@ -347,8 +478,10 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
     const cplx<T>* w = W + ((long)b * n3 + z) * ncol;
     for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
     __syncthreads();
-    fft_lines<T, +1, PLANE_ME>(lds, nxact, xact, 0, ps, dy, ty);
-    fft_lines<T, +1, PLANE_ME>(lds, n2, nullptr, ps, 1, dx, tx);
+    wave_lines<T, +1, FFT_ME>(lds, nxact, xl, 0, ps, dy, ty);
+    __syncthreads();
+    wave_lines<T, +1, FFT_ME>(lds, n2, nullptr, ps, 1, dx, tx);
+    __syncthreads();
     const double wb = wts[b];
     for (int i = tid; i < np; i += nthr) {
       const int j = i / n1, x = i - j * n1;
@@ -392,32 +525,20 @@ __global__ void k_kprec(const cplx<double>* __restrict__ in, long ldin, cplx<dou
 
 // ----------------------------------------------------------------------------- host drivers
 
-static int per_thread(int me, int R) { return (me / R) > 0 ? (me / R) : 1; }
-
 static int cols_per_block(const pw_ctx* c, size_t esz) {
-  // keep the column tile at <= 48 KiB of LDS and within one pass's register budget:
-  // C * (n3 / R) butterflies must fit per_thread(COL_ME, R) per thread for every radix R.
+  // column tile <= 48 KiB of LDS, at least one full round of line groups
   const FftDim& d = c->dim[2];
   int C = (int)((48 * 1024) / (esz * odd_stride(d.n)));
-  for (int p = 0; p < d.npass; ++p) {
-    const int R = d.radix[p];
-    const int cap = per_thread(COL_ME, R) * COL_NT / (d.n / R);
-    if (C > cap) C = cap;
-  }
+  const int round = d.gpw * (COL_NT / 64);
+  if (C > round) C = (C / round) * round;
   if (C < 1) C = 1;
   return C;
 }
 
-static bool pass_fits(const FftDim& d, int nlines, int nt, int me) {
-  for (int p = 0; p < d.npass; ++p) {
-    const int R = d.radix[p];
-    if ((long)nlines * (d.n / R) > (long)per_thread(me, R) * nt) return false;
-  }
-  return true;
-}
+static bool lanes_ok(const FftDim& d) { return d.L >= 1 && d.L <= 64 && d.gpw >= 1; }
 
-static size_t plane_lds(const pw_ctx* c, size_t esz) {
-  return ((size_t)odd_stride(c->n1) * c->n2 + c->n1 + c->n2) * esz;
+static size_t plane_lds(const pw_ctx* c, size_t esz, int nxact) {
+  return ((size_t)odd_stride(c->n1) * c->n2 + c->n1 + c->n2) * esz + (size_t)nxact * sizeof(int);
 }
 static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
@@ -428,12 +549,11 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
   const int nchunk = (k.ncol + C - 1) / C;
-  const size_t plane_bytes = plane_lds(c, esz);
+  const size_t plane_bytes = plane_lds(c, esz, k.nxact);
   if (plane_bytes > 160 * 1024)
     return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
-  if (!pass_fits(c->dim[2], C, COL_NT, COL_ME) || !pass_fits(c->dim[0], c->n2, PLANE_NT, PLANE_ME) ||
-      !pass_fits(c->dim[1], k.nxact, PLANE_NT, PLANE_ME))
-    return set_error(c, PW_ERR_ARG, "FFT plane too large for one workgroup");
+  if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
+    return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
   const int bb = std::max(1, std::min(nb, c->band_batch));
   const size_t wbytes = (size_t)bb * c->n3 * k.ncol * esz;
   int rc = ensure(c, &c->d_work, &c->work_bytes, wbytes);
@@ -455,10 +575,10 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
       hipLaunchKernelGGL(k_col_ifft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, psi, ldpsi, b0,
                          W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2], tw[2]);
     }
-    dim3 g2(c->n3, nbb);
     {
+      dim3 g2(c->n3, nbb);
       ProfScope ps(c, PW_PROF_PLANE_VLOC, 2.0 * wb + (double)c->N * sizeof(T));
-      hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PLANE_NT), plane_bytes, c->stream, W, V, k.d_col_ij,
+      hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V, k.d_col_ij,
                          k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     }
     {
@@ -506,12 +626,11 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const int C = cols_per_block(c, esz);
   const int nchunk = (k.ncol + C - 1) / C;
   const int np = c->n1 * c->n2;
-  const size_t plane_bytes = plane_lds(c, esz) + (size_t)np * sizeof(double);
+  const size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
   if (plane_bytes > 160 * 1024)
     return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
-  if (!pass_fits(c->dim[2], C, COL_NT, COL_ME) || !pass_fits(c->dim[0], c->n2, PLANE_NT, PLANE_ME) ||
-      !pass_fits(c->dim[1], k.nxact, PLANE_NT, PLANE_ME))
-    return set_error(c, PW_ERR_ARG, "FFT plane too large for one workgroup");
+  if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
+    return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
   const int bb = std::max(1, std::min(nb, c->band_batch));
   int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)bb * c->n3 * k.ncol * esz);
   if (rc) return rc;
@@ -531,7 +650,7 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
                        psi, ldpsi, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
                        tw[2]);
     ProfScope ps(c, PW_PROF_PLANE_RHOE, (double)nbb * c->n3 * k.ncol * esz + (double)ngu * c->N * 8.0);
-    hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PLANE_NT), plane_bytes, c->stream, W,
+    hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
                        d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,
                        k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     hipLaunchKernelGGL(k_rho_reduce, dim3(1024), dim3(256), 0, c->stream, c->d_rho,

@@ -45,7 +45,35 @@ struct FftDim {
   int n;
   int npass;
   int radix[8];
+  int L;     // lanes of one wavefront that own one line
+  int gpw;   // lines per wavefront (64 / L)
+  int plan;  // n when a compile-time specialised line FFT exists for this radix plan, else 0
 };
+
+// Lanes per line so that no lane holds more than `me` elements across one radix pass
+// (shared by the host plan and the compile-time specialised line FFTs).
+__host__ __device__ constexpr int lanes_for_pass(int n, int R, int me) {
+  return (n / R + ((me / R) > 0 ? (me / R) : 1) - 1) / ((me / R) > 0 ? (me / R) : 1);
+}
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int lanes_for(int n, int r0, int r1, int r2, int me) {
+  return cmax(cmax(lanes_for_pass(n, r0, me), lanes_for_pass(n, r1, me)), r2 > 1 ? lanes_for_pass(n, r2, me) : 1);
+}
+
+inline void plan_lanes(FftDim& d, int me) {
+  int L = 1;
+  for (int p = 0; p < d.npass; ++p) L = cmax(L, lanes_for_pass(d.n, d.radix[p], me));
+  d.L = L;
+  d.gpw = L <= 64 ? 64 / L : 0;
+  // compile-time specialisations (pw_fft.hip: line_static) for these radix plans
+  const int r0 = d.npass > 0 ? d.radix[0] : 0, r1 = d.npass > 1 ? d.radix[1] : 0;
+  const int r2 = d.npass > 2 ? d.radix[2] : 1;
+  d.plan = 0;
+  if (d.npass == 2 && ((d.n == 72 && r0 == 9 && r1 == 8) || (d.n == 27 && r0 == 9 && r1 == 3) ||
+                       (d.n == 45 && r0 == 9 && r1 == 5) || (d.n == 20 && r0 == 5 && r1 == 4)))
+    d.plan = d.n;
+  if (d.npass == 3 && d.n == 108 && r0 == 9 && r1 == 4 && r2 == 3) d.plan = 108;
+}
 
 // LDS strides: odd line lengths keep strided butterfly reads off a single bank group.
 __host__ __device__ inline int odd_stride(int n) { return (n % 2 == 0) ? n + 1 : n; }
