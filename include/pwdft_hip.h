@@ -60,6 +60,16 @@ const char* pw_last_error(pw_ctx* ctx);
 /* Bands processed per fused FFT launch (intermediates live in HBM/L2 per batch). */
 int pw_set_band_batch(pw_ctx* ctx, int nb);
 
+/* Tuning knobs set by the autotuner (pwdft_hip/autotune.py); no reference counterpart. */
+enum {
+  PW_TUNE_GEMM_CN_SHAPE = 0,  /* A^H B products: 0 = 64x64 tile, 1 = 128x64, 2 = 64x128, 3 = 128x128 */
+  PW_TUNE_GEMM_NN_SHAPE = 1,  /* A B products: same codes */
+  PW_TUNE_GEMM_CN_WG = 2,     /* A^H B split-K target workgroup count (0 = default 1024) */
+  PW_TUNE_GEMM_NN_WG = 3,     /* A B split-K target workgroup count */
+  PW_TUNE_BAND_BATCH = 4      /* bands per fused FFT launch (same as pw_set_band_batch) */
+};
+int pw_tune_set(pw_ctx* ctx, int key, int value);
+
 /* ---- set-up: the state op_H reads from Hamiltonian ------------------------------- */
 
 /* FFT grid of PWGrid (PWGrid.jl:31-47: Ns, CellVolume). */

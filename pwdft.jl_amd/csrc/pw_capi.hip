@@ -271,6 +271,26 @@ int pw_set_band_batch(pw_ctx* c, int nb) {
   return PW_OK;
 }
 
+int pw_tune_set(pw_ctx* c, int key, int value) {
+  CTX_CHECK(c);
+  switch (key) {
+    case PW_TUNE_GEMM_CN_SHAPE:
+    case PW_TUNE_GEMM_NN_SHAPE:
+      if (value < 0 || value > 3) return set_error(c, PW_ERR_ARG, "pw_tune_set: tile shape must be 0..3");
+      c->tune_gemm_shape[key - PW_TUNE_GEMM_CN_SHAPE] = value;
+      return PW_OK;
+    case PW_TUNE_GEMM_CN_WG:
+    case PW_TUNE_GEMM_NN_WG:
+      if (value < 0) return set_error(c, PW_ERR_ARG, "pw_tune_set: workgroup target must be >= 0");
+      c->tune_gemm_wg[key - PW_TUNE_GEMM_CN_WG] = value;
+      return PW_OK;
+    case PW_TUNE_BAND_BATCH:
+      return pw_set_band_batch(c, value);
+    default:
+      return set_error(c, PW_ERR_ARG, "pw_tune_set: unknown key");
+  }
+}
+
 int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
   CTX_CHECK(c);
   hipSetDevice(c->device);

@@ -10,11 +10,12 @@
 // At is a signed copy of one element of A, so one MFMA stream computes 8 real flops per
 // complex multiply-add with no wasted lanes.
 //
-// Tiles: a 256-thread workgroup owns 32 complex rows (64 real rows) x 64 columns of C; each
-// of its 4 waves owns 32 real rows x 32 columns as 2 x 2 MFMA 16x16 accumulators.  K is
-// consumed in steps of 16 complex (32 real) staged through LDS.  Long K (the Ngw-long
-// inner products) is split over blockIdx.z; partial tiles are reduced by a second kernel
-// in a fixed order, so results are bitwise repeatable.
+// Tiles: a 256-thread workgroup owns 64 complex rows (128 real rows) x 64 columns of C; each
+// of its 4 waves owns 64 real rows x 32 columns as 4 x 2 MFMA 16x16 accumulators.  K is
+// consumed in stages of 16 complex (32 real) staged through k-contiguous, padded LDS rows;
+// the next stage is loaded into registers while the current one feeds the MFMAs.  Long K
+// (the Ngw-long inner products) is split into chunks; partial tiles are reduced by a
+// second kernel in a fixed order, so results are bitwise repeatable.
 #include "pw_internal.h"
 
 namespace pw {
@@ -45,75 +46,123 @@ struct Mfma<float> {
   __device__ static inline int row(int i) { return i; }
 };
 
-constexpr int GBM = 32;  // complex rows of C per workgroup
-constexpr int GBN = 64;  // columns of C per workgroup
-constexpr int GBK = 16;  // complex k per LDS stage
-constexpr int GPAD = 4;
+// Workgroup of WM x WN wavefronts; each wavefront owns 64 real rows (32 complex) x 32
+// columns of C as 4 x 2 MFMA 16x16 accumulators, so the workgroup tile is
+// (32 WM complex rows) x (32 WN columns) while per-lane registers stay fixed.
+constexpr int GBK = 16;           // complex k per LDS stage (32 real k)
+constexpr int GKS = 2 * GBK + 4;  // LDS row stride (reals): k-contiguous rows, padded
 
-template <typename TC, typename TA, typename TB, int TRANSA>
-__global__ void __launch_bounds__(256)
+// Workgroup id -> (m tile, n tile, k chunk).  Consecutive workgroup ids land on different
+// XCDs (round-robin over 8), so the mapping sends every tile of one k chunk (which share
+// the chunk of A^H / B rows) to one XCD and its L2.  Without split-K the n tiles of one
+// m tile (which share the rows of A) go to one XCD.
+__device__ inline void tile_of(int w, int tm, int tn, int nz, int& mt, int& nt, int& z) {
+  const int xcd = w & 7, idx = w >> 3;
+  if (nz > 1) {
+    const int tiles = tm * tn;
+    const int zq = idx / tiles, t = idx - zq * tiles;
+    z = zq * 8 + xcd;
+    mt = t % tm;
+    nt = t / tm;
+  } else {
+    // the tn tiles of one m tile run back to back on one XCD; m tiles rotate over XCDs
+    const int mq = idx / tn;
+    nt = idx - mq * tn;
+    mt = mq * 8 + xcd;
+    z = 0;
+  }
+}
+
+template <typename TC, typename TA, typename TB, int TRANSA, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
 k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cplx<TB>* __restrict__ B,
         long ldb, int kchunk, cplx<double>* __restrict__ C, long ldc, cplx<double> alpha,
-        cplx<double> beta, int direct, int use_beta, cplx<double>* __restrict__ part) {
-  __shared__ TC As[GBK][2 * GBM + GPAD];
-  __shared__ TC Bs[2 * GBK][GBN + GPAD];
+        cplx<double> beta, int direct, int use_beta, cplx<double>* __restrict__ part, int tm, int tn,
+        int nz) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BMC = 32 * WM;   // complex rows per workgroup
+  constexpr int BNC = 32 * WN;   // columns per workgroup
+  constexpr int NA = (BMC * GBK) / NT, NB = (BNC * GBK) / NT;
+  __shared__ TC As[BMC][GKS];   // [complex row m][real k'] raw (re, im) pairs along k
+  __shared__ TC Bs[BNC][GKS];   // [column n][real k']
   typedef typename Mfma<TC>::acc_t acc_t;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN, z = blockIdx.z;
+  const int wm = wave % WM, wn = wave / WM;
+  int mt, nt, z;
+  tile_of(blockIdx.x, tm, tn, nz, mt, nt, z);
+  if (z >= nz || mt >= tm || nt >= tn) return;
+  const int m0 = mt * BMC, n0 = nt * BNC;
   const int kbeg = z * kchunk;
   const int kend = min(K, kbeg + kchunk);
-  acc_t acc[2][2];
+  acc_t acc[4][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = (TC)0;
-
   const int li = lane & 15, lg = lane >> 4;
-  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+
+  // global -> register staging (NA complex of A and NB of B per thread per stage)
+  cplx<TC> ra[NA], rb[NB];
+  auto a_idx = [&](int s, int& mm, int& kk) {
+    if (TRANSA == 0) { mm = tid % BMC; kk = tid / BMC + (NT / BMC) * s; }
+    else { kk = tid & 15; mm = (tid >> 4) + (NT / 16) * s; }
+  };
+  auto load_stage = [&](int k0) {
 #pragma unroll
-    for (int s = 0; s < (GBM * GBK) / 256; ++s) {
-      const int e = tid + 256 * s;
+    for (int s = 0; s < NA; ++s) {
       int mm, kk;
-      if (TRANSA == 0) { mm = e % GBM; kk = e / GBM; }
-      else { kk = e % GBK; mm = e / GBK; }
+      a_idx(s, mm, kk);
       const int gm = m0 + mm, gk = k0 + kk;
-      cplx<TC> v = mk((TC)0, (TC)0);
-      if (gm < M && gk < kend) v = cvt<TC>(TRANSA == 0 ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]);
-      As[kk][2 * mm] = v.x;
-      As[kk][2 * mm + 1] = v.y;
+      ra[s] = mk((TC)0, (TC)0);
+      if (gm < M && gk < kend) ra[s] = cvt<TC>(TRANSA == 0 ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]);
     }
 #pragma unroll
-    for (int s = 0; s < (GBN * GBK) / 256; ++s) {
-      const int e = tid + 256 * s;
-      const int kk = e % GBK, nn = e / GBK;
-      const int gk = k0 + kk, gn = n0 + nn;
-      cplx<TC> v = mk((TC)0, (TC)0);
-      if (gk < kend && gn < N) v = cvt<TC>(B[(long)gn * ldb + gk]);
-      Bs[2 * kk][nn] = v.x;
-      Bs[2 * kk + 1][nn] = v.y;
+    for (int s = 0; s < NB; ++s) {
+      const int kb = tid & 15, nn = (tid >> 4) + (NT / 16) * s;
+      const int gkb = k0 + kb, gn = n0 + nn;
+      rb[s] = mk((TC)0, (TC)0);
+      if (gkb < kend && gn < N) rb[s] = cvt<TC>(B[(long)gn * ldb + gkb]);
     }
+  };
+  auto store_stage = [&]() {
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      int mm, kk;
+      a_idx(s, mm, kk);
+      *reinterpret_cast<cplx<TC>*>(&As[mm][2 * kk]) = ra[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      const int kb = tid & 15, nn = (tid >> 4) + (NT / 16) * s;
+      *reinterpret_cast<cplx<TC>*>(&Bs[nn][2 * kb]) = rb[s];
+    }
+  };
+
+  if (kbeg < kend) load_stage(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+    store_stage();
     __syncthreads();
+    if (k0 + GBK < kend) load_stage(k0 + GBK);  // in flight during the MFMA phase
 #pragma unroll
     for (int q = 0; q < (2 * GBK) / 4; ++q) {
       const int kp = 4 * q + lg;
       const int kc = kp >> 1, t = kp & 1;
-      TC a[2], b[2];
+      TC a[4], b[2];
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        const int ip = wm * 32 + rt * 16 + Mfma<TC>::row(li);
-        const int s = ip & 1, ml = ip >> 1;
-        TC av = As[kc][2 * ml + (s ^ t)];
-        const bool neg = (TRANSA == 0) ? (s == 0 && t == 1) : (s == 1 && t == 0);
+      for (int rt = 0; rt < 4; ++rt) {
+        const int ip = wm * 64 + rt * 16 + Mfma<TC>::row(li);
+        const int sgn = ip & 1, ml = ip >> 1;
+        const TC av = As[ml][2 * kc + (sgn ^ t)];
+        const bool neg = (TRANSA == 0) ? (sgn == 0 && t == 1) : (sgn == 1 && t == 0);
         a[rt] = neg ? -av : av;
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) b[ct] = Bs[kp][wn * 32 + ct * 16 + li];
+      for (int ct = 0; ct < 2; ++ct) b[ct] = Bs[wn * 32 + ct * 16 + li][kp];
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
+      for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = Mfma<TC>::run(a[rt], b[ct], acc[rt][ct]);
     }
@@ -122,12 +171,12 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
   // with the row maps above, registers (0, 1) and (2, 3) of lane group lg hold the
   // (re, im) pairs of complex rows 2*lg and 2*lg + 1 of each 16-row tile.
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        const int ip = wm * 32 + rt * 16 + 4 * lg + 2 * p;
+        const int ip = wm * 64 + rt * 16 + 4 * lg + 2 * p;
         const int m = m0 + (ip >> 1);
         const int n = n0 + wn * 32 + ct * 16 + li;
         if (m < M && n < N) {
@@ -157,25 +206,27 @@ __global__ void k_zgemm_reduce(int M, int N, int nz, const cplx<double>* __restr
   }
 }
 
-template <typename TC, typename TA, typename TB>
-static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<TA>* A,
-                   long lda, const cplx<TB>* B, long ldb, cplx<double> beta, cplx<double>* C, long ldc) {
-  if (M <= 0 || N <= 0) return PW_OK;
+// Launch one (WM x WN) instance: tiles, split-K chunks, optional ordered reduction.
+template <typename TC, typename TA, typename TB, int WM, int WN>
+static int zgemm_launch(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<TA>* A,
+                        long lda, const cplx<TB>* B, long ldb, cplx<double> beta, cplx<double>* C, long ldc,
+                        int target_wg) {
+  constexpr int BMC = 32 * WM, BNC = 32 * WN;
   const int use_beta = (beta.x != 0.0 || beta.y != 0.0) ? 1 : 0;
-  const int tm = (M + GBM - 1) / GBM, tn = (N + GBN - 1) / GBN;
+  const int tm = (M + BMC - 1) / BMC, tn = (N + BNC - 1) / BNC;
   const long tiles = (long)tm * tn;
-  int nz = 1;
   const int ksteps = (K + GBK - 1) / GBK;
-  if (tiles < 1024 && ksteps > 8) {
-    nz = (int)std::min<long>((1024 + tiles - 1) / tiles, ksteps / 8);
-    if (nz < 1) nz = 1;
+  int nz = 1;
+  if (tiles < target_wg && ksteps > 8) {
+    nz = (int)std::min<long>((target_wg + tiles - 1) / tiles, ksteps / 8);
+    nz = std::max(8, (nz + 7) / 8 * 8);  // whole rounds over the 8 XCDs
   }
-  int kchunk = ((ksteps + nz - 1) / nz) * GBK;
-  if (K <= 0) {
-    kchunk = GBK;
-    nz = 1;
-  } else {
+  int kchunk = GBK;
+  if (K > 0) {
+    kchunk = ((ksteps + nz - 1) / nz) * GBK;
     nz = (K + kchunk - 1) / kchunk;
+  } else {
+    nz = 1;
   }
   cplx<double>* part = nullptr;
   if (nz > 1) {
@@ -183,18 +234,18 @@ static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alph
     if (rc) return rc;
     part = reinterpret_cast<cplx<double>*>(c->d_gpart);
   }
-  dim3 grid(tm, tn, nz);
   const int direct = nz == 1 ? 1 : 0;
-  // algorithmic bytes: A and B read once, C written (and read when beta != 0)
+  // ids past the work (the last partial round of 8) return at once
+  const long nwg = nz > 1 ? ((nz + 7) / 8) * 8 * tiles : (long)8 * ((tm + 7) / 8) * tn;
   const double gbytes = (double)M * K * sizeof(cplx<TA>) + (double)K * N * sizeof(cplx<TB>) +
                         (double)M * N * 16.0 * (use_beta ? 2 : 1);
   ProfScope ps(c, transa ? PW_PROF_GEMM_CN : PW_PROF_GEMM_NN, gbytes);
   if (transa)
-    hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 1>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, B, ldb,
-                       kchunk, C, ldc, alpha, beta, direct, use_beta, part);
+    hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 1, WM, WN>), dim3(nwg), dim3(64 * WM * WN), 0, c->stream, M, N, K, A,
+                       lda, B, ldb, kchunk, C, ldc, alpha, beta, direct, use_beta, part, tm, tn, nz);
   else
-    hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 0>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, B, ldb,
-                       kchunk, C, ldc, alpha, beta, direct, use_beta, part);
+    hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 0, WM, WN>), dim3(nwg), dim3(64 * WM * WN), 0, c->stream, M, N, K, A,
+                       lda, B, ldb, kchunk, C, ldc, alpha, beta, direct, use_beta, part, tm, tn, nz);
   if (nz > 1) {
     const long tot = (long)M * N;
     const int blocks = (int)std::min<long>((tot + 255) / 256, 2048);
@@ -202,6 +253,25 @@ static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alph
                        alpha, beta, use_beta);
   }
   return hip_check(c, hipGetLastError(), "zgemm");
+}
+
+// Tile choice: big workgroup tiles cut the re-reads of A and B from HBM (each element is
+// read once per tile along the other dimension); small problems keep 64 x 64 tiles so the
+// grid still fills the chip.
+template <typename TC, typename TA, typename TB>
+static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<TA>* A,
+                   long lda, const cplx<TB>* B, long ldb, cplx<double> beta, cplx<double>* C, long ldc) {
+  if (M <= 0 || N <= 0) return PW_OK;
+  // tuning knobs (pw_tune_set): tile shape and the split-K target workgroup count
+  const int slot = transa ? 0 : 1;
+  const int shape = c->tune_gemm_shape[slot];
+  const int target = c->tune_gemm_wg[slot] > 0 ? c->tune_gemm_wg[slot] : 1024;
+  switch (shape) {
+    case 1: return zgemm_launch<TC, TA, TB, 4, 2>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+    case 2: return zgemm_launch<TC, TA, TB, 2, 4>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+    case 3: return zgemm_launch<TC, TA, TB, 4, 4>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+    default: return zgemm_launch<TC, TA, TB, 2, 2>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+  }
 }
 
 int zgemm(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alpha, const cplx<double>* A, long lda,

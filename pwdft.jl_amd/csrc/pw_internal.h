@@ -126,6 +126,9 @@ struct pw_ctx {
   void* d_tmp = nullptr;   size_t tmp_bytes = 0;      // staging for host-pointer entry points
   void* d_wts = nullptr;   size_t wts_bytes = 0;      // band weights of calc_rhoe
   int band_batch = 64;
+  // tuning (pw_tune_set): [0] = A^H B products, [1] = A B products
+  int tune_gemm_shape[2] = {0, 0};  // 0: 2x2 wavefronts (64x64), 1: 4x2, 2: 2x4, 3: 4x4
+  int tune_gemm_wg[2] = {0, 0};     // split-K target workgroups (0 = 1024)
   std::string err;
   // per-kernel timing (pw_profile_*): event pairs recorded around launches on `stream`
   bool profile = false;

@@ -34,6 +34,7 @@ _PROTOS = {
     "pw_ctx_create": [_c.POINTER(_vp), _i],
     "pw_ctx_destroy": [_vp],
     "pw_set_band_batch": [_vp, _i],
+    "pw_tune_set": [_vp, _i, _i],
     "pw_set_grid": [_vp, _i, _i, _i, _d],
     "pw_set_kpoint": [_vp, _i, _i, _vp, _i, _vp],
     "pw_set_vloc": [_vp, _i, _vp],
