@@ -9,6 +9,9 @@ the Gaussian atomic-superposition density.  Inputs are resident in HBM before ti
 A step is one op_H over all 160 bands (a band-application is one column through op_H).
 `value` is the mixed-precision path (FP32 FFT + FP32-MFMA projectors); the FP64 path is
 timed beside it, and dE/atom between the two (sum_b <psi_b|H|psi_b> / Natoms) is reported.
+Beside the op_H line: diag_LOBPCG! band-applications/s over a fixed number of iterations
+(op_H plus the Rayleigh-Ritz products) and KS_solve_SCF! seconds per iteration, each in
+FP64 and in mixed precision.
 """
 import argparse
 import json
@@ -115,6 +118,38 @@ def time_op_H(H, X, Y, steps, warmup, mixed, profile=False):
     return ms.value / 1e3, wall, prof
 
 
+def time_lobpcg(H, X0, iters, mixed):
+    """diag_LOBPCG! on the Si64 bands for a fixed number of iterations (tol = 0, so no band
+    converges or locks) from the same orthonormal start; returns band-applications/s
+    (op_H columns: the initial HX plus Nst per iteration) and seconds per iteration."""
+    import pwdft_hip as ph
+    from pwdft_hip.solvers import diag_LOBPCG_
+    X = ph.DeviceMatrix(H.ctx, X0.rows, X0.cols)
+    H.ctx.call("pw_copy_cols_dev", X0.rows, X0.cols, X0.p, X0.rows, X.p, X.rows)
+    H.ctx.call("pw_synchronize")
+    H.mixed = mixed
+    H.ik = 0
+    t0 = time.perf_counter()
+    _, st = diag_LOBPCG_(H, X, tol=0.0, NiterMax=iters)
+    H.ctx.call("pw_synchronize")
+    dt = time.perf_counter() - t0
+    H.mixed = False
+    X.free()
+    return {"band_apps_per_s": st["ncols"] / dt, "s_per_iter": dt / st["iter"], "iterations": st["iter"],
+            "band_apps": st["ncols"]}
+
+
+def time_scf(H, iters, mixed):
+    """KS_solve_SCF! iterations on Si64 (LOBPCG + calc_rhoe + potential update + energies),
+    seconds per SCF iteration after the initial guess."""
+    import pwdft_hip as ph
+    H.mixed = mixed
+    out = ph.KS_solve_SCF_(H, betamix=0.5, NiterMax=iters, etot_conv_thr=0.0)
+    H.mixed = False
+    return {"s_per_iter": out["seconds"] / out["iterations"], "iterations": out["iterations"],
+            "band_apps_per_s": out["ncols"] / out["seconds"], "Etot": out["energies"]["Total"]}
+
+
 def roofline(prof, H, mixed):
     """Roofline object for the kernel with the largest total time in one op_H step."""
     name = max(prof, key=lambda k: prof[k][0])
@@ -156,6 +191,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
+    ap.add_argument("--solver-iters", type=int, default=4,
+                    help="LOBPCG iterations timed per precision (0 = skip)")
+    ap.add_argument("--scf-iters", type=int, default=3, help="SCF iterations timed per precision (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
@@ -192,6 +230,20 @@ def main():
     t_mix = res["mixed"]["sec"] / args.steps
     t_64 = res["fp64"]["sec"] / args.steps
     value = NBANDS / t_mix
+    lob = {}
+    scf = {}
+    if args.solver_iters > 0:
+        for mixed in (False, True):
+            tag = "mixed" if mixed else "fp64"
+            lob[tag] = time_lobpcg(H, X, args.solver_iters, mixed)
+            print(f"# LOBPCG {tag}: {lob[tag]['s_per_iter'] * 1e3:.2f} ms/iteration, "
+                  f"{lob[tag]['band_apps_per_s']:.0f} band-applications/s", file=sys.stderr, flush=True)
+        lob["mixed_over_fp64"] = lob["mixed"]["band_apps_per_s"] / lob["fp64"]["band_apps_per_s"]
+    if args.scf_iters > 0:
+        for mixed in (False, True):
+            tag = "mixed" if mixed else "fp64"
+            scf[tag] = time_scf(H, args.scf_iters, mixed)
+            print(f"# SCF {tag}: {scf[tag]['s_per_iter']:.3f} s/iteration", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "Hpsi band-applications/s (mixed precision; FP64 beside it)",
@@ -213,6 +265,8 @@ def main():
         "dE_per_atom_mixed_vs_fp64": dE,
         "roofline": roofline(res["mixed"]["prof"], H, True),
         "roofline_fp64": roofline(res["fp64"]["prof"], H, False),
+        "lobpcg": lob or None,
+        "scf_s_per_iter": scf or None,
         "kernels_us_per_step": {t: {k: v[0] * 1e3 / args.steps for k, v in res[t]["prof"].items() if v[1]}
                                 for t in ("fp64", "mixed")},
         "cpu_baseline": cpu,

@@ -119,6 +119,45 @@ int pw_rhoe_accum_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldps
                       unsigned flags);
 int pw_rhoe_get(pw_ctx* ctx, double* rho);
 
+/* ---- forces and energy gradients (the adjoint of the pseudopotential energies) ---- */
+
+/* Atoms for the force kernels: positions tau (3 x natoms, bohr, column-major), 0-based
+ * species per atom, and the atom of every projector (beta_atom[nbeta], 0-based; the
+ * (atom, l, m, iprj) ordering of PsPotNL.jl:45-57).  Call again after atoms move
+ * (together with pw_set_projectors, whose betaNL depends on tau). */
+int pw_set_atoms(pw_ctx* ctx, int natoms, const double* tau, const int* species, int nbeta,
+                 const int* beta_atom);
+
+/* The rho-sphere G-vectors (pw.gvec: idx_g2r minus base, G as 3 x ng) and the local
+ * pseudopotential form factors vg[ng x nsp] = V_sp(|G|) as eval_Vloc_G! returns them
+ * (PsPot_GTH.jl:164-189, no 1/Omega), one column per species. */
+int pw_set_rho_sphere(pw_ctx* ctx, int ng, const int64_t* idx, int base, const double* G, int nsp,
+                      const double* vg);
+
+/* E_Ps_loc = sum_G sum_sp S_sp(G) V_sp(G) conj(rho(G)) with rho(G) = FFT[rho] / N and
+ * S_sp(G) = sum_{a in sp} exp(-i G.tau_a): the energy of calc_E_Ps_loc (calc_energies.jl)
+ * written on the rho sphere, and its gradients at fixed rho
+ *   dE_dvg[g + ng*sp] = d E_Ps_loc / d V_sp(G_g) = Re(S_sp(G) conj(rho(G))),
+ *   dE_dtau[3a + x]   = d E_Ps_loc / d tau_a,x   = -F_Ps_loc (calc_forces_Ps_loc.jl:13-71).
+ * rho: host array of N doubles (spin-summed), or NULL for the density accumulated on the
+ * device by pw_rhoe_accum*.  Any output pointer may be NULL.  FP64. */
+int pw_E_Ps_loc_grad(pw_ctx* ctx, const double* rho, double* E, double* dE_dvg, double* dE_dtau);
+
+/* calc_forces_Ps_loc(Ham, Rhoe) (calc_forces_Ps_loc.jl:1-71): F (3 x natoms) overwritten. */
+int pw_calc_forces_Ps_loc(pw_ctx* ctx, const double* rho, double* F);
+
+/* The GTH nonlocal force of one k-point (calc_forces_Ps_nloc.jl:291-343 with calc_dbetaNL
+ * :347-391 and calc_dbetaNL_psi :393-411), added into F (3 x natoms):
+ *   F[:, a] += 2 sum_ist w[ist] sum_{i,j in a} h_ij Re(conj(<beta_i|psi>) <i(G+k) beta_j|psi>),
+ * w[ist] = wk[ik] * Focc[ist, ik] (host array).  When E is not NULL the band energy
+ * sum_ist w[ist] sum_ij h_ij conj(<beta_i|psi>) <beta_j|psi> (calc_E_Ps_nloc) is added to
+ * *E.  Device pointers for psi; FP64. */
+int pw_calc_forces_Ps_nloc_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi,
+                               const double* w, double* F, double* E);
+/* Same with psi in host memory. */
+int pw_calc_forces_Ps_nloc(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi,
+                           const double* w, double* F, double* E);
+
 /* ---- dense products of diag_LOBPCG! / diag_davidson! ------------------------------ */
 
 /* C = alpha * op(A) * B + beta * C with op(A) = A (transa 0, A m x k) or A^H (transa 1,

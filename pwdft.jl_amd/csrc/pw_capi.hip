@@ -66,7 +66,7 @@ static bool factorize(int n, FftDim& d) {
   return true;
 }
 
-static void free_k(KData& k) {
+void free_k(KData& k) {
   hipFree(k.d_ent_igw);
   hipFree(k.d_ent_pos);
   hipFree(k.d_col_ptr);
@@ -82,7 +82,7 @@ static void free_k(KData& k) {
 }
 
 template <typename T>
-static int upload(pw_ctx* c, T** dst, const T* src, size_t n) {
+int upload(pw_ctx* c, T** dst, const T* src, size_t n) {
   if (*dst) {
     hipFree(*dst);
     *dst = nullptr;
@@ -255,6 +255,7 @@ int pw_ctx_destroy(pw_ctx* c) {
   hipFree(c->d_rho);
   hipFree(c->d_tmp);
   hipFree(c->d_wts);
+  free_forces(c);
   drain_events(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
@@ -306,6 +307,8 @@ int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
   }
   for (auto& k : c->kd) free_k(k);
   c->kd.clear();
+  free_k(c->krho);
+  c->krho_g0 = -1;
   c->n1 = n1;
   c->n2 = n2;
   c->n3 = n3;
@@ -334,13 +337,11 @@ int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
   return hip_check(c, e, "memset rho");
 }
 
-int pw_set_kpoint(pw_ctx* c, int ik, int ngw, const int64_t* idx, int base, const double* gk) {
-  CTX_CHECK(c);
-  hipSetDevice(c->device);
-  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid must precede pw_set_kpoint");
-  if (ik < 0 || ngw <= 0 || !idx || !gk) return set_error(c, PW_ERR_ARG, "pw_set_kpoint: bad arguments");
-  if ((int)c->kd.size() <= ik) c->kd.resize(ik + 1);
-  KData& k = c->kd[ik];
+}  // extern "C"
+
+namespace pw {
+
+int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, const double* gk, const char* who) {
   free_k(k);
   const int n1 = c->n1, n2 = c->n2, n3 = c->n3;
   struct Ent { int col; int z; int igw; };
@@ -349,8 +350,8 @@ int pw_set_kpoint(pw_ctx* c, int ik, int ngw, const int64_t* idx, int base, cons
   std::vector<char> seen(c->N, 0);
   for (int g = 0; g < ngw; ++g) {
     const int64_t ip = idx[g] - base;
-    if (ip < 0 || ip >= c->N) return set_error(c, PW_ERR_ARG, "pw_set_kpoint: index out of grid");
-    if (seen[ip]) return set_error(c, PW_ERR_ARG, "pw_set_kpoint: duplicate grid index");
+    if (ip < 0 || ip >= c->N) return set_error(c, PW_ERR_ARG, std::string(who) + ": index out of grid");
+    if (seen[ip]) return set_error(c, PW_ERR_ARG, std::string(who) + ": duplicate grid index");
     seen[ip] = 1;
     const int i = (int)(ip % n1), j = (int)((ip / n1) % n2), z = (int)(ip / ((int64_t)n1 * n2));
     keys[g] = i + n1 * j;
@@ -401,6 +402,22 @@ int pw_set_kpoint(pw_ctx* c, int ik, int ngw, const int64_t* idx, int base, cons
   return PW_OK;
 }
 
+template int upload<double>(pw_ctx*, double**, const double*, size_t);
+template int upload<int>(pw_ctx*, int**, const int*, size_t);
+
+}  // namespace pw
+
+extern "C" {
+
+int pw_set_kpoint(pw_ctx* c, int ik, int ngw, const int64_t* idx, int base, const double* gk) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid must precede pw_set_kpoint");
+  if (ik < 0 || ngw <= 0 || !idx || !gk) return set_error(c, PW_ERR_ARG, "pw_set_kpoint: bad arguments");
+  if ((int)c->kd.size() <= ik) c->kd.resize(ik + 1);
+  return build_kdata(c, c->kd[ik], ngw, idx, base, gk, "pw_set_kpoint");
+}
+
 int pw_set_vloc(pw_ctx* c, int ispin, const double* V) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
@@ -428,6 +445,7 @@ int pw_set_dij(pw_ctx* c, int nbeta, const double* D) {
   if (nbeta < 0 || (nbeta > 0 && !D)) return set_error(c, PW_ERR_ARG, "pw_set_dij: bad arguments");
   c->nbeta = nbeta;
   c->h_D.assign(D, D + (size_t)nbeta * nbeta);
+  c->dcsr_valid = false;
   return PW_OK;
 }
 

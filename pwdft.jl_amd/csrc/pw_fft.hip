@@ -336,19 +336,34 @@ __device__ inline void stage(T* dst, const T* __restrict__ src, int n) {
 
 // ----------------------------------------------------------------------------- kernels
 
+// Column kernels run on a 1-D grid of nchunk * roundup(nbb, 8) workgroups.  Workgroups are
+// dispatched round-robin over the 8 XCDs (w mod 8), so block w is given band
+// (w mod 8) + 8 * ((w / 8) / nchunk) and column chunk (w / 8) mod nchunk: every chunk of a band
+// then runs on one XCD and the scattered psi / Hpsi rows of that band share one L2 instead
+// of being fetched into all eight.  Blocks past the last band return at once.
+__device__ inline bool col_block(int ncol, int C, int nbb, int& c0, int& bl) {
+  const int nchunk = (ncol + C - 1) / C;
+  const int w = blockIdx.x, s = w >> 3;
+  bl = (w & 7) + 8 * (s / nchunk);
+  c0 = (s % nchunk) * C;
+  return bl < nbb;
+}
+
+static inline int col_grid(int ncol, int C, int nbb) { return ((ncol + C - 1) / C) * ((nbb + 7) / 8 * 8); }
+
 // 1. scatter + inverse z-FFT of the columns [c0, c0+C) of band (b0 + blockIdx.y).
 template <typename T>
 __global__ void __launch_bounds__(COL_NT)
 k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __restrict__ W,
            const int* __restrict__ ent_igw, const int* __restrict__ ent_pos,
-           const int* __restrict__ col_ptr, int ncol, int C, FftDim dz,
+           const int* __restrict__ col_ptr, int ncol, int C, int nbb, FftDim dz,
            const cplx<T>* __restrict__ twz) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int n3 = dz.n, cs = odd_stride(n3);
   cplx<T>* tw = lds + C * cs;
-  const int bl = blockIdx.y;
-  const int c0 = blockIdx.x * C;
+  int bl, c0;
+  if (!col_block(ncol, C, nbb, c0, bl)) return;
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < cn * cs; i += nthr) lds[i] = mk((T)0, (T)0);
@@ -417,15 +432,15 @@ template <typename T>
 __global__ void __launch_bounds__(COL_NT)
 k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, long ldpsi,
           cplx<double>* __restrict__ hpsi, long ldh, int b0, const int* __restrict__ ent_igw,
-          const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol, int C,
+          const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol, int C, int nbb,
           FftDim dz, const cplx<T>* __restrict__ twz, const double* __restrict__ gk2, int kin,
           int acc) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int n3 = dz.n, cs = odd_stride(n3);
   cplx<T>* tw = lds + C * cs;
-  const int bl = blockIdx.y;
-  const int c0 = blockIdx.x * C;
+  int bl, c0;
+  if (!col_block(ncol, C, nbb, c0, bl)) return;
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
   const cplx<T>* w = W + (long)bl * n3 * ncol;
@@ -543,12 +558,10 @@ static size_t plane_lds(const pw_ctx* c, size_t esz, int nxact) {
 static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
 template <typename T>
-static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
+static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>* psi, long ldpsi,
                     cplx<double>* hpsi, long ldh, int kin, int acc) {
-  KData& k = c->kd[ik];
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
-  const int nchunk = (k.ncol + C - 1) / C;
   const size_t plane_bytes = plane_lds(c, esz, k.nxact);
   if (plane_bytes > 160 * 1024)
     return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
@@ -562,18 +575,16 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
   const cplx<T>* tw[3];
   for (int d = 0; d < 3; ++d)
     tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
-  const T* V = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_v64[ispin])
-                              : reinterpret_cast<const T*>(c->d_v32[ispin]);
   for (int b0 = 0; b0 < nb; b0 += bb) {
     const int nbb = std::min(bb, nb - b0);
-    dim3 g1(nchunk, nbb);
+    dim3 g1(col_grid(k.ncol, C, nbb));
     // algorithmic HBM bytes: psi in (16 B/coef) + index maps, work array out
     const double wb = (double)nbb * c->n3 * k.ncol * esz;
     const double psib = (double)nbb * k.ngw * 16.0;
     {
       ProfScope ps(c, PW_PROF_COL_IFFT, psib + wb + k.ngw * 8.0);
       hipLaunchKernelGGL(k_col_ifft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, psi, ldpsi, b0,
-                         W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2], tw[2]);
+                         W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb, c->dim[2], tw[2]);
     }
     {
       dim3 g2(c->n3, nbb);
@@ -584,7 +595,7 @@ static int vloc_run(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* ps
     {
       ProfScope ps(c, PW_PROF_COL_FFT, wb + psib * ((kin ? 1 : 0) + (acc ? 1 : 0) + 1) + k.ngw * 16.0);
       hipLaunchKernelGGL(k_col_fft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, W, psi, ldpsi,
-                         hpsi, ldh, b0, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
+                         hpsi, ldh, b0, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb, c->dim[2],
                          tw[2], k.d_gk2, kin, acc);
     }
   }
@@ -595,8 +606,13 @@ int vloc_apply(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, lo
                cplx<double>* hpsi, long ldh, unsigned flags) {
   const int kin = (flags & PW_TERM_K) ? 1 : 0;
   const int acc = (flags & PW_ACCUMULATE) ? 1 : 0;
-  if (flags & PW_MIXED) return vloc_run<float>(c, ik, ispin, nb, psi, ldpsi, hpsi, ldh, kin, acc);
-  return vloc_run<double>(c, ik, ispin, nb, psi, ldpsi, hpsi, ldh, kin, acc);
+  if (flags & PW_MIXED) return vloc_run<float>(c, c->kd[ik], c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
+  return vloc_run<double>(c, c->kd[ik], c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
+}
+
+int vloc_apply_k(pw_ctx* c, KData& k, const double* V64, int nb, const cplx<double>* psi, long ldpsi,
+                 cplx<double>* out, long ldout) {
+  return vloc_run<double>(c, k, V64, nb, psi, ldpsi, out, ldout, 0, 0);
 }
 
 int kinetic_apply(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,
@@ -624,7 +640,6 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   KData& k = c->kd[ik];
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
-  const int nchunk = (k.ncol + C - 1) / C;
   const int np = c->n1 * c->n2;
   const size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
   if (plane_bytes > 160 * 1024)
@@ -646,9 +661,9 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
     const int ng = std::max(1, std::min(nbb, ngroup));
     const int bpg = (nbb + ng - 1) / ng;
     const int ngu = (nbb + bpg - 1) / bpg;
-    hipLaunchKernelGGL(k_col_ifft<T>, dim3(nchunk, nbb), dim3(COL_NT), col_lds(c, C, esz), c->stream,
-                       psi, ldpsi, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, c->dim[2],
-                       tw[2]);
+    hipLaunchKernelGGL(k_col_ifft<T>, dim3(col_grid(k.ncol, C, nbb)), dim3(COL_NT), col_lds(c, C, esz),
+                       c->stream, psi, ldpsi, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb,
+                       c->dim[2], tw[2]);
     ProfScope ps(c, PW_PROF_PLANE_RHOE, (double)nbb * c->n3 * k.ncol * esz + (double)ngu * c->N * 8.0);
     hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
                        d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,

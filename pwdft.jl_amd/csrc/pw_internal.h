@@ -137,6 +137,25 @@ struct pw_ctx {
   double prof_ms[PW_PROF_SLOTS] = {0};
   long prof_count[PW_PROF_SLOTS] = {0};
   double prof_bytes[PW_PROF_SLOTS] = {0};
+  // forces and energy gradients (pw_forces.hip)
+  pw::KData krho;                      // rho-sphere G-vectors (pw_set_rho_sphere)
+  int krho_g0 = -1;                    // sphere entry of G = 0
+  int nsp_loc = 0;
+  double* d_vg = nullptr;              // [ng * nsp] V_sp(|G|), column-major
+  int natoms = 0;
+  int species_max = -1;                // largest species index of pw_set_atoms
+  int nbeta_atoms = 0;                 // length of beta_atom of pw_set_atoms
+  double* d_tau = nullptr;             // [3 * natoms]
+  int* d_species = nullptr;            // [natoms]
+  int* d_beta_atom = nullptr;          // [nbeta]
+  bool dcsr_valid = false;             // D columns on the device match h_D
+  int* d_dcol_ptr = nullptr;           // [nbeta + 1]  non-zeros of column j of D
+  int* d_drow = nullptr;               // row index of each non-zero
+  double* d_dval = nullptr;            // value of each non-zero
+  void* d_fbeta = nullptr; size_t fbeta_bytes = 0;  // (G+k)_alpha * beta, alpha = x, y, z
+  void* d_fmat = nullptr;  size_t fmat_bytes = 0;   // beta^H psi and ((G+k)_alpha beta)^H psi
+  void* d_fgrid = nullptr; size_t fgrid_bytes = 0;  // rho / N on the grid, delta_G0, rho(G)
+  void* d_fpart = nullptr; size_t fpart_bytes = 0;  // per-block partial sums
 };
 
 namespace pw {
@@ -164,6 +183,16 @@ int rhoe_accum(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, c
                unsigned flags);
 int kprec_apply(pw_ctx* c, int ik, int nb, const cplx<double>* in, long ldin, cplx<double>* out,
                 long ldout);
+// out = FFT[ V(r) * IFFT[psi] ] restricted to the sphere k (FP64, no kinetic term, overwrite);
+// V64 holds V(r) / N, the convention of d_v64.
+int vloc_apply_k(pw_ctx* c, KData& k, const double* V64, int nb, const cplx<double>* psi, long ldpsi,
+                 cplx<double>* out, long ldout);
+// Sphere bookkeeping for the index list idx (grid offsets, minus base) with vectors gk (3 x ngw).
+int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, const double* gk, const char* who);
+void free_k(KData& k);
+template <typename T>
+int upload(pw_ctx* c, T** dst, const T* src, size_t n);
+void free_forces(pw_ctx* c);
 
 // GEMM drivers (pw_gemm.hip)
 // C (M x N) = alpha * op(A) * B + beta * C ; op(A) = A (transa = 0, A is M x K) or

@@ -12,9 +12,12 @@ from .hamiltonian import (Hamiltonian, Kprec, calc_betaNL_psi, calc_rhoe, calc_r
 from .solvers import (diag_LOBPCG, diag_LOBPCG_, diag_davidson, diag_davidson_, ortho_sqrt_,
                       rand_BlochWavefunc)
 from .scf import KS_solve_SCF_, calc_energies
+from .forces import (E_Ps_loc_grad, calc_forces, calc_forces_NN, calc_forces_Ps_loc,
+                     calc_forces_Ps_nloc, rrule_E_Ps_loc, rrule_E_Ps_nloc)
 
 __all__ = [
-    "Atoms", "Context", "DeviceMatrix", "Hamiltonian", "KS_solve_SCF_", "Kprec", "PWError", "PWGrid",
+    "Atoms", "Context", "E_Ps_loc_grad", "calc_forces", "calc_forces_NN", "calc_forces_Ps_loc",
+    "calc_forces_Ps_nloc", "rrule_E_Ps_loc", "rrule_E_Ps_nloc", "DeviceMatrix", "Hamiltonian", "KS_solve_SCF_", "Kprec", "PWError", "PWGrid",
     "PosDefException", "PsPot_GTH", "PsPotNL", "calc_betaNL_psi", "calc_energies", "calc_rhoe",
     "calc_rhoe_", "diag_LOBPCG", "diag_LOBPCG_", "diag_davidson", "diag_davidson_", "diamond_supercell",
     "gen_lattice_fcc", "gen_lattice_sc", "kpoint_grid", "load", "op_H", "op_H_", "op_K", "op_K_",

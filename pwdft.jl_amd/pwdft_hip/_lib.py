@@ -48,6 +48,12 @@ _PROTOS = {
     "pw_rhoe_accum": [_vp, _i, _i, _vp, _i64, _vp, _c.c_uint],
     "pw_rhoe_accum_dev": [_vp, _i, _i, _vp, _i64, _vp, _c.c_uint],
     "pw_rhoe_get": [_vp, _vp],
+    "pw_set_atoms": [_vp, _i, _vp, _vp, _i, _vp],
+    "pw_set_rho_sphere": [_vp, _i, _vp, _i, _vp, _i, _vp],
+    "pw_E_Ps_loc_grad": [_vp, _vp, _vp, _vp, _vp],
+    "pw_calc_forces_Ps_loc": [_vp, _vp, _vp],
+    "pw_calc_forces_Ps_nloc_dev": [_vp, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    "pw_calc_forces_Ps_nloc": [_vp, _i, _i, _vp, _i64, _vp, _vp, _vp],
     "pw_zgemm_dev": [_vp, _i, _i, _i, _i, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _c.c_uint],
     "pw_malloc": [_vp, _c.c_size_t, _c.POINTER(_vp)],
     "pw_free": [_vp, _vp],

@@ -64,6 +64,43 @@ def good_fft_order(nr):
             raise ValueError("fft_order is too large")
 
 
+def crystal_symmetry(atoms, tol=1e-5):
+    """Space-group operations of the crystal, as SymmetryInfo(atoms) finds them
+    (SymmetryInfo.jl:711-770 via find_symm_bravais_latt + sgam_at): the rotations of the
+    Bravais lattice (integer matrices R on crystal coordinates with R^T g R = g, g = L^T L)
+    that map every atom onto an atom of the same species after some fractional translation
+    t.  Returns [(R, t)] with the identity first; R acts on direct crystal coordinates, and
+    the matrix acting on crystal coordinates of k is inv(R)^T."""
+    L = np.asarray(atoms.LatVecs, dtype=float)
+    g = L.T @ L
+    frac = np.linalg.solve(L, atoms.positions)
+    sp = np.asarray(atoms.atm2species)
+    ops = []
+    scale = np.abs(g).max()
+    for e in np.ndindex(*(3,) * 9):
+        R = np.array(e, dtype=np.int64).reshape(3, 3) - 1
+        if round(abs(np.linalg.det(R))) != 1:
+            continue
+        if np.abs(R.T @ g @ R - g).max() > tol * scale:
+            continue
+        Rf = R @ frac
+        for b in np.flatnonzero(sp == sp[0]):
+            t = frac[:, b] - Rf[:, 0]
+            d = (Rf + t[:, None])[:, :, None] - frac[:, None, :]        # 3 x Natoms x Natoms
+            d -= np.round(d)
+            hit = (np.abs(d).max(axis=0) < tol) & (sp[:, None] == sp[None, :])
+            if np.all(hit.any(axis=1)):
+                ops.append((R, t - np.round(t)))
+                break
+    ops.sort(key=lambda o: 0 if np.array_equal(o[0], np.eye(3, dtype=np.int64)) else 1)
+    return ops
+
+
+def kpoint_rotations(ops):
+    """Matrices acting on crystal coordinates of k for the operations of crystal_symmetry."""
+    return [np.rint(np.linalg.inv(R).T).astype(np.int64) for R, _ in ops]
+
+
 def kpoint_grid(LatVecs, mesh, shift, rots=None, time_reversal=True):
     """KPoints.jl:121-259 (QE kpoint_grid): Monkhorst-Pack grid reduced by the integer
     rotations `rots` (crystal coordinates of k) and time reversal.  -> (k_cart, wk)."""
@@ -305,3 +342,60 @@ def V_Ps_loc(atoms, pw, pspots):
         Vg[pw.idx_g2r] += Sf[:, isp] * Vgl[pw.idx_g2shells] / pw.CellVolume
     f = np.fft.ifftn(Vg.reshape(pw.Ns, order="F")).reshape(-1, order="F")
     return np.real(f) * pw.Npoints
+
+
+class RhoeSymmetrizer:
+    """symmetrize_rhoe! (RhoeSymmetrizer.jl:111-236) for the operations of crystal_symmetry:
+    rho(G) on the rho sphere is replaced by its average over the operations {R|t},
+        rho_sym(m) = 1/Nsym sum_s rho(R_s^-T m) exp(2 pi i (R_s^-T m).t_s),
+    m = integer crystal coordinates of G; grid components outside the sphere are kept, as
+    the reference keeps them.  Host-side, once per density (O(Nsym Ng))."""
+
+    def __init__(self, atoms, pw, ops):
+        self.pw = pw
+        self.Nsyms = len(ops)
+        m = np.rint(np.asarray(atoms.LatVecs, dtype=float).T @ pw.G / (2 * np.pi)).astype(np.int64)
+        lo = m.min(axis=1)
+        span = m.max(axis=1) - lo + 1
+        table = np.full(tuple(span), -1, dtype=np.int64)
+        table[tuple(m - lo[:, None])] = np.arange(pw.Ng)
+        self.idx = np.empty((self.Nsyms, pw.Ng), dtype=np.int64)
+        self.phase = np.empty((self.Nsyms, pw.Ng), dtype=complex)
+        for s, (R, t) in enumerate(ops):
+            ms = np.rint(np.linalg.inv(R).T).astype(np.int64) @ m
+            rel = ms - lo[:, None]
+            if (rel < 0).any() or (rel >= span[:, None]).any():
+                raise ValueError("RhoeSymmetrizer: the G sphere is not closed under the operations")
+            j = table[tuple(rel)]
+            if (j < 0).any():
+                raise ValueError("RhoeSymmetrizer: the G sphere is not closed under the operations")
+            self.idx[s] = j
+            self.phase[s] = np.exp(2j * np.pi * (t @ ms))
+
+    def __call__(self, rho):
+        pw = self.pw
+        r = np.asarray(rho, dtype=float).reshape(-1)
+        f = np.fft.fftn(r.reshape(pw.Ns, order="F")).reshape(-1, order="F")
+        v = f[pw.idx_g2r]
+        f[pw.idx_g2r] = np.mean(v[self.idx] * self.phase, axis=0)
+        return np.real(np.fft.ifftn(f.reshape(pw.Ns, order="F")).reshape(-1, order="F"))
+
+
+def symmetrize_vector(atoms, ops, v):
+    """symmetrize_vector! (SymmetryInfo.jl:923-967) for a per-atom Cartesian vector field
+    v (3 x Natoms): v_sym[a] = 1/Nsym sum_s C_s^-1 v[g_s(a)], with C_s = L R_s L^-1 and
+    g_s(a) the atom that {R_s|t_s} carries atom a onto."""
+    if len(ops) <= 1:
+        return np.array(v, dtype=float)
+    L = np.asarray(atoms.LatVecs, dtype=float)
+    Li = np.linalg.inv(L)
+    frac = Li @ atoms.positions
+    v = np.asarray(v, dtype=float)
+    out = np.zeros_like(v)
+    for R, t in ops:
+        C = L @ R @ Li
+        d = (R @ frac + t[:, None])[:, :, None] - frac[:, None, :]
+        d -= np.round(d)
+        g = np.argmin(np.abs(d).max(axis=0), axis=1)
+        out += np.linalg.solve(C, v[:, g])
+    return out / len(ops)

@@ -21,7 +21,8 @@ import numpy as np
 from . import _lib
 from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC,
                    Context, DeviceMatrix, ptr)
-from .basis import PsPot_GTH, PsPotNL, PWGrid, V_Ps_loc, kpoint_grid
+from .basis import (PsPot_GTH, PsPotNL, PWGrid, RhoeSymmetrizer, V_Ps_loc, crystal_symmetry,
+                    eval_Vloc_G, kpoint_grid, kpoint_rotations)
 from .potentials import calc_E_NN, epsxc_vxc_vwn, guess_rhoe, poisson_solve
 
 
@@ -71,7 +72,8 @@ class Hamiltonian:
 
     def __init__(self, atoms, pspfiles, ecutwfc, meshk=(1, 1, 1), shiftk=(0, 0, 0), Ns_=None,
                  kpoints=None, rots=None, extra_states=None, Nstates=None, use_smearing=False,
-                 smearing_kT=0.0, xcfunc="VWN", device=0, band_batch=64, mixed=False):
+                 smearing_kT=0.0, xcfunc="VWN", device=0, band_batch=64, mixed=False,
+                 use_symmetry=True, time_reversal=True):
         if len(pspfiles) != atoms.Nspecies:
             raise ValueError(f"Length of pspfiles is not equal to Nspecies={atoms.Nspecies}")
         self.pspots = [p if isinstance(p, PsPot_GTH) else PsPot_GTH(p) for p in pspfiles]
@@ -82,9 +84,16 @@ class Hamiltonian:
             raise NotImplementedError("only the LDA (Slater + VWN) functional is implemented")
         self.atoms = atoms
         self.xcfunc = xcfunc
+        # symmetry (Hamiltonian.jl:53-83, 261-265): the crystal's operations reduce the
+        # k-mesh, and the density built from the reduced set is symmetrised
+        self.sym_ops = crystal_symmetry(atoms) if use_symmetry else [(np.eye(3, dtype=np.int64), np.zeros(3))]
         if kpoints is None:
-            kpoints = kpoint_grid(atoms.LatVecs, meshk, shiftk, rots=rots)
+            if rots is None:
+                rots = kpoint_rotations(self.sym_ops)
+            kpoints = kpoint_grid(atoms.LatVecs, meshk, shiftk, rots=rots, time_reversal=time_reversal)
         self.pw = PWGrid(ecutwfc, atoms.LatVecs, kpoints=kpoints, Ns=Ns_)
+        self.rhoe_symmetrizer = (RhoeSymmetrizer(atoms, self.pw, self.sym_ops) if len(self.sym_ops) > 1
+                                 else None)
         self.pspotNL = PsPotNL(atoms, self.pw, self.pspots)
         self.Zvals = np.array([p.zval for p in self.pspots])
         Ne = float(np.sum(self.Zvals[atoms.atm2species]))
@@ -118,6 +127,51 @@ class Hamiltonian:
             if nb > 0:
                 b = np.asfortranarray(self.pspotNL.betaNL[ik])
                 c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
+        # rho sphere and V_sp(|G|) of every species for the force / energy-gradient kernels
+        vg = np.empty((pw.Ng, self.atoms.Nspecies), order="F")
+        for isp, psp in enumerate(self.pspots):
+            vg[:, isp] = eval_Vloc_G(psp, pw.G2_shells)[pw.idx_g2shells]
+        idx = np.ascontiguousarray(pw.idx_g2r, dtype=np.int64)
+        G = np.ascontiguousarray(pw.G.T, dtype=float)
+        c.call("pw_set_rho_sphere", int(pw.Ng), ptr(idx), 0, ptr(G), self.atoms.Nspecies, ptr(vg))
+        self._upload_atoms()
+
+    def _upload_atoms(self):
+        atoms = self.atoms
+        tau = np.asfortranarray(atoms.positions, dtype=float)
+        sp = np.ascontiguousarray(atoms.atm2species, dtype=np.int32)
+        ba = np.array([ia for ia, l, m, ids in self.pspotNL.blocks for _ in ids], dtype=np.int32)
+        self.ctx.call("pw_set_atoms", atoms.Natoms, ptr(tau), ptr(sp), len(ba),
+                      ptr(ba) if len(ba) else None)
+
+    def set_positions(self, positions):
+        """Move the atoms (Cartesian bohr, 3 x Natoms): rebuild the structure-factor
+        dependent set-up (betaNL, V_Ps_loc, E_NN) on the host as Hamiltonian() does and
+        upload it; V_loc is rebuilt from V_Ps_loc + the current Hartree and XC parts."""
+        atoms = self.atoms
+        old = atoms.positions
+        atoms.positions = np.array(positions, dtype=float).reshape(3, atoms.Natoms)
+        if len(self.sym_ops) > 1:
+            keep = {R.tobytes() for R, _ in crystal_symmetry(atoms)}
+            if any(R.tobytes() not in keep for R, _ in self.sym_ops):
+                atoms.positions = old
+                raise ValueError("set_positions: the new positions break symmetries used to reduce the "
+                                 "k-points; build the Hamiltonian with use_symmetry=False")
+            self.sym_ops = [op for op in crystal_symmetry(atoms)
+                            if any(np.array_equal(op[0], R) for R, _ in self.sym_ops)]
+            self.rhoe_symmetrizer = RhoeSymmetrizer(atoms, self.pw, self.sym_ops)
+        pw, c = self.pw, self.ctx
+        self.pspotNL = PsPotNL(atoms, pw, self.pspots)
+        nb = self.pspotNL.NbetaNL
+        for ik in range(pw.Nkpt):
+            if nb > 0:
+                b = np.asfortranarray(self.pspotNL.betaNL[ik])
+                c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
+        self._upload_atoms()
+        self.potentials.Ps_loc = V_Ps_loc(atoms, pw, self.pspots)
+        self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + self.potentials.XC[:, 0]
+        self.E_NN = calc_E_NN(atoms, self.Zvals)
+        self._upload_vloc()
 
     def _upload_vloc(self):
         V = np.ascontiguousarray(self.potentials.Total[:, 0])
@@ -243,8 +297,9 @@ def calc_betaNL_psi(Ham, psi):
 
 
 def calc_rhoe_(Ham, psiks, Rhoe, Focc=None):
-    """calc_rhoe! (calc_rhoe.jl:9-80, Nspin = 1, no symmetrisation): Rhoe[:, 0] is
-    overwritten with sum_k wk sum_ist Focc |psi(r)|^2 (psi(r) = N/sqrt(Omega) IFFT psi)."""
+    """calc_rhoe! (calc_rhoe.jl:9-135, Nspin = 1): Rhoe[:, 0] is overwritten with
+    sum_k wk sum_ist Focc |psi(r)|^2 (psi(r) = N/sqrt(Omega) IFFT psi), symmetrised when the
+    k-points were reduced by crystal symmetry (calc_rhoe.jl:130-132)."""
     c = Ham.ctx
     Focc = Ham.electrons.Focc if Focc is None else Focc
     c.call("pw_rhoe_zero")
@@ -258,6 +313,8 @@ def calc_rhoe_(Ham, psiks, Rhoe, Focc=None):
             c.call("pw_rhoe_accum", ik, a.shape[1], ptr(a), a.shape[0], ptr(w), Ham.flags_prec)
     r = np.empty(Ham.pw.Npoints)
     c.call("pw_rhoe_get", ptr(r))
+    if Ham.rhoe_symmetrizer is not None:  # calc_rhoe.jl:130-132
+        r = Ham.rhoe_symmetrizer(r)
     Rhoe = np.asarray(Rhoe)
     if Rhoe.ndim == 2:
         Rhoe[:, 0] = r

@@ -22,8 +22,11 @@ def oracle_ham(symbols, frac, LatVecs, pspfile, ecut, **kw):
 
 
 def product_ham(symbols, frac, LatVecs, pspfile, ecut, **kw):
+    """Product Hamiltonian; k-points reduced by time reversal only (as the oracle's
+    default) unless use_symmetry is passed."""
     import pwdft_hip as ph
     atoms = ph.Atoms(symbols, frac, LatVecs)
+    kw.setdefault("use_symmetry", False)
     return ph.Hamiltonian(atoms, [pspfile], ecut, **kw)
 
 

@@ -79,3 +79,25 @@ def test_scf_si_fcc_matches_reference_log():
     out = ph.KS_solve_SCF_(H, betamix=0.5, etot_conv_thr=1e-6, verbose=True)
     assert abs(out["energies"]["Total"] - ETOT_LOG) < 1e-6, out["energies"]
     assert abs(out["energies"]["NN"] - (-8.397927400714764)) < 1e-10
+
+
+def test_scf_si_fcc_symmetry():
+    """The reference's own set-up for that log: use_symmetry = true reduces the 3x3x3 mesh to
+    4 k-points (weights 1:8:6:12) and symmetrises the density (calc_rhoe.jl:130-132).  The
+    converged energy equals the log value and the time-reversal-only result; the
+    nonlocal force on the undisplaced crystal is zero after symmetrisation."""
+    import pwdft_hip as ph
+    H = _si(meshk=(3, 3, 3), use_symmetry=True)
+    assert H.pw.Nkpt == 4 and len(H.sym_ops) == 48
+    assert np.allclose(np.sort(H.pw.wk * 27), [1, 6, 8, 12])
+    print("case scf si fcc symmetry", flush=True)
+    out = ph.KS_solve_SCF_(H, betamix=0.5, etot_conv_thr=1e-8, ethr_evals_last=1e-9, verbose=True)
+    assert abs(out["energies"]["Total"] - ETOT_LOG) < 1e-6, out["energies"]
+    H2 = _si(meshk=(3, 3, 3))
+    out2 = ph.KS_solve_SCF_(H2, betamix=0.5, etot_conv_thr=1e-8, ethr_evals_last=1e-9)
+    assert abs(out["energies"]["Total"] - out2["energies"]["Total"]) < 1e-7
+    rho = ph.calc_rhoe(H, out["psiks"])[:, 0]
+    rho2 = ph.calc_rhoe(H2, out2["psiks"])[:, 0]
+    assert np.abs(rho - rho2).max() < 1e-4 * np.abs(rho2).max()
+    F = ph.calc_forces(H, out["psiks"], rho)
+    assert np.abs(F).max() < 1e-6, F

@@ -103,3 +103,60 @@ def test_null_context_is_an_error_not_a_crash():
     lib = _lib.load()
     assert lib.pw_set_grid(None, 8, 8, 8, 1.0) == -1
     assert lib.pw_last_error(None) == b"null context"
+
+
+def test_symmetry_kpoint_counts():
+    """Crystal operations and the reduced k-meshes: 48 operations for diamond Si and fcc
+    Al; IBZ sizes 8 (Si 4x4x4, test/test_spglib.jl:19), 10 (4x4x4 shifted), 60 (8x8x8
+    shifted), 4 with weights 1:8:6:12 (Si 3x3x3) and 182 (Al 12x12x12 shifted); a displaced
+    Si8 cell keeps only the identity."""
+    sym, frac = ["Si", "Si"], [[0, 0, 0], [0.25, 0.25, 0.25]]
+    si = B.Atoms(sym, frac, B.gen_lattice_fcc(A_SI))
+    ops = B.crystal_symmetry(si)
+    assert len(ops) == 48
+    rots = B.kpoint_rotations(ops)
+    for mesh, sh, n in (((4, 4, 4), (0, 0, 0), 8), ((4, 4, 4), (1, 1, 1), 10), ((8, 8, 8), (1, 1, 1), 60)):
+        assert B.kpoint_grid(si.LatVecs, mesh, sh, rots=rots)[0].shape[1] == n
+    k, wk = B.kpoint_grid(si.LatVecs, (3, 3, 3), (0, 0, 0), rots=rots)
+    assert np.allclose(np.sort(wk * 27), [1, 6, 8, 12])
+    al = B.Atoms(["Al"], [[0, 0, 0]], B.gen_lattice_fcc(7.6))
+    assert B.kpoint_grid(al.LatVecs, (12, 12, 12), (1, 1, 1),
+                         rots=B.kpoint_rotations(B.crystal_symmetry(al)))[0].shape[1] == 182
+    s8 = B.Atoms(["Si"] * 8, B.diamond_supercell(1.0, 1), B.gen_lattice_sc(A_SI))
+    s8.positions += np.random.default_rng(0).uniform(-0.05, 0.05, size=s8.positions.shape)
+    assert len(B.crystal_symmetry(s8)) == 1
+
+
+def test_rhoe_symmetrizer_and_vectors():
+    """symmetrize_rhoe!: idempotent, charge-conserving, identity on a symmetric density,
+    and its output is invariant under every operation (rho(R x + t) = rho(x) on the grid
+    for the symmorphic ones).  symmetrize_vector!: a projection onto symmetric fields."""
+    from pwdft_hip.potentials import guess_rhoe
+    si = B.Atoms(["Si", "Si"], [[0, 0, 0], [0.25, 0.25, 0.25]], B.gen_lattice_fcc(A_SI))
+    ops = B.crystal_symmetry(si)
+    pw = B.PWGrid(8.0, si.LatVecs)
+    S = B.RhoeSymmetrizer(si, pw, ops)
+    rho = np.random.default_rng(0).uniform(size=pw.Npoints)
+    f = np.fft.fftn(rho.reshape(pw.Ns, order="F")).reshape(-1, order="F")
+    keep = np.zeros(pw.Npoints, dtype=bool)
+    keep[pw.idx_g2r] = True
+    f[~keep] = 0.0   # band-limited to the rho sphere: outside it the reference keeps rho(G)
+    rho = np.real(np.fft.ifftn(f.reshape(pw.Ns, order="F")).reshape(-1, order="F"))
+    r1 = S(rho)
+    assert np.abs(S(r1) - r1).max() < 1e-13
+    assert abs(r1.sum() - rho.sum()) < 1e-9 * abs(rho).sum()
+    g = guess_rhoe(si, pw, [B.PsPot_GTH("Si-q4.lda.gth")], 8.0)
+    assert np.abs(S(g) - g).max() < 1e-14
+    n = np.array(pw.Ns)
+    x = np.stack(np.meshgrid(*[np.arange(m) for m in pw.Ns], indexing="ij"), axis=-1).reshape(-1, 3)
+    lin = lambda q: q[:, 0] + n[0] * (q[:, 1] + n[1] * q[:, 2])
+    for R, t in ops:
+        tn = t * n
+        if np.abs(tn - np.round(tn)).max() > 1e-9:
+            continue
+        y = (x @ R.T + np.round(tn).astype(int)) % n
+        assert np.abs(r1[lin(y)] - r1[lin(x)]).max() < 1e-12
+    v = np.random.default_rng(1).standard_normal((3, 2))
+    vs = B.symmetrize_vector(si, ops, v)
+    assert np.abs(B.symmetrize_vector(si, ops, vs) - vs).max() < 1e-14
+    assert np.abs(vs).max() < 1e-14   # no symmetric force on the diamond sites
