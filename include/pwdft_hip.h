@@ -100,6 +100,23 @@ int pw_op_H(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldp
 int pw_op_H_dev(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldpsi, void* Hpsi,
                 int64_t ldh, unsigned flags);
 
+/* op_H over several k-points in one launch per stage (the k loop of op_H(Ham, psiks) for a
+ * BlochWavefunc, op_H.jl:29-50): k-point iks[i] applies to psi[i] (Ngw[iks[i]] x nb, leading
+ * dimension ldpsi[i]) and writes Hpsi[i]; every k-point has the same band count nb.  Flags
+ * as pw_op_H.  Device pointers.  With unchanged arguments, consecutive calls upload no
+ * descriptors. */
+int pw_op_H_batch_dev(pw_ctx* ctx, int nk, const int* iks, int ispin, int nb, void* const* psi,
+                      const int64_t* ldpsi, void* const* Hpsi, const int64_t* ldh, unsigned flags);
+
+/* Kprec for several k-points (Kprec.jl:10-28), device pointers. */
+int pw_kprec_batch_dev(pw_ctx* ctx, int nk, const int* iks, int nb, void* const* in, const int64_t* ldin,
+                       void* const* out, const int64_t* ldout);
+
+/* Copy column blocks (rows[i] x cols[i]) from src[i] to dst[i] for a batch, one launch;
+ * device pointers.  No reference counterpart (the eigensolver's block assignments). */
+int pw_copy_cols_batch_dev(pw_ctx* ctx, int nbatch, const int* rows, const int* cols, void* const* src,
+                           const int64_t* lds, void* const* dst, const int64_t* ldd);
+
 /* Kinetic preconditioner Kprec (Kprec.jl:10-28): out = in / (1 + |G+k|^2), device pointers. */
 int pw_kprec_dev(pw_ctx* ctx, int ik, int nb, const void* in, int64_t ldin, void* out, int64_t ldout);
 
@@ -167,6 +184,31 @@ int pw_calc_forces_Ps_nloc(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t
 int pw_zgemm_dev(pw_ctx* ctx, int transa, int m, int n, int k, const double* alpha, const void* A,
                  int64_t lda, const void* B, int64_t ldb, const double* beta, void* C, int64_t ldc,
                  unsigned flags);
+
+/* A batch of independent products C_i = alpha op(A_i) B_i + beta C_i with per-product sizes
+ * (m_i, n_i, k_i) and leading dimensions, op as pw_zgemm_dev (transa shared); complex
+ * column-major, device pointers, FP64.  The per-k Gram products and column updates of
+ * diag_LOBPCG! run over all k-points at once through this (diag_LOBPCG.jl:85-203). */
+int pw_zgemm_batch_dev(pw_ctx* ctx, int transa, int nbatch, const int* m, const int* n, const int* k,
+                       const double* alpha, void* const* A, const int64_t* lda, void* const* B,
+                       const int64_t* ldb, const double* beta, void* const* C, const int64_t* ldc);
+
+/* ---- small dense FP64 problems of the eigensolvers, on the device ------------------- */
+
+/* The host LAPACK calls of diag_LOBPCG! / diag_davidson! / ortho_sqrt (cholesky(Hermitian(C)),
+ * inv(U), eigen(Hermitian(T), Hermitian(G)), eigen(Hermitian(S))) on device matrices
+ * (diag_LOBPCG.jl:106-118, 176-203, 217-222).  Column-major complex, upper triangle read.
+ *   pw_herm_fix_dev: mode 0 A <- Hermitian(A); 1 zero the strict lower triangle;
+ *                    2 A <- (A + A^H) / 2.
+ *   pw_potrf_dev:  A = U^H U, U into the upper triangle (PW_ERR_NOT_POSDEF on failure).
+ *   pw_trtri_dev:  upper-triangular A <- inv(A).
+ *   pw_heevd_dev:  eigenvectors into A (ascending), eigenvalues into w (host, n).
+ *   pw_hegvd_dev:  T x = l G x; eigenvectors (G-orthonormal) into A, B overwritten. */
+int pw_herm_fix_dev(pw_ctx* ctx, int n, void* A, int64_t lda, int mode);
+int pw_potrf_dev(pw_ctx* ctx, int n, void* A, int64_t lda);
+int pw_trtri_dev(pw_ctx* ctx, int n, void* A, int64_t lda);
+int pw_heevd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
+int pw_hegvd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w);
 
 /* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */
 

@@ -1,0 +1,366 @@
+// Batched entry points: many k-points (each with its own plane-wave count) in one launch
+// per stage.  The Hamiltonian application runs the three sphere-pruned FFT kernels over all
+// k-points together (pw_fft.hip, *_batch kernels) and the nonlocal products through a
+// batched complex GEMM for small, variable-size products; the eigensolver's Gram products
+// and column updates use the same GEMM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+constexpr int BT = 16;   // output tile BT x BT per workgroup
+constexpr int BK = 32;   // k-chunk staged in LDS
+
+// C = alpha op(A) B + beta C for every product of the batch; one workgroup of 256 threads
+// per 16 x 16 output tile, thread (tx, ty) owns C[i0 + tx, j0 + ty].  A and B chunks are
+// staged through LDS with loads that run along the contiguous (column) direction.
+template <int TA>
+__global__ void __launch_bounds__(256) k_zgemm_batch(const GBatch* __restrict__ gb, int ng, cplx<double> alpha,
+                                                     cplx<double> beta, int use_beta) {
+  __shared__ cplx<double> As[BK][BT + 1];
+  __shared__ cplx<double> Bs[BK][BT + 1];
+  const int w = blockIdx.x;
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gb[mid].tile0 <= w) lo = mid; else hi = mid - 1;
+  }
+  const GBatch& g = gb[lo];
+  const int tm = (g.m + BT - 1) / BT;
+  const int t = w - g.tile0;
+  const int i0 = (t % tm) * BT, j0 = (t / tm) * BT;
+  const int tid = threadIdx.x, tx = tid % BT, ty = tid / BT;
+  cplx<double> acc = mk(0.0, 0.0);
+  for (int k0 = 0; k0 < g.k; k0 += BK) {
+    for (int e = tid; e < BK * BT; e += 256) {
+      int kc, i;
+      if (TA) {
+        kc = e % BK;
+        i = e / BK;
+      } else {
+        i = e % BT;
+        kc = e / BT;
+      }
+      const int kk = k0 + kc, ii = i0 + i;
+      cplx<double> a = mk(0.0, 0.0);
+      if (kk < g.k && ii < g.m) a = TA ? g.A[(long)ii * g.lda + kk] : g.A[(long)kk * g.lda + ii];
+      As[kc][i] = a;
+      const int kb = e % BK, j = e / BK, jj = j0 + j;
+      cplx<double> b = mk(0.0, 0.0);
+      if (k0 + kb < g.k && jj < g.n) b = g.B[(long)jj * g.ldb + k0 + kb];
+      Bs[kb][j] = b;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kc = 0; kc < BK; ++kc) {
+      const cplx<double> a = As[kc][tx], b = Bs[kc][ty];
+      if (TA) {  // conj(a) * b
+        acc.x += a.x * b.x + a.y * b.y;
+        acc.y += a.x * b.y - a.y * b.x;
+      } else {
+        acc.x += a.x * b.x - a.y * b.y;
+        acc.y += a.x * b.y + a.y * b.x;
+      }
+    }
+    __syncthreads();
+  }
+  const int i = i0 + tx, j = j0 + ty;
+  if (i < g.m && j < g.n) {
+    cplx<double>* cp = g.C + (long)j * g.ldc + i;
+    cplx<double> r = cmul(alpha, acc);
+    if (use_beta) r = r + cmul(beta, *cp);
+    *cp = r;
+  }
+}
+
+int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out) {
+  const int n = (int)c->desc_host.size();
+  for (int i = 0; i < n; ++i)
+    if (c->desc_host[i].size() == bytes && std::memcmp(c->desc_host[i].data(), data, bytes) == 0) {
+      *out = c->desc_dev[i];
+      return PW_OK;
+    }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "descriptor sync");
+  int slot;
+  if (n < 64) {
+    c->desc_host.emplace_back();
+    c->desc_dev.push_back(nullptr);
+    c->desc_cap.push_back(0);
+    slot = n;
+  } else {
+    slot = c->desc_next;
+    c->desc_next = (c->desc_next + 1) % 64;
+  }
+  int rc = ensure(c, &c->desc_dev[slot], &c->desc_cap[slot], bytes);
+  if (rc) return rc;
+  e = hipMemcpy(c->desc_dev[slot], data, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_check(c, e, "descriptor upload");
+  const char* p = reinterpret_cast<const char*>(data);
+  c->desc_host[slot].assign(p, p + bytes);
+  *out = c->desc_dev[slot];
+  return PW_OK;
+}
+
+void free_desc(pw_ctx* c) {
+  for (void* p : c->desc_dev) hipFree(p);
+  c->desc_dev.clear();
+  c->desc_host.clear();
+  c->desc_cap.clear();
+  c->desc_next = 0;
+  hipFree(c->d_bnlb);
+  c->d_bnlb = nullptr;
+  c->bnlb_bytes = 0;
+}
+
+int zgemm_batch(pw_ctx* c, int transa, const std::vector<GBatch>& g0, cplx<double> alpha, cplx<double> beta) {
+  std::vector<GBatch> g;
+  g.reserve(g0.size());
+  long tiles = 0;
+  for (const GBatch& x : g0) {
+    if (x.m <= 0 || x.n <= 0) continue;
+    GBatch y = x;
+    y.tile0 = (int)tiles;
+    tiles += (long)((x.m + BT - 1) / BT) * ((x.n + BT - 1) / BT);
+    g.push_back(y);
+  }
+  if (g.empty()) return PW_OK;
+  if (tiles > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "batched GEMM too large for one launch");
+  const void* dptr = nullptr;
+  int rc = desc_get(c, g.data(), g.size() * sizeof(GBatch), &dptr);
+  if (rc) return rc;
+  const GBatch* d = reinterpret_cast<const GBatch*>(dptr);
+  const int ub = (beta.x != 0.0 || beta.y != 0.0) ? 1 : 0;
+  if (transa)
+    hipLaunchKernelGGL(k_zgemm_batch<1>, dim3((unsigned)tiles), dim3(256), 0, c->stream, d, (int)g.size(), alpha,
+                       beta, ub);
+  else
+    hipLaunchKernelGGL(k_zgemm_batch<0>, dim3((unsigned)tiles), dim3(256), 0, c->stream, d, (int)g.size(), alpha,
+                       beta, ub);
+  return hip_check(c, hipGetLastError(), "k_zgemm_batch");
+}
+
+__global__ void k_zero_cols_b(cplx<double>* __restrict__ a, long ld, int nrows, int ncols) {
+  const long tot = (long)nrows * ncols;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x)
+    a[(t / nrows) * ld + (t % nrows)] = mk(0.0, 0.0);
+}
+
+constexpr int EPB = 1024;  // elements per workgroup of k_elem_batch
+
+__global__ void __launch_bounds__(256) k_elem_batch(const EBatch* __restrict__ eb, int ne) {
+  const int w = blockIdx.x;
+  int lo = 0, hi = ne - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (eb[mid].blk0 <= w) lo = mid; else hi = mid - 1;
+  }
+  const EBatch& d = eb[lo];
+  const long tot = (long)d.m * d.n;
+  const long base = (long)(w - d.blk0) * EPB;
+  for (long t = base + threadIdx.x; t < min(base + EPB, tot); t += 256) {
+    const int i = (int)(t % d.m), j = (int)(t / d.m);
+    cplx<double> v = d.in[(long)j * d.ldin + i];
+    if (d.gk2) v = scal(1.0 / (1.0 + d.gk2[i]), v);
+    d.out[(long)j * d.ldout + i] = v;
+  }
+}
+
+static int elem_batch(pw_ctx* c, std::vector<EBatch>& e0) {
+  std::vector<EBatch> e;
+  long blk = 0;
+  for (EBatch x : e0) {
+    if (x.m <= 0 || x.n <= 0) continue;
+    x.blk0 = (int)blk;
+    blk += ((long)x.m * x.n + EPB - 1) / EPB;
+    e.push_back(x);
+  }
+  if (e.empty()) return PW_OK;
+  if (blk > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "batched copy too large for one launch");
+  const void* dptr = nullptr;
+  int rc = desc_get(c, e.data(), e.size() * sizeof(EBatch), &dptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_elem_batch, dim3((unsigned)blk), dim3(256), 0, c->stream,
+                     reinterpret_cast<const EBatch*>(dptr), (int)e.size());
+  return hip_check(c, hipGetLastError(), "k_elem_batch");
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+#define CTX_CHECK(c) \
+  if (!(c)) return PW_ERR_ARG
+
+static int check_batch_k(pw_ctx* c, int nk, const int* iks) {
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
+  if (nk < 0 || (nk > 0 && !iks)) return set_error(c, PW_ERR_ARG, "batch: bad k-point list");
+  for (int kk = 0; kk < nk; ++kk)
+    if (iks[kk] < 0 || iks[kk] >= (int)c->kd.size() || c->kd[iks[kk]].ngw == 0)
+      return set_error(c, PW_ERR_STATE, "batch: k-point " + std::to_string(iks[kk]) + " is not set up");
+  return PW_OK;
+}
+
+extern "C" {
+
+int pw_zgemm_batch_dev(pw_ctx* c, int transa, int nbatch, const int* m, const int* n, const int* k,
+                       const double* alpha, void* const* A, const int64_t* lda, void* const* B, const int64_t* ldb,
+                       const double* beta, void* const* C, const int64_t* ldc) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (nbatch < 0 || (nbatch > 0 && (!m || !n || !k || !A || !lda || !B || !ldb || !C || !ldc)) || !alpha || !beta)
+    return set_error(c, PW_ERR_ARG, "pw_zgemm_batch_dev: null argument");
+  std::vector<GBatch> g(nbatch);
+  for (int i = 0; i < nbatch; ++i) {
+    if (m[i] < 0 || n[i] < 0 || k[i] < 0 || ldc[i] < std::max(1, m[i]) ||
+        ldb[i] < std::max(1, k[i]) || lda[i] < std::max(1, transa ? k[i] : m[i]))
+      return set_error(c, PW_ERR_ARG, "pw_zgemm_batch_dev: bad sizes or leading dimensions");
+    GBatch x{};
+    x.A = reinterpret_cast<const cplx<double>*>(A[i]);
+    x.B = reinterpret_cast<const cplx<double>*>(B[i]);
+    x.C = reinterpret_cast<cplx<double>*>(C[i]);
+    x.lda = lda[i];
+    x.ldb = ldb[i];
+    x.ldc = ldc[i];
+    x.m = m[i];
+    x.n = n[i];
+    x.k = k[i];
+    g[i] = x;
+  }
+  return zgemm_batch(c, transa ? 1 : 0, g, mk(alpha[0], alpha[1]), mk(beta[0], beta[1]));
+}
+
+int pw_op_H_batch_dev(pw_ctx* c, int nk, const int* iks, int ispin, int nb, void* const* psi, const int64_t* ldpsi,
+                      void* const* Hpsi, const int64_t* ldh, unsigned flags) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  int rc = check_batch_k(c, nk, iks);
+  if (rc) return rc;
+  if (nb < 0) return set_error(c, PW_ERR_ARG, "op_H batch: nb < 0");
+  if (nk == 0 || nb == 0) return PW_OK;
+  if (!psi || !ldpsi || !Hpsi || !ldh) return set_error(c, PW_ERR_ARG, "op_H batch: null argument");
+  for (int kk = 0; kk < nk; ++kk)
+    if (!psi[kk] || !Hpsi[kk] || ldpsi[kk] < c->kd[iks[kk]].ngw || ldh[kk] < c->kd[iks[kk]].ngw)
+      return set_error(c, PW_ERR_ARG, "op_H batch: bad pointer or leading dimension");
+  const unsigned terms = flags & PW_TERM_ALL;
+  bool have = (flags & PW_ACCUMULATE) != 0;
+  const cplx<double>* const* P = reinterpret_cast<const cplx<double>* const*>(psi);
+  cplx<double>* const* Hp = reinterpret_cast<cplx<double>* const*>(Hpsi);
+  if (terms & PW_TERM_VLOC) {
+    if (ispin < 0 || ispin >= c->nspin || !c->d_v64[ispin])
+      return set_error(c, PW_ERR_STATE, "op_V_loc: potential of spin " + std::to_string(ispin) + " not set");
+    rc = vloc_apply_batch(c, nk, iks, ispin, nb, P, ldpsi, Hp, ldh, flags);
+    if (rc) return rc;
+    have = true;
+  } else if (terms & PW_TERM_K) {
+    for (int kk = 0; kk < nk; ++kk) {
+      rc = kinetic_apply(c, iks[kk], nb, P[kk], ldpsi[kk], Hp[kk], ldh[kk], have ? PW_ACCUMULATE : 0);
+      if (rc) return rc;
+    }
+    have = true;
+  }
+  const int nbeta = c->kd[iks[0]].nbeta;
+  bool nloc = (terms & PW_TERM_NLOC) && nbeta > 0;
+  if (nloc) {
+    for (int kk = 0; kk < nk; ++kk)
+      if (c->kd[iks[kk]].nbeta != nbeta) return set_error(c, PW_ERR_STATE, "op_H batch: projectors not set");
+    rc = ensure(c, &c->d_bnlb, &c->bnlb_bytes, (size_t)nk * nbeta * nb * sizeof(cplx<double>));
+    if (rc) return rc;
+    cplx<double>* bnl = reinterpret_cast<cplx<double>*>(c->d_bnlb);
+    std::vector<GBatch> g1(nk), g2(nk);
+    for (int kk = 0; kk < nk; ++kk) {
+      const KData& k = c->kd[iks[kk]];
+      GBatch a{};
+      a.A = reinterpret_cast<const cplx<double>*>(k.d_beta64);
+      a.lda = k.ngw;
+      a.B = P[kk];
+      a.ldb = ldpsi[kk];
+      a.C = bnl + (size_t)kk * nbeta * nb;
+      a.ldc = nbeta;
+      a.m = nbeta;
+      a.n = nb;
+      a.k = k.ngw;
+      g1[kk] = a;
+      GBatch b{};
+      b.A = reinterpret_cast<const cplx<double>*>(k.d_betaD64);
+      b.lda = k.ngw;
+      b.B = a.C;
+      b.ldb = nbeta;
+      b.C = Hp[kk];
+      b.ldc = ldh[kk];
+      b.m = k.ngw;
+      b.n = nb;
+      b.k = nbeta;
+      g2[kk] = b;
+    }
+    const cplx<double> one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+    rc = zgemm_batch(c, 1, g1, one, zero);
+    if (rc) return rc;
+    rc = zgemm_batch(c, 0, g2, one, have ? one : zero);
+    if (rc) return rc;
+    have = true;
+  }
+  if (!have) {
+    for (int kk = 0; kk < nk; ++kk) {
+      const long tot = (long)c->kd[iks[kk]].ngw * nb;
+      hipLaunchKernelGGL(k_zero_cols_b, dim3((int)std::min<long>((tot + 255) / 256, 4096)), dim3(256), 0, c->stream,
+                         Hp[kk], ldh[kk], c->kd[iks[kk]].ngw, nb);
+    }
+    return hip_check(c, hipGetLastError(), "op_H batch: zero");
+  }
+  return PW_OK;
+}
+
+int pw_kprec_batch_dev(pw_ctx* c, int nk, const int* iks, int nb, void* const* in, const int64_t* ldin,
+                       void* const* out, const int64_t* ldout) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  int rc = check_batch_k(c, nk, iks);
+  if (rc) return rc;
+  if (nk == 0 || nb <= 0) return nb < 0 ? set_error(c, PW_ERR_ARG, "kprec batch: nb < 0") : PW_OK;
+  if (!in || !ldin || !out || !ldout) return set_error(c, PW_ERR_ARG, "kprec batch: null argument");
+  std::vector<EBatch> e(nk);
+  for (int kk = 0; kk < nk; ++kk) {
+    const KData& k = c->kd[iks[kk]];
+    if (ldin[kk] < k.ngw || ldout[kk] < k.ngw) return set_error(c, PW_ERR_ARG, "kprec batch: leading dimension");
+    EBatch x{};
+    x.in = reinterpret_cast<const cplx<double>*>(in[kk]);
+    x.out = reinterpret_cast<cplx<double>*>(out[kk]);
+    x.gk2 = k.d_gk2;
+    x.ldin = ldin[kk];
+    x.ldout = ldout[kk];
+    x.m = k.ngw;
+    x.n = nb;
+    e[kk] = x;
+  }
+  return elem_batch(c, e);
+}
+
+int pw_copy_cols_batch_dev(pw_ctx* c, int nbatch, const int* rows, const int* cols, void* const* src,
+                           const int64_t* lds, void* const* dst, const int64_t* ldd) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (nbatch < 0 || (nbatch > 0 && (!rows || !cols || !src || !lds || !dst || !ldd)))
+    return set_error(c, PW_ERR_ARG, "pw_copy_cols_batch_dev: null argument");
+  std::vector<EBatch> e(nbatch);
+  for (int i = 0; i < nbatch; ++i) {
+    if (rows[i] < 0 || cols[i] < 0 || lds[i] < rows[i] || ldd[i] < rows[i])
+      return set_error(c, PW_ERR_ARG, "pw_copy_cols_batch_dev: bad sizes");
+    EBatch x{};
+    x.in = reinterpret_cast<const cplx<double>*>(src[i]);
+    x.out = reinterpret_cast<cplx<double>*>(dst[i]);
+    x.ldin = lds[i];
+    x.ldout = ldd[i];
+    x.m = rows[i];
+    x.n = cols[i];
+    e[i] = x;
+  }
+  return elem_batch(c, e);
+}
+
+}  // extern "C"

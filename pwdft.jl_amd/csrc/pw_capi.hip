@@ -256,6 +256,8 @@ int pw_ctx_destroy(pw_ctx* c) {
   hipFree(c->d_tmp);
   hipFree(c->d_wts);
   free_forces(c);
+  free_desc(c);
+  free_dense(c);
   drain_events(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   hipStreamDestroy(c->stream);

@@ -1,0 +1,207 @@
+// Small dense FP64 linear algebra of the eigensolvers on the device: the Cholesky
+// factorisation and triangular inverse of the Gram matrices and the (generalised) Hermitian
+// eigenproblems of the Rayleigh-Ritz step (diag_LOBPCG.jl:106-118, 176-203;
+// ortho_sqrt.jl:6-9).  These are the LAPACK calls the reference makes on the host
+// (ZPOTRF, ZTRTRI via inv(U), ZHEGV, ZHEEV); here they run through rocSOLVER on the context
+// stream so that the subspace matrices never leave HBM.  Only the upper triangle of the
+// input is read, as Julia's Hermitian(A) does.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <string>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+static int ensure_solver(pw_ctx* c) {
+  if (c->rb) return PW_OK;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success)
+    return set_error(c, PW_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(h, c->stream) != rocblas_status_success) {
+    rocblas_destroy_handle(h);
+    return set_error(c, PW_ERR_HIP, "rocblas_set_stream failed");
+  }
+  c->rb = h;
+  int rc = ensure(c, &c->d_dinfo, &c->dinfo_bytes, 64 * sizeof(double) + 16 * sizeof(int));
+  return rc;
+}
+
+void free_dense(pw_ctx* c) {
+  if (c->rb) rocblas_destroy_handle(reinterpret_cast<rocblas_handle>(c->rb));
+  c->rb = nullptr;
+  hipFree(c->d_dinfo);
+  c->d_dinfo = nullptr;
+  c->dinfo_bytes = 0;
+  hipFree(c->d_dwork);
+  c->d_dwork = nullptr;
+  c->dwork_bytes = 0;
+}
+
+static int st_check(pw_ctx* c, rocblas_status s, const char* what) {
+  if (s == rocblas_status_success) return PW_OK;
+  return set_error(c, PW_ERR_HIP, std::string(what) + ": rocSOLVER status " + std::to_string((int)s));
+}
+
+static int read_info(pw_ctx* c, const int* d_info, int* info) {
+  hipError_t e = hipMemcpyAsync(info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return hip_check(c, e, "solver info");
+}
+
+// mode 0: A <- Hermitian(A) (upper triangle mirrored, real diagonal); 1: strict lower
+// triangle zeroed (an upper-triangular factor); 2: A <- (A + A^H) / 2.
+__global__ void k_herm_fix(cplx<double>* __restrict__ A, int n, long lda, int mode) {
+  const long tot = (long)n * n;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % n), j = (int)(t / n);
+    if (i > j) {
+      if (mode == 1) {
+        A[(long)j * lda + i] = mk(0.0, 0.0);
+      } else if (mode == 0) {
+        const cplx<double> u = A[(long)i * lda + j];
+        A[(long)j * lda + i] = mk(u.x, -u.y);
+      }
+    } else if (i == j && mode != 1) {
+      A[(long)j * lda + i].y = 0.0;
+    }
+  }
+}
+
+__global__ void k_herm_avg(cplx<double>* __restrict__ A, int n, long lda) {
+  const long tot = (long)n * n;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % n), j = (int)(t / n);
+    if (i < j) {  // one thread per pair writes both entries
+      const cplx<double> u = A[(long)j * lda + i], l = A[(long)i * lda + j];
+      const cplx<double> m = mk(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+      A[(long)j * lda + i] = m;
+      A[(long)i * lda + j] = mk(m.x, -m.y);
+    } else if (i == j) {
+      A[(long)j * lda + i].y = 0.0;
+    }
+  }
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+#define CTX_CHECK(c) \
+  if (!(c)) return PW_ERR_ARG
+
+extern "C" {
+
+int pw_herm_fix_dev(pw_ctx* c, int n, void* A, int64_t lda, int mode) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !A || lda < (n > 0 ? n : 1) || mode < 0 || mode > 2)
+    return set_error(c, PW_ERR_ARG, "pw_herm_fix_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  const long tot = (long)n * n;
+  const int blocks = (int)std::min<long>((tot + 255) / 256, 2048);
+  cplx<double>* a = reinterpret_cast<cplx<double>*>(A);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_herm_avg, dim3(blocks), dim3(256), 0, c->stream, a, n, (long)lda);
+  else
+    hipLaunchKernelGGL(k_herm_fix, dim3(blocks), dim3(256), 0, c->stream, a, n, (long)lda, mode);
+  return hip_check(c, hipGetLastError(), "k_herm_fix");
+}
+
+int pw_potrf_dev(pw_ctx* c, int n, void* A, int64_t lda) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !A || lda < (n > 0 ? n : 1)) return set_error(c, PW_ERR_ARG, "pw_potrf_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
+  rc = st_check(c, rocsolver_zpotrf(h, rocblas_fill_upper, n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
+                                    dinfo),
+                "zpotrf");
+  if (rc) return rc;
+  int info = 0;
+  rc = read_info(c, dinfo, &info);
+  if (rc) return rc;
+  if (info != 0) return set_error(c, PW_ERR_NOT_POSDEF, "cholesky: leading minor " + std::to_string(info) + " not positive definite");
+  return PW_OK;
+}
+
+int pw_trtri_dev(pw_ctx* c, int n, void* A, int64_t lda) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !A || lda < (n > 0 ? n : 1)) return set_error(c, PW_ERR_ARG, "pw_trtri_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
+  rc = st_check(c, rocsolver_ztrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, n,
+                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda, dinfo),
+                "ztrtri");
+  if (rc) return rc;
+  int info = 0;
+  rc = read_info(c, dinfo, &info);
+  if (rc) return rc;
+  if (info != 0) return set_error(c, PW_ERR_NOT_POSDEF, "triangular inverse: singular diagonal " + std::to_string(info));
+  return PW_OK;
+}
+
+int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !A || !w || lda < (n > 0 ? n : 1)) return set_error(c, PW_ERR_ARG, "pw_heevd_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, 2 * (size_t)n * sizeof(double));
+  if (rc) return rc;
+  double* D = reinterpret_cast<double*>(c->d_dwork);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
+  rc = st_check(c, rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n,
+                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda, D, D + n, dinfo),
+                "zheevd");
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "eigenvalues download");
+  int info = 0;
+  rc = read_info(c, dinfo, &info);
+  if (rc) return rc;
+  if (info != 0) return set_error(c, PW_ERR_ARG, "eigen: no convergence (" + std::to_string(info) + ")");
+  return PW_OK;
+}
+
+int pw_hegvd_dev(pw_ctx* c, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !A || !B || !w || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1))
+    return set_error(c, PW_ERR_ARG, "pw_hegvd_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, 2 * (size_t)n * sizeof(double));
+  if (rc) return rc;
+  double* D = reinterpret_cast<double*>(c->d_dwork);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
+  rc = st_check(c, rocsolver_zhegvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
+                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
+                                    reinterpret_cast<rocblas_double_complex*>(B), (int)ldb, D, D + n, dinfo),
+                "zhegvd");
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "eigenvalues download");
+  int info = 0;
+  rc = read_info(c, dinfo, &info);
+  if (rc) return rc;
+  if (info > n) return set_error(c, PW_ERR_NOT_POSDEF, "eigen(T, G): G not positive definite");
+  if (info != 0) return set_error(c, PW_ERR_ARG, "eigen(T, G): no convergence (" + std::to_string(info) + ")");
+  return PW_OK;
+}
+
+}  // extern "C"

@@ -351,60 +351,66 @@ __device__ inline bool col_block(int ncol, int C, int nbb, int& c0, int& bl) {
 
 static inline int col_grid(int ncol, int C, int nbb) { return ((ncol + C - 1) / C) * ((nbb + 7) / 8 * 8); }
 
-// 1. scatter + inverse z-FFT of the columns [c0, c0+C) of band (b0 + blockIdx.y).
+// 1. scatter + inverse z-FFT of the columns [c0, c0+C) of one band: p is the band's psi
+// column, w its block of W (n3 x ncol).
 template <typename T>
-__global__ void __launch_bounds__(COL_NT)
-k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __restrict__ W,
-           const int* __restrict__ ent_igw, const int* __restrict__ ent_pos,
-           const int* __restrict__ col_ptr, int ncol, int C, int nbb, FftDim dz,
-           const cplx<T>* __restrict__ twz) {
+__device__ inline void col_ifft_body(const cplx<double>* __restrict__ p, cplx<T>* __restrict__ w,
+                                     const int* __restrict__ ent_igw, const int* __restrict__ ent_pos,
+                                     const int* __restrict__ col_ptr, int ncol, int C, int c0, FftDim dz,
+                                     const cplx<T>* __restrict__ twz) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int n3 = dz.n, cs = odd_stride(n3);
   cplx<T>* tw = lds + C * cs;
-  int bl, c0;
-  if (!col_block(ncol, C, nbb, c0, bl)) return;
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < cn * cs; i += nthr) lds[i] = mk((T)0, (T)0);
   stage(tw, twz, n3);
   __syncthreads();
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
-  const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
   const int off = c0 * cs;
   for (int e = e0 + tid; e < e1; e += nthr) lds[ent_pos[e] - off] = cvt<T>(p[ent_igw[e]]);
   __syncthreads();
   wave_lines<T, +1, FFT_ME>(lds, cn, nullptr, cs, 1, dz, tw);
   __syncthreads();
-  cplx<T>* w = W + (long)bl * n3 * ncol;
   for (int i = tid; i < cn * n3; i += nthr) {
     const int cl = i % cn, z = i / cn;
     w[(long)z * ncol + c0 + cl] = lds[cl * cs + z];
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(COL_NT)
+k_col_ifft(const cplx<double>* __restrict__ psi, long ldpsi, int b0, cplx<T>* __restrict__ W,
+           const int* __restrict__ ent_igw, const int* __restrict__ ent_pos,
+           const int* __restrict__ col_ptr, int ncol, int C, int nbb, FftDim dz,
+           const cplx<T>* __restrict__ twz) {
+  int bl, c0;
+  if (!col_block(ncol, C, nbb, c0, bl)) return;
+  col_ifft_body<T>(psi + (long)(b0 + bl) * ldpsi, W + (long)bl * dz.n * ncol, ent_igw, ent_pos, col_ptr, ncol, C,
+                   c0, dz, twz);
+}
+
 // 2. per (z-plane, band): inverse y-FFT on the active x lines, then per x-row one wavefront
 // does inverse x-FFT, * V/N (read from L2 inside the first forward pass) and forward
-// x-FFT, then forward y-FFT on the active x lines.  In place on W.
+// x-FFT, then forward y-FFT on the active x lines.  In place on the plane w (ncol entries).
 template <typename T>
-__global__ void __launch_bounds__(PlaneNT<T>::value)
-k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __restrict__ col_ij,
-             int ncol, const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
-             const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, int n3) {
+__device__ inline void plane_vloc_body(cplx<T>* __restrict__ w, const T* __restrict__ V, int z,
+                                       const int* __restrict__ col_ij, int ncol, const int* __restrict__ xact,
+                                       int nxact, FftDim dx, FftDim dy, const cplx<T>* __restrict__ twx,
+                                       const cplx<T>* __restrict__ twy) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
   cplx<T>* tx = lds + ps * n2;
   cplx<T>* ty = tx + n1;
   int* xl = reinterpret_cast<int*>(ty + n2);
-  const int z = blockIdx.x, bl = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
   stage(tx, twx, n1);
   stage(ty, twy, n2);
   stage(xl, xact, nxact);
   __syncthreads();
-  cplx<T>* w = W + ((long)bl * n3 + z) * ncol;
   for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
   __syncthreads();
   wave_lines<T, +1, FFT_ME>(lds, nxact, xl, 0, ps, dy, ty);
@@ -427,23 +433,29 @@ k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __rest
   for (int c = tid; c < ncol; c += nthr) w[c] = lds[col_ij[c]];
 }
 
-// 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi.
 template <typename T>
-__global__ void __launch_bounds__(COL_NT)
-k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, long ldpsi,
-          cplx<double>* __restrict__ hpsi, long ldh, int b0, const int* __restrict__ ent_igw,
-          const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol, int C, int nbb,
-          FftDim dz, const cplx<T>* __restrict__ twz, const double* __restrict__ gk2, int kin,
-          int acc) {
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __restrict__ col_ij,
+             int ncol, const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
+             const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, int n3) {
+  const int z = blockIdx.x, bl = blockIdx.y;
+  plane_vloc_body<T>(W + ((long)bl * n3 + z) * ncol, V, z, col_ij, ncol, xact, nxact, dx, dy, twx, twy);
+}
+
+// 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi (one band:
+// w its block of W, p / h its psi / Hpsi columns).
+template <typename T>
+__device__ inline void col_fft_body(const cplx<T>* __restrict__ w, const cplx<double>* __restrict__ p,
+                                    cplx<double>* __restrict__ h, const int* __restrict__ ent_igw,
+                                    const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol,
+                                    int C, int c0, FftDim dz, const cplx<T>* __restrict__ twz,
+                                    const double* __restrict__ gk2, int kin, int acc) {
   extern __shared__ char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int n3 = dz.n, cs = odd_stride(n3);
   cplx<T>* tw = lds + C * cs;
-  int bl, c0;
-  if (!col_block(ncol, C, nbb, c0, bl)) return;
   const int cn = min(C, ncol - c0);
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const cplx<T>* w = W + (long)bl * n3 * ncol;
   for (int i = tid; i < cn * n3; i += nthr) {
     const int cl = i % cn, z = i / cn;
     lds[cl * cs + z] = w[(long)z * ncol + c0 + cl];
@@ -454,8 +466,6 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
   __syncthreads();
   const int e0 = col_ptr[c0], e1 = col_ptr[c0 + cn];
   const int off = c0 * cs;
-  const cplx<double>* p = psi + (long)(b0 + bl) * ldpsi;
-  cplx<double>* h = hpsi + (long)(b0 + bl) * ldh;
   for (int e = e0 + tid; e < e1; e += nthr) {
     const int g = ent_igw[e];
     cplx<double> r = cvt<double>(lds[ent_pos[e] - off]);
@@ -463,6 +473,74 @@ k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, l
     if (acc) r = r + h[g];
     h[g] = r;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(COL_NT)
+k_col_fft(const cplx<T>* __restrict__ W, const cplx<double>* __restrict__ psi, long ldpsi,
+          cplx<double>* __restrict__ hpsi, long ldh, int b0, const int* __restrict__ ent_igw,
+          const int* __restrict__ ent_pos, const int* __restrict__ col_ptr, int ncol, int C, int nbb,
+          FftDim dz, const cplx<T>* __restrict__ twz, const double* __restrict__ gk2, int kin,
+          int acc) {
+  int bl, c0;
+  if (!col_block(ncol, C, nbb, c0, bl)) return;
+  col_fft_body<T>(W + (long)bl * dz.n * ncol, psi + (long)(b0 + bl) * ldpsi, hpsi + (long)(b0 + bl) * ldh,
+                  ent_igw, ent_pos, col_ptr, ncol, C, c0, dz, twz, gk2, kin, acc);
+}
+
+// ---- batched forms: several k-points (same band count nb) in one launch per stage.
+// Column kernels: k-point kk owns workgroups [kb[kk].blk0, kb[kk+1].blk0), a multiple of 8,
+// mapped to (band, chunk) as col_block does.  Plane kernel: blockIdx.y = kk * nb + band.
+__device__ inline int batch_find(const KBatch* __restrict__ kb, int nk, int w) {
+  int lo = 0, hi = nk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (kb[mid].blk0 <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline bool batch_col_block(const KBatch& d, int C, int nb, int wl, int& c0, int& bl) {
+  const int nchunk = (d.ncol + C - 1) / C;
+  const int s = wl >> 3;
+  bl = (wl & 7) + 8 * (s / nchunk);
+  c0 = (s % nchunk) * C;
+  return bl < nb;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(COL_NT)
+k_col_ifft_batch(const KBatch* __restrict__ kb, int nk, int nb, cplx<T>* __restrict__ W, int C, FftDim dz,
+                 const cplx<T>* __restrict__ twz) {
+  const int w = blockIdx.x;
+  const KBatch& d = kb[batch_find(kb, nk, w)];
+  int bl, c0;
+  if (!batch_col_block(d, C, nb, w - d.blk0, c0, bl)) return;
+  col_ifft_body<T>(d.psi + (long)bl * d.ldpsi, W + d.woff + (long)bl * dz.n * d.ncol, d.ent_igw, d.ent_pos,
+                   d.col_ptr, d.ncol, C, c0, dz, twz);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_plane_vloc_batch(const KBatch* __restrict__ kb, int nb, cplx<T>* __restrict__ W, const T* __restrict__ V,
+                   FftDim dx, FftDim dy, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
+                   int n3) {
+  const int z = blockIdx.x, kk = blockIdx.y / nb, bl = blockIdx.y - kk * nb;
+  const KBatch& d = kb[kk];
+  plane_vloc_body<T>(W + d.woff + ((long)bl * n3 + z) * d.ncol, V, z, d.col_ij, d.ncol, d.xact, d.nxact, dx, dy,
+                     twx, twy);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(COL_NT)
+k_col_fft_batch(const KBatch* __restrict__ kb, int nk, int nb, const cplx<T>* __restrict__ W, int C, FftDim dz,
+                const cplx<T>* __restrict__ twz, int kin, int acc) {
+  const int w = blockIdx.x;
+  const KBatch& d = kb[batch_find(kb, nk, w)];
+  int bl, c0;
+  if (!batch_col_block(d, C, nb, w - d.blk0, c0, bl)) return;
+  col_fft_body<T>(W + d.woff + (long)bl * dz.n * d.ncol, d.psi + (long)bl * d.ldpsi, d.hpsi + (long)bl * d.ldh,
+                  d.ent_igw, d.ent_pos, d.col_ptr, d.ncol, C, c0, dz, twz, d.gk2, kin, acc);
 }
 
 // rho planes: per (z-plane, band group): y/x inverse FFT of each band and
@@ -613,6 +691,72 @@ int vloc_apply(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, lo
 int vloc_apply_k(pw_ctx* c, KData& k, const double* V64, int nb, const cplx<double>* psi, long ldpsi,
                  cplx<double>* out, long ldout) {
   return vloc_run<double>(c, k, V64, nb, psi, ldpsi, out, ldout, 0, 0);
+}
+
+template <typename T>
+static int vloc_batch_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, const cplx<double>* const* psi,
+                          const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, int kin, int acc) {
+  const size_t esz = sizeof(cplx<T>);
+  const int C = cols_per_block(c, esz);
+  if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
+    return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
+  const int nb8 = (nb + 7) / 8 * 8;
+  std::vector<KBatch> kb(nk);
+  long woff = 0;
+  long blk = 0;
+  int maxx = 0;
+  for (int kk = 0; kk < nk; ++kk) {
+    const KData& k = c->kd[iks[kk]];
+    KBatch d{};
+    d.ent_igw = k.d_ent_igw;
+    d.ent_pos = k.d_ent_pos;
+    d.col_ptr = k.d_col_ptr;
+    d.col_ij = k.d_col_ij;
+    d.xact = k.d_xact;
+    d.gk2 = k.d_gk2;
+    d.psi = psi[kk];
+    d.hpsi = hpsi[kk];
+    d.ldpsi = ldpsi[kk];
+    d.ldh = ldh[kk];
+    d.woff = woff;
+    d.ncol = k.ncol;
+    d.nxact = k.nxact;
+    d.blk0 = (int)blk;
+    kb[kk] = d;
+    woff += (long)nb * c->n3 * k.ncol;
+    blk += (long)((k.ncol + C - 1) / C) * nb8;
+    maxx = std::max(maxx, k.nxact);
+  }
+  if (blk > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "op_H batch too large for one launch");
+  const size_t plane_bytes = plane_lds(c, esz, maxx);
+  if (plane_bytes > 160 * 1024)
+    return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
+  int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)woff * esz);
+  if (rc) return rc;
+  const void* dptr = nullptr;
+  rc = desc_get(c, kb.data(), kb.size() * sizeof(KBatch), &dptr);
+  if (rc) return rc;
+  const KBatch* dkb = reinterpret_cast<const KBatch*>(dptr);
+  cplx<T>* W = reinterpret_cast<cplx<T>*>(c->d_work);
+  const cplx<T>* tw[3];
+  for (int d = 0; d < 3; ++d)
+    tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
+  hipLaunchKernelGGL(k_col_ifft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb,
+                     nk, nb, W, C, c->dim[2], tw[2]);
+  hipLaunchKernelGGL(k_plane_vloc_batch<T>, dim3(c->n3, nk * nb), dim3(PlaneNT<T>::value), plane_bytes, c->stream,
+                     dkb, nb, W, V, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+  hipLaunchKernelGGL(k_col_fft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb, nk,
+                     nb, W, C, c->dim[2], tw[2], kin, acc);
+  return hip_check(c, hipGetLastError(), "batched vloc kernels");
+}
+
+int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
+                     const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
+  const int kin = (flags & PW_TERM_K) ? 1 : 0;
+  const int acc = (flags & PW_ACCUMULATE) ? 1 : 0;
+  if (flags & PW_MIXED)
+    return vloc_batch_run<float>(c, nk, iks, c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
+  return vloc_batch_run<double>(c, nk, iks, c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
 }
 
 int kinetic_apply(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,

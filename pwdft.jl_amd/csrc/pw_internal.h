@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,46 @@ struct KData {
   std::vector<int> h_xact;
 };
 
+// One k-point of a batched op_H launch (pw_op_H_batch_dev): its sphere bookkeeping, its
+// psi / Hpsi columns and its block of the column work array.
+struct KBatch {
+  const int* ent_igw;
+  const int* ent_pos;
+  const int* col_ptr;
+  const int* col_ij;
+  const int* xact;
+  const double* gk2;
+  const cplx<double>* psi;
+  cplx<double>* hpsi;
+  long ldpsi, ldh;
+  long woff;   // first element of this k-point's block of W (nb x n3 x ncol)
+  int ncol, nxact;
+  int blk0;    // first column-kernel workgroup of this k-point (a multiple of 8)
+  int pad;
+};
+
+// One product of a batched complex GEMM (pw_zgemm_batch_dev).
+struct GBatch {
+  const cplx<double>* A;
+  const cplx<double>* B;
+  cplx<double>* C;
+  long lda, ldb, ldc;
+  int m, n, k;
+  int tile0;   // first workgroup of this product
+};
+
+// One column block of a batched element-wise launch (copy, Kprec): out = in, or
+// out = in / (1 + |G+k|^2) when gk2 is set.
+struct EBatch {
+  const cplx<double>* in;
+  cplx<double>* out;
+  const double* gk2;
+  long ldin, ldout;
+  int m, n;
+  int blk0;
+  int pad;
+};
+
 }  // namespace pw
 
 struct pw_ctx {
@@ -156,6 +197,16 @@ struct pw_ctx {
   void* d_fmat = nullptr;  size_t fmat_bytes = 0;   // beta^H psi and ((G+k)_alpha beta)^H psi
   void* d_fgrid = nullptr; size_t fgrid_bytes = 0;  // rho / N on the grid, delta_G0, rho(G)
   void* d_fpart = nullptr; size_t fpart_bytes = 0;  // per-block partial sums
+  // batched launches: device copies of descriptor arrays, keyed by content (pw::desc_get)
+  std::vector<std::vector<char>> desc_host;
+  std::vector<void*> desc_dev;
+  std::vector<size_t> desc_cap;
+  int desc_next = 0;
+  void* d_bnlb = nullptr;  size_t bnlb_bytes = 0;   // beta^H psi of every k-point of a batch
+  // dense solvers (pw_dense.hip): rocBLAS handle bound to `stream`, info words, eigenvalues
+  void* rb = nullptr;
+  void* d_dinfo = nullptr; size_t dinfo_bytes = 0;
+  void* d_dwork = nullptr; size_t dwork_bytes = 0;
 };
 
 namespace pw {
@@ -193,6 +244,19 @@ void free_k(KData& k);
 template <typename T>
 int upload(pw_ctx* c, T** dst, const T* src, size_t n);
 void free_forces(pw_ctx* c);
+
+// Batched op_V_loc (+ kinetic) over the k-points iks (pw_fft.hip); psi / hpsi per k-point.
+int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
+                     const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags);
+// Batched complex GEMM C_i = alpha op(A_i) B_i + beta C_i (pw_batch.hip).
+int zgemm_batch(pw_ctx* c, int transa, const std::vector<GBatch>& g, cplx<double> alpha, cplx<double> beta);
+// Device copy of a descriptor array (bytes at `data`).  Arrays are cached by content in up
+// to 64 slots, so a call sequence that repeats (the steps of an eigensolver iteration)
+// uploads nothing after its first pass.  A miss synchronises the stream before it
+// overwrites a slot, since kernels still queued may read it.
+int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out);
+void free_desc(pw_ctx* c);
+void free_dense(pw_ctx* c);
 
 // GEMM drivers (pw_gemm.hip)
 // C (M x N) = alpha * op(A) * B + beta * C ; op(A) = A (transa = 0, A is M x K) or

@@ -18,7 +18,7 @@ module PWDFTHip
 
 using PWDFT
 import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
-              calc_rhoe!, Hamiltonian, PsPot_GTH, BlochWavefunc
+              calc_rhoe!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH, BlochWavefunc
 
 const libpw = joinpath(@__DIR__, "..", "pwdft_hip", "libpwdft_hip.so")
 
@@ -85,6 +85,7 @@ function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
     end
     CONTEXTS[Ham] = ctx
     sync_potential!(Ham)
+    sync_atoms!(Ham)
     return ctx
 end
 
@@ -143,6 +144,98 @@ function calc_rhoe!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Mat
         PWDFT.symmetrize_rhoe!(Ham.pw, Ham.sym_info, Ham.rhoe_symmetrizer, Rhoe)
     end
     return
+end
+
+# ---- forces and the adjoint (calc_forces_Ps_loc.jl:1-71, calc_forces_Ps_nloc.jl:291-411) ----
+
+"""Upload the atoms, projector-to-atom map and the rho-sphere form factors V_sp(|G|) the
+force kernels read (call again after atoms move, with the rebuilt betaNL)."""
+function sync_atoms!(Ham::Hamiltonian{PsPot_GTH})
+    ctx = context(Ham)
+    atoms, pw = Ham.atoms, Ham.pw
+    nb = Ham.pspotNL.NbetaNL
+    beta_atom = zeros(Cint, nb)
+    for ia in 1:atoms.Natoms
+        psp = Ham.pspots[atoms.atm2species[ia]]
+        for l in 0:psp.lmax, m in -l:l, i in 1:psp.Nproj_l[l+1]
+            beta_atom[Ham.pspotNL.prj2beta[i, ia, l+1, m+psp.lmax+1]] = ia - 1
+        end
+    end
+    species = Cint.(atoms.atm2species .- 1)
+    check(ctx, ccall((:pw_set_atoms, libpw), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Cint}, Cint, Ptr{Cint}),
+                     ctx.h, atoms.Natoms, atoms.positions, species, nb, beta_atom))
+    Ng = pw.gvec.Ng
+    vg = zeros(Float64, Ng, atoms.Nspecies)
+    for isp in 1:atoms.Nspecies
+        Vgl = zeros(Float64, length(pw.gvec.G2_shells))
+        PWDFT.eval_Vloc_G!(Ham.pspots[isp], pw.gvec.G2_shells, Vgl)
+        vg[:, isp] = Vgl[pw.gvec.idx_g2shells]
+    end
+    check(ctx, ccall((:pw_set_rho_sphere, libpw), Cint,
+                     (Ptr{Cvoid}, Cint, Ptr{Int64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}),
+                     ctx.h, Ng, pw.gvec.idx_g2r, 1, pw.gvec.G, atoms.Nspecies, vg))
+    return nothing
+end
+
+"""calc_forces_Ps_loc(Ham, Rhoe) on the device (rho = sum over spin of Rhoe)."""
+function calc_forces_Ps_loc(Ham::Hamiltonian{PsPot_GTH}, Rhoe::Matrix{Float64})
+    ctx = context(Ham)
+    F = zeros(Float64, 3, Ham.atoms.Natoms)
+    rho = vec(sum(Rhoe, dims=2))
+    check(ctx, ccall((:pw_calc_forces_Ps_loc, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.h, rho, F))
+    return F
+end
+
+"""calc_forces_Ps_nloc(Ham, psiks) on the device (GTH), symmetrised as calc_forces.jl:11."""
+function calc_forces_Ps_nloc(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc)
+    ctx = context(Ham)
+    F = zeros(Float64, 3, Ham.atoms.Natoms)
+    wk, Focc = Ham.pw.gvecw.kpoints.wk, Ham.electrons.Focc
+    for ik in 1:Ham.pw.gvecw.kpoints.Nkpt
+        psi = psiks[ik]
+        w = wk[ik] .* Focc[:, ik]
+        check(ctx, ccall((:pw_calc_forces_Ps_nloc, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.h, ik - 1, size(psi, 2), psi, size(psi, 1), w, F, C_NULL))
+    end
+    PWDFT.symmetrize_vector!(Ham.pw.LatVecs, Ham.sym_info, F)
+    return F
+end
+
+"""E_Ps_loc at fixed Rhoe as a function of the form factors Vgl[shell, species] and the
+positions, with its ChainRulesCore rrule backed by pw_E_Ps_loc_grad (install with
+`using ChainRulesCore` before including this file)."""
+function E_Ps_loc(Ham::Hamiltonian{PsPot_GTH}, Rhoe::Matrix{Float64})
+    ctx = context(Ham)
+    E = Ref(0.0)
+    check(ctx, ccall((:pw_E_Ps_loc_grad, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.h, vec(sum(Rhoe, dims=2)), E, C_NULL, C_NULL))
+    return E[]
+end
+
+if isdefined(Main, :ChainRulesCore)
+    import ChainRulesCore
+    function ChainRulesCore.rrule(::typeof(E_Ps_loc), Ham::Hamiltonian{PsPot_GTH}, Rhoe::Matrix{Float64})
+        ctx = context(Ham)
+        pw = Ham.pw
+        E = Ref(0.0)
+        dvg = zeros(Float64, pw.gvec.Ng, Ham.atoms.Nspecies)
+        dtau = zeros(Float64, 3, Ham.atoms.Natoms)
+        check(ctx, ccall((:pw_E_Ps_loc_grad, libpw), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.h, vec(sum(Rhoe, dims=2)), E, dvg, dtau))
+        # cotangent of the shell form factors: sum over the G vectors of each shell
+        dshell = zeros(Float64, length(pw.gvec.G2_shells), Ham.atoms.Nspecies)
+        for ig in 1:pw.gvec.Ng
+            dshell[pw.gvec.idx_g2shells[ig], :] .+= dvg[ig, :]
+        end
+        pullback(Ebar) = (ChainRulesCore.NoTangent(),
+                          ChainRulesCore.Tangent{typeof(Ham)}(; atoms = (; positions = Ebar .* dtau),
+                                                                Vgl = Ebar .* dshell),
+                          ChainRulesCore.NoTangent())
+        return E[], pullback
+    end
 end
 
 """KS_solve_SCF! with the potential kept in sync on the device after every update!."""

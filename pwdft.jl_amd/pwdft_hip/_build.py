@@ -4,7 +4,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
-SOURCES = ["pw_capi.hip", "pw_fft.hip", "pw_gemm.hip", "pw_forces.hip"]
+SOURCES = ["pw_capi.hip", "pw_fft.hip", "pw_gemm.hip", "pw_forces.hip", "pw_batch.hip", "pw_dense.hip"]
 OUT = os.path.join(HERE, "libpwdft_hip.so")
 
 HIPCC_FLAGS = [
@@ -15,11 +15,15 @@ HIPCC_FLAGS = [
 ]
 
 
+# rocSOLVER for the small dense Cholesky / eigen problems of the eigensolvers
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+
+
 def build_library(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -48,6 +48,15 @@ _PROTOS = {
     "pw_rhoe_accum": [_vp, _i, _i, _vp, _i64, _vp, _c.c_uint],
     "pw_rhoe_accum_dev": [_vp, _i, _i, _vp, _i64, _vp, _c.c_uint],
     "pw_rhoe_get": [_vp, _vp],
+    "pw_op_H_batch_dev": [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _c.c_uint],
+    "pw_kprec_batch_dev": [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
+    "pw_copy_cols_batch_dev": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pw_herm_fix_dev": [_vp, _i, _vp, _i64, _i],
+    "pw_potrf_dev": [_vp, _i, _vp, _i64],
+    "pw_trtri_dev": [_vp, _i, _vp, _i64],
+    "pw_heevd_dev": [_vp, _i, _vp, _i64, _vp],
+    "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
+    "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pw_set_atoms": [_vp, _i, _vp, _vp, _i, _vp],
     "pw_set_rho_sphere": [_vp, _i, _vp, _i, _vp, _i, _vp],
     "pw_E_Ps_loc_grad": [_vp, _vp, _vp, _vp, _vp],

@@ -1,0 +1,286 @@
+"""diag_LOBPCG! over all k-points at once (diag_LOBPCG.jl:1-225 run in lockstep over k).
+
+The reference diagonalises k-point after k-point; with many small k-points (hundreds of
+k-points of a few hundred plane waves and a handful of bands) every step of that loop is a
+tiny launch.  Here every step of one LOBPCG iteration is one batched launch for all k-points
+that have not converged yet (pw_op_H_batch_dev, pw_zgemm_batch_dev, pw_kprec_batch_dev,
+pw_copy_cols_batch_dev), and the small dense problems of all k-points (Nst..3Nst) are
+solved together as stacked numpy arrays.  A k-point leaves the batch when it meets the
+reference's convergence test; one that reaches the reference's lock branch finishes that
+iteration through the per-k code (solvers._lobpcg_lock_step).  The arithmetic per k-point is
+that of solvers.diag_LOBPCG_."""
+import ctypes
+
+import numpy as np
+
+from ._lib import DeviceMatrix, PosDefException, PW_TERM_ALL, ptr
+from .solvers import EPS, _Dense, _lobpcg_lock_step
+
+_ONE = np.array([1.0, 0.0])
+_ZERO = np.array([0.0, 0.0])
+_MONE = np.array([-1.0, 0.0])
+
+
+def _pa(vals):
+    return (ctypes.c_void_p * len(vals))(*vals)
+
+
+class _Slab:
+    """One device allocation holding, for every k-point, an Ngw[k] x ncols column-major
+    block (leading dimension Ngw[k])."""
+
+    def __init__(self, ctx, rows, ncols):
+        self.rows = [int(r) for r in rows]
+        self.ncols = ncols
+        self.off = np.concatenate([[0], np.cumsum([16 * r * ncols for r in self.rows])]).astype(np.int64)
+        self.mem = DeviceMatrix(ctx, int(self.off[-1] // 16) or 1, 1)
+        self.base = self.mem.p.value
+
+    def col(self, k, j):
+        return self.base + int(self.off[k]) + 16 * self.rows[k] * int(j)
+
+    def view(self, k):
+        return _View(self.col(k, 0), self.rows[k], self.ncols)
+
+    def free(self):
+        self.mem.free()
+
+
+class _View:
+    """Column-block view with the DeviceMatrix interface the per-k lock step uses."""
+
+    def __init__(self, p, rows, cols):
+        self.p = ctypes.c_void_p(p)
+        self.rows, self.cols = rows, cols
+
+    def col(self, j):
+        return ctypes.c_void_p(self.p.value + 16 * self.rows * int(j))
+
+
+class _Small:
+    """Stacked small matrices (nk x r x c) in one device buffer, column-major per k."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.buf = None
+
+    def ptrs(self, nk, r, c):
+        need = nk * r * c
+        if self.buf is None or self.buf.rows < need:
+            if self.buf is not None:
+                self.buf.free()
+            self.buf = DeviceMatrix(self.ctx, max(need, 1), 1)
+        return [self.buf.p.value + 16 * r * c * i for i in range(nk)]
+
+    def upload(self, M):
+        """M: nk x r x c complex -> device pointers of each k's column-major block."""
+        nk, r, c = M.shape
+        p = self.ptrs(nk, r, c)
+        a = np.ascontiguousarray(np.transpose(M, (0, 2, 1)), dtype=np.complex128)   # column-major blocks
+        self.ctx.call("pw_memcpy_h2d", ctypes.c_void_p(p[0]), ptr(a), a.nbytes)
+        return p
+
+    def download(self, nk, r, c):
+        a = np.empty((nk, c, r), dtype=np.complex128)
+        self.ctx.call("pw_memcpy_d2h", ptr(a), ctypes.c_void_p(self.buf.p.value), a.nbytes)
+        return np.transpose(a, (0, 2, 1))
+
+    def free(self):
+        if self.buf is not None:
+            self.buf.free()
+
+
+class _Batch:
+    """Batched device operations on the active k-points."""
+
+    def __init__(self, Ham, mixed):
+        self.Ham = Ham
+        self.c = Ham.ctx
+        self.mixed = mixed
+        self.small = _Small(Ham.ctx)
+        self.small2 = _Small(Ham.ctx)
+
+    def gemm(self, transa, m, n, k, A, lda, B, ldb, C, ldc, alpha=_ONE, beta=_ZERO):
+        nb = len(A)
+        i32 = lambda v: np.ascontiguousarray(np.broadcast_to(v, nb), dtype=np.int32)
+        i64 = lambda v: np.ascontiguousarray(np.broadcast_to(v, nb), dtype=np.int64)
+        ms, ns, ks = i32(m), i32(n), i32(k)
+        la, lb, lc = i64(lda), i64(ldb), i64(ldc)
+        self.c.call("pw_zgemm_batch_dev", transa, nb, ptr(ms), ptr(ns), ptr(ks), ptr(alpha), _pa(A), ptr(la), _pa(B),
+                    ptr(lb), ptr(beta), _pa(C), ptr(lc))
+
+    def gram(self, A, B, na, nbc, rows):
+        """A_k^H B_k for column blocks -> host nk x na x nbc."""
+        nk = len(A)
+        out = self.small.ptrs(nk, na, nbc)
+        self.gemm(1, na, nbc, rows, A, rows, B, rows, out, na)
+        return self.small.download(nk, na, nbc)
+
+    def copy(self, rows, ncols, src, dst):
+        nb = len(src)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        cl = np.full(nb, ncols, dtype=np.int32)
+        ld = np.ascontiguousarray(rows, dtype=np.int64)
+        self.c.call("pw_copy_cols_batch_dev", nb, ptr(r), ptr(cl), _pa(src), ptr(ld), _pa(dst), ptr(ld))
+
+    def opH(self, iks, src, dst, ncols, rows):
+        from ._lib import PW_MIXED
+        ik = np.ascontiguousarray(iks, dtype=np.int32)
+        ld = np.ascontiguousarray(rows, dtype=np.int64)
+        self.c.call("pw_op_H_batch_dev", len(iks), ptr(ik), self.Ham.ispin, ncols, _pa(src), ptr(ld), _pa(dst), ptr(ld),
+                    PW_TERM_ALL | (PW_MIXED if self.mixed else 0))
+
+    def kprec(self, iks, src, dst, ncols, rows):
+        ik = np.ascontiguousarray(iks, dtype=np.int32)
+        ld = np.ascontiguousarray(rows, dtype=np.int64)
+        self.c.call("pw_kprec_batch_dev", len(iks), ptr(ik), ncols, _pa(src), ptr(ld), _pa(dst), ptr(ld))
+
+    def free(self):
+        self.small.free()
+        self.small2.free()
+
+
+def _herm_u(A):
+    """Hermitian(A) of each stacked matrix: upper triangle mirrored, real diagonal."""
+    U = np.triu(A, 1)
+    d = np.real(np.diagonal(A, axis1=-2, axis2=-1))
+    out = U + np.conj(np.swapaxes(U, -1, -2))
+    idx = np.arange(A.shape[-1])
+    out[..., idx, idx] = d
+    return out
+
+
+def _chol_upper_inv(C, iks):
+    """cholesky(Hermitian(C)).U and its inverse for every stacked matrix."""
+    try:
+        L = np.linalg.cholesky(_herm_u(C))
+    except np.linalg.LinAlgError as e:
+        raise PosDefException(-4, f"cholesky (k-points {list(iks)}): {e}")
+    U = np.conj(np.swapaxes(L, -1, -2))
+    return U, np.linalg.inv(U)
+
+
+def _geneig_lowest(T, G, n):
+    """Lowest n eigenvectors of T x = l G x for every stacked pair (eigen(T, G))."""
+    try:
+        L = np.linalg.cholesky(G)
+    except np.linalg.LinAlgError as e:
+        raise PosDefException(-4, f"eigen(T, G): {e}")
+    Li = np.linalg.inv(L)
+    A = Li @ T @ np.conj(np.swapaxes(Li, -1, -2))
+    A = 0.5 * (A + np.conj(np.swapaxes(A, -1, -2)))
+    _, Y = np.linalg.eigh(A)
+    return np.conj(np.swapaxes(Li, -1, -2)) @ Y[..., :n]
+
+
+def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, iks=None):
+    """diag_LOBPCG!(Ham, psiks) with all k-points iks (default: all) iterated together.
+    psiks: list of DeviceMatrix (Ngw[ik] x Nst), overwritten with the Ritz vectors.
+    Returns (evals Nst x nk, stats per k-point {"iter", "ncols"})."""
+    mixed = Ham.mixed if mixed is None else mixed
+    iks = list(range(Ham.pw.Nkpt)) if iks is None else list(iks)
+    nk = len(iks)
+    Nst = psiks[0].cols
+    if Nstates_conv == 0:
+        Nstates_conv = Nst
+    rows = [psiks[i].rows for i in range(nk)]
+    c = Ham.ctx
+    B = _Batch(Ham, mixed)
+    Q = _Slab(c, rows, 3 * Nst)
+    HQ = _Slab(c, rows, 3 * Nst)
+    Tmp = _Slab(c, rows, 4 * Nst)
+    allk = list(range(nk))
+    cols = lambda S, ks, j: [S.col(k, j) for k in ks]
+    rws = lambda ks: [rows[k] for k in ks]
+    B.copy(rws(allk), Nst, [psiks[k].p.value for k in allk], cols(Q, allk, 0))
+    B.opH([iks[k] for k in allk], cols(Q, allk, 0), cols(HQ, allk, 0), Nst, rws(allk))
+    ncols = np.full(nk, Nst)
+    iters = np.zeros(nk, dtype=int)
+    evals_old = np.zeros((nk, Nst))
+    SMALL = 10.0 * EPS
+    active = list(allk)
+    for it in range(1, NiterMax + 1):
+        if not active:
+            break
+        a = active
+        ka = [iks[k] for k in a]
+        iters[a] = it
+        x, hx, w, hw = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, Nst), cols(HQ, a, Nst)
+        XHX = _herm_u(B.gram(x, hx, Nst, Nst, rws(a)))
+        evals = np.linalg.eigvalsh(XHX)
+        devals = np.abs(evals - evals_old[a])
+        evals_old[a] = evals
+        # R = HX - X*XHX (into the W slot)
+        B.copy(rws(a), Nst, hx, w)
+        dX = B.small2.upload(XHX)
+        B.gemm(0, rws(a), Nst, Nst, x, rws(a), dX, Nst, w, rws(a), alpha=_MONE, beta=_ONE)
+        nconv = np.count_nonzero(devals < tol, axis=1)
+        done = nconv >= Nstates_conv
+        lock = (~done) & np.any(devals <= SMALL, axis=1)
+        keep = [k for k, d, l in zip(a, done, lock) if not d and not l]
+        locked = [k for k, d, l in zip(a, done, lock) if l]
+        if locked:
+            for k in locked:
+                B.kprec([iks[k]], [Q.col(k, Nst)], [Q.col(k, Nst)], Nst, [rows[k]])
+                D = _Dense(Ham, mixed)
+                Ham.ik = iks[k]
+                dv = devals[a.index(k)]
+                ilock = np.flatnonzero(dv <= SMALL)
+                iact = np.flatnonzero(dv > SMALL)
+                ncols[k] += _lobpcg_lock_step(D, Q.view(k), HQ.view(k), Tmp.view(k), rows[k], Nst, ilock, iact)
+        active = keep + locked
+        active.sort()
+        a = keep
+        if not a:
+            continue
+        ka = [iks[k] for k in a]
+        ra = rws(a)
+        w, hw = cols(Q, a, Nst), cols(HQ, a, Nst)
+        B.kprec(ka, w, w, Nst, ra)
+        B.opH(ka, w, hw, Nst, ra)
+        ncols[a] += Nst
+        U, Ui = _chol_upper_inv(B.gram(w, w, Nst, Nst, ra), ka)
+        dUi = B.small2.upload(Ui)
+        t0, t1 = cols(Tmp, a, 0), cols(Tmp, a, Nst)
+        B.gemm(0, ra, Nst, Nst, w, ra, dUi, Nst, t0, ra)
+        B.gemm(0, ra, Nst, Nst, hw, ra, dUi, Nst, t1, ra)
+        B.copy(ra, Nst, t0, w)
+        B.copy(ra, Nst, t1, hw)
+        nq = 3 * Nst if it > 1 else 2 * Nst
+        qa, hqa = cols(Q, a, 0), cols(HQ, a, 0)
+        T = _herm_u(B.gram(qa, hqa, nq, nq, ra))
+        G = _herm_u(B.gram(qa, qa, nq, nq, ra))
+        Um = _geneig_lowest(T, G, Nst)
+        dU = B.small2.upload(Um)
+        B.gemm(0, ra, Nst, nq, qa, ra, dU, nq, t0, ra)
+        B.gemm(0, ra, Nst, nq, hqa, ra, dU, nq, t1, ra)
+        x, hx, p, hp = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, 2 * Nst), cols(HQ, a, 2 * Nst)
+        if it > 1:
+            dU23 = B.small2.upload(np.ascontiguousarray(Um[:, Nst:3 * Nst, :]))
+            t2, t3 = cols(Tmp, a, 2 * Nst), cols(Tmp, a, 3 * Nst)
+            B.gemm(0, ra, Nst, 2 * Nst, w, ra, dU23, 2 * Nst, t2, ra)
+            B.gemm(0, ra, Nst, 2 * Nst, hw, ra, dU23, 2 * Nst, t3, ra)
+            B.copy(ra, Nst, t0, x)
+            B.copy(ra, Nst, t1, hx)
+            U, Ui = _chol_upper_inv(B.gram(t2, t2, Nst, Nst, ra), ka)
+            dUi = B.small2.upload(Ui)
+            B.gemm(0, ra, Nst, Nst, t2, ra, dUi, Nst, p, ra)
+            B.gemm(0, ra, Nst, Nst, t3, ra, dUi, Nst, hp, ra)
+        else:
+            B.copy(ra, Nst, w, p)
+            B.copy(ra, Nst, hw, hp)
+            B.copy(ra, Nst, t0, x)
+            B.copy(ra, Nst, t1, hx)
+    # final Rayleigh-Ritz of every k-point
+    x, hx = cols(Q, allk, 0), cols(HQ, allk, 0)
+    S = B.gram(x, hx, Nst, Nst, rws(allk))
+    S = _herm_u(0.5 * (S + np.conj(np.swapaxes(S, -1, -2))))
+    ev, Qm = np.linalg.eigh(S)
+    dQ = B.small2.upload(Qm)
+    B.gemm(0, rws(allk), Nst, Nst, x, rws(allk), dQ, Nst, [psiks[k].p.value for k in allk], rws(allk))
+    c.call("pw_synchronize")
+    for s in (Q, HQ, Tmp):
+        s.free()
+    B.free()
+    stats = [{"iter": int(iters[k]), "ncols": int(ncols[k])} for k in allk]
+    return ev.T.copy(), stats

@@ -14,6 +14,7 @@ Ham.ik / Ham.ispin are the reference's mutable cursor (Hamiltonian.jl:16-17).
 op_H_ skips op_Vexx when there is no EXX object: the reference's op_H! calls op_Vexx!
 unconditionally and fails for every Hamiltonian it builds (op_H.jl:61, Hamiltonian.jl:293).
 """
+import ctypes
 import math
 
 import numpy as np
@@ -234,9 +235,35 @@ def _per_k(Ham, psiks, fn):
     return out
 
 
+def _ptr_array(ptrs):
+    return (ctypes.c_void_p * len(ptrs))(*[p.value for p in ptrs])
+
+
+def op_H_batch(Ham, psiks, out=None, terms=PW_TERM_ALL, accumulate=False, iks=None):
+    """op_H over the k-points iks (default: all) of a BlochWavefunc of DeviceMatrix in one
+    launch per stage (pw_op_H_batch_dev); every k-point must have the same band count.
+    Returns the list of H psi (DeviceMatrix)."""
+    iks = list(range(Ham.pw.Nkpt)) if iks is None else list(iks)
+    nb = psiks[0].cols
+    if any(p.cols != nb for p in psiks):
+        raise ValueError("op_H_batch: every k-point needs the same number of bands")
+    if out is None:
+        out = [DeviceMatrix(Ham.ctx, p.rows, p.cols) for p in psiks]
+    flags = terms | Ham.flags_prec | (PW_ACCUMULATE if accumulate else 0)
+    ik_a = np.ascontiguousarray(iks, dtype=np.int32)
+    ldp = np.ascontiguousarray([p.rows for p in psiks], dtype=np.int64)
+    ldh = np.ascontiguousarray([h.rows for h in out], dtype=np.int64)
+    Ham.ctx.call("pw_op_H_batch_dev", len(iks), ptr(ik_a), Ham.ispin, nb, _ptr_array([p.p for p in psiks]),
+                 ptr(ldp), _ptr_array([h.p for h in out]), ptr(ldh), flags)
+    return out
+
+
 def op_H(Ham, psi):
-    """op_H.jl:4-27 (allocating): K + V_loc + V_Ps_nloc (+ op_Vtau = 0 for LDA)."""
+    """op_H.jl:4-27 (allocating): K + V_loc + V_Ps_nloc (+ op_Vtau = 0 for LDA).  A list of
+    DeviceMatrix (a BlochWavefunc on the device) goes through the batched launch."""
     if isinstance(psi, list):
+        if all(isinstance(p, DeviceMatrix) for p in psi) and len({p.cols for p in psi}) == 1:
+            return op_H_batch(Ham, psi)
         return _per_k(Ham, psi, lambda p: op_H(Ham, p))
     return _apply(Ham, psi, PW_TERM_ALL)
 

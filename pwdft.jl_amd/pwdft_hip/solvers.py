@@ -7,6 +7,8 @@ dense FP64 problems the reference hands to LAPACK (Hermitian eigensolves of size
 Nst..3Nst, Cholesky of Nst x Nst, triangular inverses) run on the host, as the reference
 also runs them serially.  X, W and P are column blocks of one buffer Q = [X W P] so that
 Q'HQ and Q'Q are single products."""
+import ctypes
+
 import numpy as np
 import scipy.linalg as sla
 
@@ -81,11 +83,109 @@ def _colptr(M, j):
     return M.col(j)
 
 
-def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None):
+def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
+    """diag_LOBPCG! with the small dense problems solved on the device (pw_potrf_dev,
+    pw_trtri_dev, pw_heevd_dev, pw_hegvd_dev): the Gram and overlap matrices and the Ritz
+    coefficients stay in HBM; only eigenvalues reach the host for the convergence test."""
+    D = _Dense(Ham, mixed)
+    Nb, Nst = X.rows, X.cols
+    c = Ham.ctx
+    Q = DeviceMatrix(c, Nb, 3 * Nst)
+    HQ = DeviceMatrix(c, Nb, 3 * Nst)
+    Tmp = DeviceMatrix(c, Nb, 4 * Nst)
+    Sm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
+    Gm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
+    S2 = DeviceMatrix(c, Nst, Nst)
+    S3 = DeviceMatrix(c, Nst, Nst)
+    x, w, p = Q.col(0), Q.col(Nst), Q.col(2 * Nst)
+    hx, hw, hp = HQ.col(0), HQ.col(Nst), HQ.col(2 * Nst)
+    t0, t1, t2, t3 = Tmp.col(0), Tmp.col(Nst), Tmp.col(2 * Nst), Tmp.col(3 * Nst)
+    ev = np.zeros(3 * Nst)
+    evp = ev.ctypes.data_as(ctypes.c_void_p)
+
+    def chol_inv_apply(src, hsrc, dst, hdst):
+        # U = cholesky(Hermitian(src' src)).U ; dst = src inv(U), hdst = hsrc inv(U)
+        D.gemm(1, Nst, Nst, Nb, src, Nb, src, Nb, S3.p, Nst)
+        c.call("pw_potrf_dev", Nst, S3.p, Nst)
+        c.call("pw_trtri_dev", Nst, S3.p, Nst)
+        c.call("pw_herm_fix_dev", Nst, S3.p, Nst, 1)
+        D.gemm(0, Nb, Nst, Nst, src, Nb, S3.p, Nst, dst, Nb)
+        D.gemm(0, Nb, Nst, Nst, hsrc, Nb, S3.p, Nst, hdst, Nb)
+
+    D.copy(Nb, Nst, X.p, x)
+    D.opH(x, hx, Nst, Nb)
+    ncols = Nst
+    evals_old = np.zeros(Nst)
+    SMALL = 10.0 * EPS
+    it_done = 0
+    for it in range(1, NiterMax + 1):
+        it_done = it
+        D.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, S2.p, Nst)          # X'HX
+        c.call("pw_herm_fix_dev", Nst, S2.p, Nst, 0)
+        c.call("pw_memcpy_d2d", S3.p, S2.p, 16 * Nst * Nst)
+        c.call("pw_heevd_dev", Nst, S3.p, Nst, evp)
+        evals = ev[:Nst].copy()
+        devals = np.abs(evals - evals_old)
+        evals_old = evals.copy()
+        D.copy(Nb, Nst, hx, w)                                      # R = HX - X*XHX
+        D.gemm(0, Nb, Nst, Nst, x, Nb, S2.p, Nst, w, Nb, alpha=_MONE, beta=_ONE)
+        nconv = int(np.count_nonzero(devals < tol))
+        ilock = np.flatnonzero(devals <= SMALL)
+        iact = np.flatnonzero(devals > SMALL)
+        if nconv >= Nstates_conv:
+            break
+        D.kprec(w, w, Nst, Nb)
+        if len(ilock) > 0:
+            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact)
+            continue
+        D.opH(w, hw, Nst, Nb)
+        ncols += Nst
+        chol_inv_apply(w, hw, t0, t1)
+        D.copy(Nb, Nst, t0, w)
+        D.copy(Nb, Nst, t1, hw)
+        nq = 3 * Nst if it > 1 else 2 * Nst
+        D.gemm(1, nq, nq, Nb, Q.p, Nb, HQ.p, Nb, Sm.p, nq)          # T = Q'HQ
+        D.gemm(1, nq, nq, Nb, Q.p, Nb, Q.p, Nb, Gm.p, nq)           # G = Q'Q
+        c.call("pw_hegvd_dev", nq, Sm.p, nq, Gm.p, nq, evp)         # Ritz vectors in Sm
+        D.gemm(0, Nb, Nst, nq, Q.p, Nb, Sm.p, nq, t0, Nb)
+        D.gemm(0, Nb, Nst, nq, HQ.p, Nb, Sm.p, nq, t1, Nb)
+        if it > 1:
+            u23 = ctypes.c_void_p(Sm.p.value + 16 * Nst)              # rows Nst:3Nst of U
+            D.gemm(0, Nb, Nst, 2 * Nst, w, Nb, u23, nq, t2, Nb)
+            D.gemm(0, Nb, Nst, 2 * Nst, hw, Nb, u23, nq, t3, Nb)
+            D.copy(Nb, Nst, t0, x)
+            D.copy(Nb, Nst, t1, hx)
+            chol_inv_apply(t2, t3, p, hp)
+        else:
+            D.copy(Nb, Nst, w, p)
+            D.copy(Nb, Nst, hw, hp)
+            D.copy(Nb, Nst, t0, x)
+            D.copy(Nb, Nst, t1, hx)
+    D.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, S2.p, Nst)
+    c.call("pw_herm_fix_dev", Nst, S2.p, Nst, 2)
+    c.call("pw_heevd_dev", Nst, S2.p, Nst, evp)
+    evals = ev[:Nst].copy()
+    D.gemm(0, Nb, Nst, Nst, x, Nb, S2.p, Nst, X.p, Nb)
+    c.call("pw_synchronize")
+    for m in (Q, HQ, Tmp, Sm, Gm, S2, S3):
+        m.free()
+    return evals, {"iter": it_done, "ncols": ncols}
+
+
+DENSE = "device"   # where diag_LOBPCG_ solves its small dense problems: "device" or "host"
+
+
+def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, dense=None):
     """diag_LOBPCG! for the current Ham.ik (diag_LOBPCG.jl:51-225).  X is a DeviceMatrix
     (Ngw x Nst, orthonormal) overwritten with the Ritz vectors.  Returns (evals, stats)
-    where stats = {"iter": iterations, "ncols": op_H columns applied}."""
+    where stats = {"iter": iterations, "ncols": op_H columns applied}.  dense = "device"
+    (default) runs the Cholesky and eigen problems through rocSOLVER on the context stream;
+    "host" runs them with LAPACK on the host, as the reference does."""
     mixed = Ham.mixed if mixed is None else mixed
+    if Nstates_conv == 0:
+        Nstates_conv = X.cols
+    if (DENSE if dense is None else dense) == "device":
+        return _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed)
     D = _Dense(Ham, mixed)
     Nb, Nst = X.rows, X.cols
     if Nstates_conv == 0:

@@ -1,0 +1,153 @@
+"""Batched k-point launches through the C ABI: op_H over all k-points of the Al fcc
+12x12x12 shifted mesh (182 symmetry-reduced k-points, uneven plane-wave counts) in one
+launch per stage against the per-k op_H and the oracle, and the batched complex GEMM
+against numpy."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from _systems import rand_psi
+
+pytestmark = pytest.mark.gpu
+
+
+def _al(**kw):
+    import pwdft_hip as ph
+    atoms = ph.Atoms(["Al"], [[0, 0, 0]], ph.gen_lattice_fcc(7.6))
+    return ph.Hamiltonian(atoms, ["Al-q3.lda.gth"], 15.0, meshk=(12, 12, 12), shiftk=(1, 1, 1), Nstates=8,
+                          use_smearing=True, smearing_kT=0.01, **kw)
+
+
+@pytest.fixture(scope="module")
+def al():
+    H = _al()
+    H.update(H.guess_rhoe())
+    rng = np.random.default_rng(31)
+    host = [rand_psi(rng, n, 8) for n in H.pw.Ngw]
+    return H, host
+
+
+def test_al_mesh_shape(al):
+    H, _ = al
+    assert H.pw.Nkpt == 182
+    assert min(H.pw.Ngw) < max(H.pw.Ngw)
+    assert abs(H.pw.wk.sum() - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_op_H_batch_vs_per_k(al, mixed):
+    import pwdft_hip as ph
+    H, host = al
+    print(f"case Al 182k op_H batch mixed={mixed}", flush=True)
+    dev = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
+    H.mixed = mixed
+    try:
+        out = ph.op_H_batch(H, dev)
+        out2 = ph.op_H_batch(H, dev)      # cached descriptors
+    finally:
+        H.mixed = False
+    worst = 0.0
+    for ik in range(H.pw.Nkpt):
+        H.ik = ik
+        ref = ph.op_H(H, host[ik])        # per-k FP64 path
+        got = out[ik].download()
+        assert np.array_equal(got, out2[ik].download())
+        worst = max(worst, np.abs(got - ref).max() / np.abs(ref).max())
+    assert worst < (2e-5 if mixed else 1e-13), worst
+
+
+def test_op_H_batch_vs_oracle(al):
+    """A few k-points of the batch against the oracle's op_H (same k list and V)."""
+    import pwdft_hip as ph
+    from _systems import oracle_ham
+    H, host = al
+    oH = oracle_ham(["Al"], [[0, 0, 0]], ph.gen_lattice_fcc(7.6), "Al-q3.lda.gth", 15.0,
+                    kpoints=(H.pw.k, H.pw.wk), Nstates=8)
+    oH.V_loc[:, 0] = H.potentials.Total[:, 0]
+    from oracle import ops as O
+    dev = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
+    out = ph.op_H_batch(H, dev)
+    for ik in (0, 57, 181):
+        oH.ik = ik
+        ref = O.op_H(oH, host[ik])
+        got = out[ik].download()
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), ik
+
+
+def test_op_H_batch_terms_and_accumulate(al):
+    import pwdft_hip as ph
+    from pwdft_hip._lib import PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC
+    H, host = al
+    dev = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host[:20]]
+    iks = list(range(20))
+    full = [m.download() for m in ph.op_H_batch(H, dev, iks=iks)]
+    acc = ph.op_H_batch(H, dev, terms=PW_TERM_K, iks=iks)
+    ph.op_H_batch(H, dev, out=acc, terms=PW_TERM_VLOC, accumulate=True, iks=iks)
+    ph.op_H_batch(H, dev, out=acc, terms=PW_TERM_NLOC, accumulate=True, iks=iks)
+    for a, f in zip(acc, full):
+        assert np.abs(a.download() - f).max() < 1e-13 * np.abs(f).max()
+
+
+def test_zgemm_batch_vs_numpy():
+    import pwdft_hip as ph
+    ctx = ph.Context(0)
+    rng = np.random.default_rng(4)
+    for transa in (0, 1):
+        shapes = [(5, 8, 300), (24, 24, 311), (1, 1, 1), (17, 33, 5), (300, 8, 24), (40, 16, 4634)]
+        As, Bs, Cs, refs = [], [], [], []
+        alpha = np.array([0.7, -0.2])
+        beta = np.array([0.3, 0.1])
+        for m, n, k in shapes:
+            A = rand_psi(rng, k, m) if transa else rand_psi(rng, m, k)
+            B = rand_psi(rng, k, n)
+            C = rand_psi(rng, m, n)
+            opA = A.conj().T if transa else A
+            refs.append(complex(*alpha) * opA @ B + complex(*beta) * C)
+            As.append(ph.DeviceMatrix.from_host(ctx, A))
+            Bs.append(ph.DeviceMatrix.from_host(ctx, B))
+            Cs.append(ph.DeviceMatrix.from_host(ctx, C))
+        arr = lambda xs: (ctypes.c_void_p * len(xs))(*[x.p.value for x in xs])
+        i32 = lambda xs: np.ascontiguousarray(xs, dtype=np.int32)
+        i64 = lambda xs: np.ascontiguousarray(xs, dtype=np.int64)
+        ms, ns, ks = (i32([s[i] for s in shapes]) for i in range(3))
+        from pwdft_hip._lib import ptr
+        ctx.call("pw_zgemm_batch_dev", transa, len(shapes), ptr(ms), ptr(ns), ptr(ks), ptr(alpha), arr(As),
+                 ptr(i64([a.rows for a in As])), arr(Bs), ptr(i64([b.rows for b in Bs])), ptr(beta), arr(Cs),
+                 ptr(i64([c.rows for c in Cs])))
+        for Cd, ref, s in zip(Cs, refs, shapes):
+            assert np.abs(Cd.download() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (transa, s)
+
+
+def test_lobpcg_batch_vs_per_k(al):
+    """diag_LOBPCG! over all 182 k-points in lockstep gives the per-k solver's eigenvalues."""
+    import pwdft_hip as ph
+    from pwdft_hip.solvers import diag_LOBPCG_
+    H, _ = al
+    print("case Al 182k LOBPCG batch", flush=True)
+    ps = ph.rand_BlochWavefunc(H, seed=8)
+    ev, st = ph.diag_LOBPCG_batch(H, ps, tol=1e-10, NiterMax=200)
+    assert ev.shape == (8, 182)
+    ps2 = ph.rand_BlochWavefunc(H, seed=8)
+    for ik in (0, 1, 90, 181):
+        H.ik = ik
+        ev2, _ = diag_LOBPCG_(H, ps2[ik], tol=1e-10, NiterMax=200)
+        assert np.abs(ev[:, ik] - ev2).max() < 1e-8, (ik, ev[:, ik], ev2)
+        X = ps[ik].download()
+        assert np.abs(X.conj().T @ X - np.eye(8)).max() < 1e-10
+
+
+def test_scf_batched_vs_per_k():
+    """Al fcc 4x4x4 shifted (10 k-points, Fermi-Dirac smearing 0.01 Ha): the SCF with the
+    batched eigensolver converges to the per-k SCF's free energy."""
+    import pwdft_hip as ph
+    print("case Al 10k SCF batched vs per-k", flush=True)
+    E = []
+    for batched in (False, True):
+        atoms = ph.Atoms(["Al"], [[0, 0, 0]], ph.gen_lattice_fcc(7.6))
+        H = ph.Hamiltonian(atoms, ["Al-q3.lda.gth"], 15.0, meshk=(4, 4, 4), shiftk=(1, 1, 1), Nstates=8,
+                           use_smearing=True, smearing_kT=0.01)
+        out = ph.KS_solve_SCF_(H, betamix=0.5, use_smearing=True, kT=0.01, etot_conv_thr=1e-9,
+                               ethr_evals_last=1e-10, batched=batched)
+        E.append(out["energies"]["Total"])
+    assert abs(E[0] - E[1]) < 1e-8, E

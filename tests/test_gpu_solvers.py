@@ -101,3 +101,29 @@ def test_scf_si_fcc_symmetry():
     assert np.abs(rho - rho2).max() < 1e-4 * np.abs(rho2).max()
     F = ph.calc_forces(H, out["psiks"], rho)
     assert np.abs(F).max() < 1e-6, F
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_lobpcg_device_dense_vs_host_dense(mixed):
+    """The device (rocSOLVER) and host (LAPACK) solutions of LOBPCG's small dense problems
+    give the same eigenpairs from the same start."""
+    import pwdft_hip as ph
+    from pwdft_hip.solvers import diag_LOBPCG_
+    H = _si(meshk=(2, 2, 2), mixed=mixed)
+    H.update(H.guess_rhoe())
+    H.ik = 1
+    n = H.pw.Ngw[1]
+    a = rand_psi(np.random.default_rng(3), n, 8)
+    res = {}
+    print(f"case LOBPCG dense device vs host mixed={mixed}", flush=True)
+    for dense in ("device", "host"):
+        X = ph.DeviceMatrix.from_host(H.ctx, a)
+        ph.ortho_sqrt_(H, X)
+        ev, st = diag_LOBPCG_(H, X, tol=1e-10, NiterMax=200, dense=dense)
+        res[dense] = (ev, X.download(), st)
+    ev_d, X_d, st_d = res["device"]
+    ev_h, X_h, st_h = res["host"]
+    assert np.abs(ev_d - ev_h).max() < (1e-9 if not mixed else 1e-6), (ev_d, ev_h)
+    # same invariant subspace: |X_d^H X_h| has unit singular values
+    sv = np.linalg.svd(X_d.conj().T @ X_h, compute_uv=False)
+    assert np.abs(sv - 1).max() < (1e-7 if not mixed else 1e-4), sv
