@@ -150,6 +150,52 @@ def time_scf(H, iters, mixed):
             "band_apps_per_s": out["ncols"] / out["seconds"], "Etot": out["energies"]["Total"]}
 
 
+def time_batched_k(steps, warmup):
+    """Al fcc (a = 7.6 bohr, GTH LDA, 15 Ha), 12x12x12 shifted mesh reduced by symmetry to
+    182 k-points, 8 bands each: op_H over all k-points in one launch per stage
+    (pw_op_H_batch_dev) against the per-k loop, FP64 and mixed, plus one batched LOBPCG."""
+    import pwdft_hip as ph
+    atoms = ph.Atoms(["Al"], [[0, 0, 0]], ph.gen_lattice_fcc(7.6))
+    H = ph.Hamiltonian(atoms, ["Al-q3.lda.gth"], 15.0, meshk=(12, 12, 12), shiftk=(1, 1, 1), Nstates=8,
+                       use_smearing=True, smearing_kT=0.01)
+    H.update(H.guess_rhoe())
+    ps = ph.rand_BlochWavefunc(H, seed=20260101)
+    out = [ph.DeviceMatrix(H.ctx, p.rows, p.cols) for p in ps]
+    nk = H.pw.Nkpt
+    res = {"workload": "Al fcc a=7.6, GTH LDA, 15 Ha, 12x12x12 shifted -> %d k-points, 8 bands" % nk,
+           "Ngw_min": int(min(H.pw.Ngw)), "Ngw_max": int(max(H.pw.Ngw))}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        H.mixed = mixed
+        for _ in range(warmup):
+            ph.op_H_batch(H, ps, out=out)
+        H.ctx.call("pw_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ph.op_H_batch(H, ps, out=out)
+        H.ctx.call("pw_synchronize")
+        tb = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps // 4)):
+            for ik in range(nk):
+                H.ik = ik
+                ph.op_H_(H, ps[ik], out[ik])
+        H.ctx.call("pw_synchronize")
+        tl = (time.perf_counter() - t0) / max(1, steps // 4)
+        res[tag] = {"batched_band_apps_per_s": nk * 8 / tb, "batched_ms": tb * 1e3,
+                    "per_k_loop_band_apps_per_s": nk * 8 / tl, "per_k_loop_ms": tl * 1e3}
+        print(f"# Al 182k {tag}: batched op_H {tb * 1e3:.3f} ms, per-k loop {tl * 1e3:.3f} ms",
+              file=sys.stderr, flush=True)
+    H.mixed = False
+    t0 = time.perf_counter()
+    _, st = ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=4)
+    H.ctx.call("pw_synchronize")
+    dt = time.perf_counter() - t0
+    res["lobpcg_batch"] = {"s_per_iter": dt / 4, "band_apps_per_s": sum(x["ncols"] for x in st) / dt}
+    print(f"# Al 182k batched LOBPCG: {dt / 4 * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
+    return res
+
+
 def roofline(prof, H, mixed):
     """Roofline object for the kernel with the largest total time in one op_H step."""
     name = max(prof, key=lambda k: prof[k][0])
@@ -194,6 +240,8 @@ def main():
     ap.add_argument("--solver-iters", type=int, default=4,
                     help="LOBPCG iterations timed per precision (0 = skip)")
     ap.add_argument("--scf-iters", type=int, default=3, help="SCF iterations timed per precision (0 = skip)")
+    ap.add_argument("--no-batched-k", dest="batched_k", action="store_false",
+                    help="skip the 182-k-point batched op_H leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
@@ -244,6 +292,7 @@ def main():
             tag = "mixed" if mixed else "fp64"
             scf[tag] = time_scf(H, args.scf_iters, mixed)
             print(f"# SCF {tag}: {scf[tag]['s_per_iter']:.3f} s/iteration", file=sys.stderr, flush=True)
+    batched = time_batched_k(args.steps, args.warmup) if args.batched_k else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "Hpsi band-applications/s (mixed precision; FP64 beside it)",
@@ -266,6 +315,7 @@ def main():
         "roofline": roofline(res["mixed"]["prof"], H, True),
         "roofline_fp64": roofline(res["fp64"]["prof"], H, False),
         "lobpcg": lob or None,
+        "batched_k": batched,
         "scf_s_per_iter": scf or None,
         "kernels_us_per_step": {t: {k: v[0] * 1e3 / args.steps for k, v in res[t]["prof"].items() if v[1]}
                                 for t in ("fp64", "mixed")},

@@ -219,6 +219,21 @@ int pw_memcpy_d2h(pw_ctx* ctx, void* dst, const void* src, size_t bytes);
 int pw_memcpy_d2d(pw_ctx* ctx, void* dst, const void* src, size_t bytes);
 int pw_synchronize(pw_ctx* ctx);
 
+/* ---- graph capture ----------------------------------------------------------------- */
+
+/* Record the device work of the _dev entry points called between pw_graph_begin and
+ * pw_graph_end into a hipGraph (stream capture of the context stream) and replay it with
+ * pw_graph_launch: one launch for a whole fixed sequence, e.g. the batched op_H of an
+ * eigensolver step.  Run the sequence once before capturing it, so that every workspace and
+ * descriptor array exists: during capture nothing may allocate, upload or synchronise, and
+ * an entry point that would returns PW_ERR_STATE (end the capture with pw_graph_end and
+ * free the result).  Replays read and write the same device addresses as the captured
+ * calls.  No reference counterpart. */
+int pw_graph_begin(pw_ctx* ctx);
+int pw_graph_end(pw_ctx* ctx, int* graph_id);
+int pw_graph_launch(pw_ctx* ctx, int graph_id);
+int pw_graph_free(pw_ctx* ctx, int graph_id);
+
 /* Strided 2-D copy of complex columns between device buffers: dst[:, j] = src[:, j]
  * for j < ncols, nrows complex each. */
 int pw_copy_cols_dev(pw_ctx* ctx, int nrows, int ncols, const void* src, int64_t lds, void* dst,

@@ -6,7 +6,7 @@ import numpy as np
 from scipy.special import erfc
 
 from . import ops
-from .density import calc_rhoe, guess_rhoe, Poisson_solve, calc_epsxc_Vxc_VWN
+from .density import calc_rhoe, guess_rhoe, Poisson_solve, calc_epsxc_Vxc_PBE, calc_epsxc_Vxc_VWN
 from .setup import PWGrid, PsPotNL, calc_V_Ps_loc, gen_kpoint_grid
 from .solvers import diag_LOBPCG, diag_davidson, rand_wavefunc
 
@@ -17,8 +17,9 @@ class Hamiltonian:
 
     def __init__(self, atoms, pspots, ecutwfc, meshk=(1, 1, 1), shiftk=(0, 0, 0),
                  Ns=None, rots=None, kpoints=None, extra_states=None, Nstates=None,
-                 use_smearing=False, kT=0.0):
+                 use_smearing=False, kT=0.0, xcfunc="VWN"):
         self.atoms, self.pspots = atoms, pspots
+        self.xcfunc = xcfunc
         if kpoints is None:
             kpoints = gen_kpoint_grid(atoms.LatVecs, meshk, shiftk, rots=rots)
         self.pw = PWGrid(ecutwfc, atoms.LatVecs, kpoints=kpoints, Ns=Ns)
@@ -53,10 +54,16 @@ class Hamiltonian:
         self.E_NN = calc_E_NN(atoms.LatVecs, atoms, self.Zvals)
 
     def update(self, rho):
-        """Hamiltonian.jl:510-545 (update!, Nspin = 1, VWN)."""
+        """Hamiltonian.jl:510-545 (update!, Nspin = 1, VWN or PBE)."""
         self.V_H = Poisson_solve(self.pw, rho)
-        _, self.V_xc = calc_epsxc_Vxc_VWN(rho)
+        _, self.V_xc = self.epsxc_vxc(rho)
         self.V_loc[:, 0] = self.V_Ps_loc + self.V_H + self.V_xc
+
+    def epsxc_vxc(self, rho):
+        """Slater + VWN, or PBE when xcfunc == "PBE" (Hamiltonian.jl:526-533)."""
+        if self.xcfunc == "PBE":
+            return calc_epsxc_Vxc_PBE(self.pw, rho)
+        return calc_epsxc_Vxc_VWN(rho)
 
 
 # ----------------------------------------------------------------------------- energies
@@ -93,7 +100,7 @@ def calc_E_local(ham, rho):
     dVol = ham.pw.CellVolume / ham.pw.Npoints
     E_H = 0.5 * np.dot(ham.V_H, rho) * dVol
     E_ps = np.dot(ham.V_Ps_loc, rho) * dVol
-    epsxc, _ = calc_epsxc_Vxc_VWN(rho)
+    epsxc, _ = ham.epsxc_vxc(rho)
     E_xc = np.sum(epsxc * rho) * dVol
     return E_ps, E_H, E_xc
 

@@ -85,10 +85,12 @@ int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out) {
       *out = c->desc_dev[i];
       return PW_OK;
     }
+  if (c->capturing)
+    return set_error(c, PW_ERR_STATE, "a new descriptor array during graph capture; run the sequence once first");
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "descriptor sync");
   int slot;
-  if (n < 64) {
+  if (n < 64 || c->graphs_live > 0) {  // never overwrite an array a captured graph may read
     c->desc_host.emplace_back();
     c->desc_dev.push_back(nullptr);
     c->desc_cap.push_back(0);

@@ -24,10 +24,15 @@ int hip_check(pw_ctx* c, hipError_t e, const char* what) {
 
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need) {
   if (need <= *have && *buf) return PW_OK;
+  if (c->capturing)
+    return set_error(c, PW_ERR_STATE, "a workspace would grow during graph capture; run the sequence once first");
   if (*buf) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_check(c, e, "ensure/sync");
-    hipFree(*buf);
+    if (c->graphs_live > 0)
+      c->retired.push_back(*buf);  // a captured graph may still address it
+    else
+      hipFree(*buf);
     *buf = nullptr;
     *have = 0;
   }
@@ -181,7 +186,7 @@ static hipEvent_t pool_event(pw_ctx* c) {
 }
 
 ProfScope::ProfScope(pw_ctx* ctx, int s, double bytes) : c(ctx), slot(s) {
-  if (!c->profile) return;
+  if (!c->profile || c->capturing) return;
   hipEvent_t e0 = pool_event(c);
   e1 = pool_event(c);
   if (!e0 || !e1) {
@@ -258,6 +263,11 @@ int pw_ctx_destroy(pw_ctx* c) {
   free_forces(c);
   free_desc(c);
   free_dense(c);
+  for (void* g : c->graphs)
+    if (g) hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(g));
+  for (void* g : c->graph_defs)
+    if (g) hipGraphDestroy(reinterpret_cast<hipGraph_t>(g));
+  for (void* p : c->retired) hipFree(p);
   drain_events(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
@@ -639,7 +649,63 @@ int pw_memcpy_d2d(pw_ctx* c, void* dst, const void* src, size_t bytes) {
 
 int pw_synchronize(pw_ctx* c) {
   CTX_CHECK(c);
+  if (c->capturing) return set_error(c, PW_ERR_STATE, "pw_synchronize during graph capture");
   return hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
+}
+
+int pw_graph_begin(pw_ctx* c) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (c->capturing) return set_error(c, PW_ERR_STATE, "pw_graph_begin: already capturing");
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_check(c, e, "hipStreamBeginCapture");
+  c->capturing = true;
+  return PW_OK;
+}
+
+int pw_graph_end(pw_ctx* c, int* id) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (!c->capturing) return set_error(c, PW_ERR_STATE, "pw_graph_end: not capturing");
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->capturing = false;
+  if (e != hipSuccess) return hip_check(c, e, "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    return hip_check(c, e, "hipGraphInstantiate");
+  }
+  c->graphs.push_back(reinterpret_cast<void*>(x));
+  c->graph_defs.push_back(reinterpret_cast<void*>(g));
+  c->graphs_live++;
+  if (id) *id = (int)c->graphs.size() - 1;
+  return PW_OK;
+}
+
+int pw_graph_launch(pw_ctx* c, int id) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (id < 0 || id >= (int)c->graphs.size() || !c->graphs[id]) return set_error(c, PW_ERR_ARG, "pw_graph_launch: bad id");
+  return hip_check(c, hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(c->graphs[id]), c->stream), "hipGraphLaunch");
+}
+
+int pw_graph_free(pw_ctx* c, int id) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (id < 0 || id >= (int)c->graphs.size() || !c->graphs[id]) return set_error(c, PW_ERR_ARG, "pw_graph_free: bad id");
+  hipStreamSynchronize(c->stream);
+  hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(c->graphs[id]));
+  hipGraphDestroy(reinterpret_cast<hipGraph_t>(c->graph_defs[id]));
+  c->graphs[id] = nullptr;
+  c->graph_defs[id] = nullptr;
+  if (--c->graphs_live == 0) {
+    for (void* p : c->retired) hipFree(p);
+    c->retired.clear();
+  }
+  return PW_OK;
 }
 
 int pw_copy_cols_dev(pw_ctx* c, int nrows, int ncols, const void* src, int64_t lds, void* dst, int64_t ldd) {

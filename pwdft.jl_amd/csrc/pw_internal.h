@@ -203,6 +203,13 @@ struct pw_ctx {
   std::vector<size_t> desc_cap;
   int desc_next = 0;
   void* d_bnlb = nullptr;  size_t bnlb_bytes = 0;   // beta^H psi of every k-point of a batch
+  // stream capture into hipGraphs (pw_graph_*): while capturing, nothing may allocate,
+  // upload or synchronise
+  bool capturing = false;
+  std::vector<void*> graphs;       // hipGraphExec_t per id (nullptr once freed)
+  std::vector<void*> graph_defs;   // hipGraph_t per id
+  int graphs_live = 0;
+  std::vector<void*> retired;      // workspaces replaced while a graph may still use them
   // dense solvers (pw_dense.hip): rocBLAS handle bound to `stream`, info words, eigenvalues
   void* rb = nullptr;
   void* d_dinfo = nullptr; size_t dinfo_bytes = 0;

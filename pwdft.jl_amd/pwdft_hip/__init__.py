@@ -4,7 +4,7 @@ The compute runs in libpwdft_hip.so (hand-written gfx950 HIP kernels behind a C 
 include/pwdft_hip.h); this package is the host-side mirror of the reference's Julia API
 over that ABI (ctypes), plus the host set-up the reference also does on the host.
 """
-from ._lib import Context, DeviceMatrix, PWError, PosDefException, load
+from ._lib import Context, DeviceMatrix, Graph, PWError, PosDefException, load
 from .basis import (Atoms, PsPot_GTH, PsPotNL, PWGrid, diamond_supercell, gen_lattice_fcc,
                     gen_lattice_sc, kpoint_grid)
 from .hamiltonian import (Hamiltonian, Kprec, calc_betaNL_psi, calc_rhoe, calc_rhoe_, op_H, op_H_, op_H_batch,
@@ -17,7 +17,7 @@ from .forces import (E_Ps_loc_grad, calc_forces, calc_forces_NN, calc_forces_Ps_
                      calc_forces_Ps_nloc, rrule_E_Ps_loc, rrule_E_Ps_nloc)
 
 __all__ = [
-    "Atoms", "Context", "E_Ps_loc_grad", "calc_forces", "calc_forces_NN", "calc_forces_Ps_loc",
+    "Atoms", "Context", "Graph", "E_Ps_loc_grad", "calc_forces", "calc_forces_NN", "calc_forces_Ps_loc",
     "calc_forces_Ps_nloc", "rrule_E_Ps_loc", "rrule_E_Ps_nloc", "DeviceMatrix", "Hamiltonian", "KS_solve_SCF_", "Kprec", "PWError", "PWGrid",
     "PosDefException", "PsPot_GTH", "PsPotNL", "calc_betaNL_psi", "calc_energies", "calc_rhoe",
     "calc_rhoe_", "diag_LOBPCG", "diag_LOBPCG_", "diag_LOBPCG_batch", "diag_davidson", "diag_davidson_", "diamond_supercell",

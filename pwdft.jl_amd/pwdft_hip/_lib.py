@@ -70,6 +70,10 @@ _PROTOS = {
     "pw_memcpy_d2h": [_vp, _vp, _vp, _c.c_size_t],
     "pw_memcpy_d2d": [_vp, _vp, _vp, _c.c_size_t],
     "pw_synchronize": [_vp],
+    "pw_graph_begin": [_vp],
+    "pw_graph_end": [_vp, _c.POINTER(_i)],
+    "pw_graph_launch": [_vp, _i],
+    "pw_graph_free": [_vp, _i],
     "pw_copy_cols_dev": [_vp, _i, _i, _vp, _i64, _vp, _i64],
     "pw_event_record": [_vp, _i],
     "pw_profile_enable": [_vp, _i],
@@ -159,6 +163,38 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Graph:
+    """Stream capture of the device work issued inside a `with` block (pw_graph_begin /
+    pw_graph_end); launch() replays it with one hipGraphLaunch."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.id = None
+
+    def __enter__(self):
+        self.ctx.call("pw_graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        gid = _i(-1)
+        rc = self.ctx.lib.pw_graph_end(self.ctx.h, ctypes.byref(gid))
+        if exc_type is not None:
+            if rc == PW_OK:
+                self.ctx.lib.pw_graph_free(self.ctx.h, gid.value)
+            return False
+        self.ctx.check(rc)
+        self.id = gid.value
+        return False
+
+    def launch(self):
+        self.ctx.call("pw_graph_launch", self.id)
+
+    def free(self):
+        if self.id is not None:
+            self.ctx.call("pw_graph_free", self.id)
+            self.id = None
 
 
 class DeviceMatrix:

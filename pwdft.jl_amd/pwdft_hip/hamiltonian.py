@@ -24,7 +24,7 @@ from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC
                    Context, DeviceMatrix, ptr)
 from .basis import (PsPot_GTH, PsPotNL, PWGrid, RhoeSymmetrizer, V_Ps_loc, crystal_symmetry,
                     eval_Vloc_G, kpoint_grid, kpoint_rotations)
-from .potentials import calc_E_NN, epsxc_vxc_vwn, guess_rhoe, poisson_solve
+from .potentials import calc_E_NN, epsxc_vxc_pbe, epsxc_vxc_vwn, guess_rhoe, poisson_solve
 
 
 class Electrons:
@@ -81,8 +81,8 @@ class Hamiltonian:
         for isp, psp in enumerate(self.pspots):
             if psp.atsymb.lower() != atoms.SpeciesSymbols[isp].lower():
                 raise ValueError("Atomic symbols and pspot atsymb does not match")
-        if xcfunc not in ("VWN", "LDA"):
-            raise NotImplementedError("only the LDA (Slater + VWN) functional is implemented")
+        if xcfunc not in ("VWN", "LDA", "PBE"):
+            raise NotImplementedError("functionals: LDA (Slater + VWN) and PBE")
         self.atoms = atoms
         self.xcfunc = xcfunc
         # symmetry (Hamiltonian.jl:53-83, 261-265): the crystal's operations reduce the
@@ -187,10 +187,16 @@ class Hamiltonian:
         """update!(Ham, psiks, Rhoe) for Nspin = 1 (Hamiltonian.jl:510-545)."""
         rho = np.asarray(rho, dtype=float).reshape(-1)
         self.potentials.Hartree = poisson_solve(self.pw, rho)
-        _, vxc = epsxc_vxc_vwn(rho)
+        _, vxc = self.epsxc_vxc(rho)
         self.potentials.XC[:, 0] = vxc
         self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + vxc
         self._upload_vloc()
+
+    def epsxc_vxc(self, rho):
+        """Slater + VWN, or PBE for xcfunc == "PBE" (Hamiltonian.jl:526-533)."""
+        if self.xcfunc == "PBE":
+            return epsxc_vxc_pbe(self.pw, rho)
+        return epsxc_vxc_vwn(rho)
 
     def guess_rhoe(self):
         return guess_rhoe(self.atoms, self.pw, self.pspots, self.electrons.Nelectrons)

@@ -11,7 +11,6 @@ import numpy as np
 
 from ._lib import DeviceMatrix, PW_TERM_K, ptr
 from .hamiltonian import calc_betaNL_psi, calc_rhoe_
-from .potentials import epsxc_vxc_vwn
 from .batch import diag_LOBPCG_batch
 from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc, _Dense
 
@@ -66,7 +65,7 @@ def calc_E_local(Ham, rho):
     """calc_energies.jl:245-333."""
     pw = Ham.pw
     dV = pw.CellVolume / pw.Npoints
-    epsxc, _ = epsxc_vxc_vwn(rho)
+    epsxc, _ = Ham.epsxc_vxc(rho)
     return (np.dot(Ham.potentials.Ps_loc, rho) * dV, 0.5 * np.dot(Ham.potentials.Hartree, rho) * dV,
             np.sum(epsxc * rho) * dV)
 
@@ -130,13 +129,16 @@ def calc_entropy(wk, kT, evals, Ef):
 
 def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6,
                   ethr_evals_last=1e-5, update_psi="LOBPCG", use_smearing=False, kT=1e-3,
-                  seed=1234, verbose=False, mixed_until=None, batched=False):
+                  seed=1234, verbose=False, mixed_until=None, batched=False, mixed_ethr_floor=1e-6):
     """KS_solve_SCF! (KS_solve_SCF.jl:24-373), mix_method = "simple", Nspin = 1.
 
     `mixed_until`: when Ham.mixed is set, switch the Hamiltonian back to FP64 once the
     energy change falls below this threshold (FP64 refinement of the final iterations).
     `batched`: diagonalise all k-points together (batch.diag_LOBPCG_batch) instead of the
-    reference's k loop; same arithmetic per k-point."""
+    reference's k loop; same arithmetic per k-point.
+    `mixed_ethr_floor`: while the Hamiltonian is in mixed precision the eigensolver
+    tolerance is not taken below this (FP32 products resolve eigenvalue changes to about
+    1e-7 Ha); the FP64 iterations after `mixed_until` use the reference's schedule."""
     pw = Ham.pw
     el = Ham.electrons
     if psiks is None:
@@ -161,14 +163,15 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
             ethr = 0.01
         else:
             ethr = max(ethr / 5.0, ethr_evals_last)
+        tol = max(ethr, mixed_ethr_floor) if Ham.mixed else ethr
         if batched and update_psi == "LOBPCG":
-            ev, sts = diag_LOBPCG_batch(Ham, psiks, tol=ethr, Nstates_conv=el.Nstates_occ)
+            ev, sts = diag_LOBPCG_batch(Ham, psiks, tol=tol, Nstates_conv=el.Nstates_occ)
             evals[:, :] = ev
             ncols += sum(st["ncols"] for st in sts)
         else:
             for ik in range(pw.Nkpt):
                 Ham.ik, Ham.ispin = ik, 0
-                ev, st = solver(Ham, psiks[ik], tol=ethr, Nstates_conv=el.Nstates_occ)
+                ev, st = solver(Ham, psiks[ik], tol=tol, Nstates_conv=el.Nstates_occ)
                 evals[:, ik] = ev
                 ncols += st["ncols"]
         if use_smearing:

@@ -88,6 +88,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     pw_trtri_dev, pw_heevd_dev, pw_hegvd_dev): the Gram and overlap matrices and the Ritz
     coefficients stay in HBM; only eigenvalues reach the host for the convergence test."""
     D = _Dense(Ham, mixed)
+    D64 = _Dense(Ham, False)   # the Gram matrices that are Cholesky-factorised stay FP64
     Nb, Nst = X.rows, X.cols
     c = Ham.ctx
     Q = DeviceMatrix(c, Nb, 3 * Nst)
@@ -105,7 +106,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
 
     def chol_inv_apply(src, hsrc, dst, hdst):
         # U = cholesky(Hermitian(src' src)).U ; dst = src inv(U), hdst = hsrc inv(U)
-        D.gemm(1, Nst, Nst, Nb, src, Nb, src, Nb, S3.p, Nst)
+        D64.gemm(1, Nst, Nst, Nb, src, Nb, src, Nb, S3.p, Nst)
         c.call("pw_potrf_dev", Nst, S3.p, Nst)
         c.call("pw_trtri_dev", Nst, S3.p, Nst)
         c.call("pw_herm_fix_dev", Nst, S3.p, Nst, 1)
@@ -145,7 +146,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
         D.copy(Nb, Nst, t1, hw)
         nq = 3 * Nst if it > 1 else 2 * Nst
         D.gemm(1, nq, nq, Nb, Q.p, Nb, HQ.p, Nb, Sm.p, nq)          # T = Q'HQ
-        D.gemm(1, nq, nq, Nb, Q.p, Nb, Q.p, Nb, Gm.p, nq)           # G = Q'Q
+        D64.gemm(1, nq, nq, Nb, Q.p, Nb, Q.p, Nb, Gm.p, nq)         # G = Q'Q (FP64: factorised)
         c.call("pw_hegvd_dev", nq, Sm.p, nq, Gm.p, nq, evp)         # Ritz vectors in Sm
         D.gemm(0, Nb, Nst, nq, Q.p, Nb, Sm.p, nq, t0, Nb)
         D.gemm(0, Nb, Nst, nq, HQ.p, Nb, Sm.p, nq, t1, Nb)
@@ -187,6 +188,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     if (DENSE if dense is None else dense) == "device":
         return _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed)
     D = _Dense(Ham, mixed)
+    D64 = _Dense(Ham, False)
     Nb, Nst = X.rows, X.cols
     if Nstates_conv == 0:
         Nstates_conv = Nst
@@ -223,7 +225,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
             continue
         D.opH(w, hw, Nst, Nb)
         ncols += Nst
-        U, Ui = _chol_upper_inv(D.gram(w, w, Nst, Nst, Nb))
+        U, Ui = _chol_upper_inv(D64.gram(w, w, Nst, Nst, Nb))
         dUi = D.upload_small(Ui)
         D.gemm(0, Nb, Nst, Nst, w, Nb, dUi.p, Nst, Tmp.col(0), Nb)
         D.gemm(0, Nb, Nst, Nst, hw, Nb, dUi.p, Nst, Tmp.col(Nst), Nb)
@@ -231,7 +233,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
         D.copy(Nb, Nst, Tmp.col(Nst), hw)
         nq = 3 * Nst if it > 1 else 2 * Nst
         T = herm_u(D.gram(Q.p, HQ.p, nq, nq, Nb))
-        G = herm_u(D.gram(Q.p, Q.p, nq, nq, Nb))
+        G = herm_u(D64.gram(Q.p, Q.p, nq, nq, Nb))
         try:
             _, S = sla.eigh(T, G)
         except np.linalg.LinAlgError as e:
@@ -246,7 +248,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
             D.gemm(0, Nb, Nst, 2 * Nst, hw, Nb, dU23.p, 2 * Nst, Tmp.col(3 * Nst), Nb)
             D.copy(Nb, Nst, Tmp.col(0), x)
             D.copy(Nb, Nst, Tmp.col(Nst), hx)
-            U, Ui = _chol_upper_inv(D.gram(Tmp.col(2 * Nst), Tmp.col(2 * Nst), Nst, Nst, Nb))
+            U, Ui = _chol_upper_inv(D64.gram(Tmp.col(2 * Nst), Tmp.col(2 * Nst), Nst, Nst, Nb))
             dUi = D.upload_small(Ui)
             D.gemm(0, Nb, Nst, Nst, Tmp.col(2 * Nst), Nb, dUi.p, Nst, p, Nb)
             D.gemm(0, Nb, Nst, Nst, Tmp.col(3 * Nst), Nb, dUi.p, Nst, hp, Nb)

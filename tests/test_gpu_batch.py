@@ -111,10 +111,10 @@ def test_zgemm_batch_vs_numpy():
         i32 = lambda xs: np.ascontiguousarray(xs, dtype=np.int32)
         i64 = lambda xs: np.ascontiguousarray(xs, dtype=np.int64)
         ms, ns, ks = (i32([s[i] for s in shapes]) for i in range(3))
+        la, lb, lc = i64([a.rows for a in As]), i64([b.rows for b in Bs]), i64([c.rows for c in Cs])
         from pwdft_hip._lib import ptr
         ctx.call("pw_zgemm_batch_dev", transa, len(shapes), ptr(ms), ptr(ns), ptr(ks), ptr(alpha), arr(As),
-                 ptr(i64([a.rows for a in As])), arr(Bs), ptr(i64([b.rows for b in Bs])), ptr(beta), arr(Cs),
-                 ptr(i64([c.rows for c in Cs])))
+                 ptr(la), arr(Bs), ptr(lb), ptr(beta), arr(Cs), ptr(lc))
         for Cd, ref, s in zip(Cs, refs, shapes):
             assert np.abs(Cd.download() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (transa, s)
 
@@ -151,3 +151,35 @@ def test_scf_batched_vs_per_k():
                                ethr_evals_last=1e-10, batched=batched)
         E.append(out["energies"]["Total"])
     assert abs(E[0] - E[1]) < 1e-8, E
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_op_H_batch_graph_replay(al, mixed):
+    """The batched op_H captured into a hipGraph and replayed: every replay, with eager
+    launches and synchronisations in between, reproduces the eager result bit for bit."""
+    import pwdft_hip as ph
+    H, host = al
+    print(f"case Al 182k op_H batch graph mixed={mixed}", flush=True)
+    dev = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
+    H.mixed = mixed
+    try:
+        eager = [m.download() for m in ph.op_H_batch(H, dev)]
+        out = [ph.DeviceMatrix(H.ctx, a.shape[0], a.shape[1]) for a in host]
+        ph.op_H_batch(H, dev, out=out)        # warm: workspaces and descriptors exist
+        H.ctx.call("pw_synchronize")
+        for o in out:                         # poison the outputs before the replays
+            o.upload(np.full(o.rows * o.cols, np.nan + 0j))
+        with ph.Graph(H.ctx) as g:
+            ph.op_H_batch(H, dev, out=out)
+        for rep in range(3):
+            g.launch()
+            H.ctx.call("pw_synchronize")
+            for ik in (0, 91, 181):
+                assert np.array_equal(out[ik].download(), eager[ik]), (rep, ik)
+            ph.op_H_batch(H, dev[:5], iks=range(5))   # eager work between replays
+            H.ctx.call("pw_synchronize")
+            for o in out:
+                o.upload(np.zeros(o.rows * o.cols, dtype=complex))
+        g.free()
+    finally:
+        H.mixed = False

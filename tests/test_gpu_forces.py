@@ -41,12 +41,6 @@ def _orthonormal_psiks(H, seed):
     return out
 
 
-def _moved(H, ia, d, h):
-    pos = H.atoms.positions.copy()
-    pos[d, ia] += h
-    return pos
-
-
 @pytest.fixture(scope="module")
 def si8():
     H, frac, L = _si8()
@@ -133,7 +127,9 @@ def test_energy_gradients_fd():
     for ia, d in ((0, 0), (3, 1), (6, 2)):
         el, en = [], []
         for s in (h, -h):
-            H.set_positions(_moved(H, ia, d, s))
+            pos = pos0.copy()
+            pos[d, ia] += s
+            H.set_positions(pos)
             el.append(ph.E_Ps_loc_grad(H, rho)[0])
             en.append(ph.rrule_E_Ps_nloc(H, psiks, Focc)[0])
             H.set_positions(pos0)
@@ -172,7 +168,9 @@ def test_scf_forces_fd():
     for ia, d in ((0, 0), (0, 1), (1, 2)):
         E = []
         for s in (h, -h):
-            H.set_positions(_moved(H, ia, d, s))
+            pos = pos0.copy()
+            pos[d, ia] += s
+            H.set_positions(pos)
             E.append(_scf(H)["energies"]["Total"])
         H.set_positions(pos0)
         fd = -(E[0] - E[1]) / (2 * h)

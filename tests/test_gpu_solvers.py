@@ -127,3 +127,15 @@ def test_lobpcg_device_dense_vs_host_dense(mixed):
     # same invariant subspace: |X_d^H X_h| has unit singular values
     sv = np.linalg.svd(X_d.conj().T @ X_h, compute_uv=False)
     assert np.abs(sv - 1).max() < (1e-7 if not mixed else 1e-4), sv
+
+
+def test_scf_si_fcc_pbe_matches_reference_log():
+    """GTH-PBE Si fcc, 15 Ha, 3x3x3 (symmetry-reduced to 4 k-points): the GPU SCF reaches the
+    reference's recorded PBE total energy (test/LOG_runtest_v.1.1.1:2012)."""
+    import pwdft_hip as ph
+    sym, frac = si_fcc_structure()
+    atoms = ph.Atoms(sym, frac, ph.gen_lattice_fcc(A_SI))
+    H = ph.Hamiltonian(atoms, ["Si-q4.pbe.gth"], 15.0, meshk=(3, 3, 3), extra_states=4, xcfunc="PBE")
+    print("case scf si fcc pbe", flush=True)
+    out = ph.KS_solve_SCF_(H, betamix=0.5, etot_conv_thr=1e-7, verbose=True)
+    assert abs(out["energies"]["Total"] - (-7.8539433194)) < 1e-6, out["energies"]

@@ -226,3 +226,42 @@ def test_scf_si_fcc_matches_reference_log():
     assert abs(e["Ps_nloc"] - 1.5811898971) < 1e-6
     assert abs(e["Ps_loc"] - (-2.1761282488 - 0.2946256268)) < 1e-6
     assert abs(e["Total"] - (-7.9110087934)) < 1e-5   # ABINIT, test/validations_old/Si_fcc.jl:15
+
+
+@pytest.mark.slow
+def test_scf_si_fcc_pbe_matches_reference_log():
+    """test/LOG_runtest_v.1.1.1:1990-2012: Si fcc GTH-PBE, 15 Ha, 3x3x3, LOBPCG, simple mixing
+    beta = 0.5: Etot -7.8539433194 (converged to |dE| 4.1e-7); components Kinetic 3.1938247323,
+    Ps_nloc 1.5608465719, Hartree 0.5765247087, XC -2.4280048103, Ps_loc + PspCore
+    -2.3592071214.  The internal PBE restatement (GGA_PBE_internal.jl) reproduces them."""
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.pbe.gth", 15.0, meshk=(3, 3, 3), extra_states=4,
+                   xcfunc="PBE")
+    e, evals, psiks, rho, hist = KS_solve_SCF(H, betamix=0.5)
+    assert abs(e["Total"] - (-7.8539433194)) < 1e-6
+    for k, v in (("Kinetic", 3.1938247323), ("Ps_nloc", 1.5608465719), ("Hartree", 0.5765247087),
+                 ("XC", -2.4280048103), ("Ps_loc", -2.1493161546 - 0.2098909668)):
+        assert abs(e[k] - v) < 3e-6, (k, e[k], v)
+
+
+def test_pbe_potential_is_functional_derivative():
+    """V_xc^PBE = dE_xc/drho on the grid: central differences of E_xc along a smooth
+    band-limited direction phi equal sum V phi dV (pins the gradient and divergence terms)."""
+    from oracle.density import calc_epsxc_Vxc_PBE, guess_rhoe
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.pbe.gth", 8.0, xcfunc="PBE")
+    pw = H.pw
+    rho = guess_rhoe(H) + 2e-3
+    dV = pw.CellVolume / pw.Npoints
+    rng = np.random.default_rng(5)
+    c = np.zeros(pw.Npoints, dtype=complex)
+    sel = pw.idx_g2r[1:40]
+    c[sel] = rng.standard_normal(sel.size) + 1j * rng.standard_normal(sel.size)
+    phi = np.real(S.G_to_R(pw, c))
+    phi *= 1e-3 / np.abs(phi).max()
+    Exc = lambda r: float(np.sum(calc_epsxc_Vxc_PBE(pw, r)[0] * r) * dV)
+    _, V = calc_epsxc_Vxc_PBE(pw, rho)
+    h = 1e-2
+    fd = (Exc(rho + h * phi) - Exc(rho - h * phi)) / (2 * h)
+    an = float(np.sum(V * phi) * dV)
+    assert abs(fd - an) < 1e-7 * max(1.0, abs(an)), (fd, an)
