@@ -66,7 +66,9 @@ enum {
   PW_TUNE_GEMM_NN_SHAPE = 1,  /* A B products: same codes */
   PW_TUNE_GEMM_CN_WG = 2,     /* A^H B split-K target workgroup count (0 = default 1024) */
   PW_TUNE_GEMM_NN_WG = 3,     /* A B split-K target workgroup count */
-  PW_TUNE_BAND_BATCH = 4      /* bands per fused FFT launch (same as pw_set_band_batch) */
+  PW_TUNE_BAND_BATCH = 4,     /* bands per fused FFT launch (same as pw_set_band_batch) */
+  PW_TUNE_PLANE_COMPRESSED = 5  /* 1: always use the compressed z-plane kernels (they are used
+                                   anyway when the padded plane does not fit in LDS) */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

@@ -76,6 +76,7 @@ void free_k(KData& k) {
   hipFree(k.d_ent_pos);
   hipFree(k.d_col_ptr);
   hipFree(k.d_col_ij);
+  hipFree(k.d_col_cy);
   hipFree(k.d_xact);
   hipFree(k.d_gk2);
   hipFree(k.d_gk);
@@ -299,6 +300,9 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       return PW_OK;
     case PW_TUNE_BAND_BATCH:
       return pw_set_band_batch(c, value);
+    case PW_TUNE_PLANE_COMPRESSED:
+      c->plane_compressed = value != 0;
+      return PW_OK;
     default:
       return set_error(c, PW_ERR_ARG, "pw_tune_set: unknown key");
   }
@@ -394,6 +398,17 @@ int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, cons
   for (int cc = 0; cc < ncol; ++cc) col_ptr[cc + 1] += col_ptr[cc];
   std::sort(xs.begin(), xs.end());
   xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+  std::vector<int> col_cy(ncol);
+  {
+    std::vector<int> apos(n1, -1);
+    for (int a = 0; a < (int)xs.size(); ++a) apos[xs[a]] = a;
+    const int ys = odd_stride(n2);
+    int cc = 0;
+    for (auto& kv : colmap) {
+      const int x = kv.first % n1, y = kv.first / n1;
+      col_cy[cc++] = apos[x] * ys + y;
+    }
+  }
   std::vector<double> gk2(ngw);
   for (int g = 0; g < ngw; ++g) {
     const double x = gk[3 * g], y = gk[3 * g + 1], z = gk[3 * g + 2];
@@ -408,6 +423,7 @@ int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, cons
   if ((rc = upload(c, &k.d_ent_pos, e_pos.data(), ngw))) return rc;
   if ((rc = upload(c, &k.d_col_ptr, col_ptr.data(), ncol + 1))) return rc;
   if ((rc = upload(c, &k.d_col_ij, col_ij.data(), ncol))) return rc;
+  if ((rc = upload(c, &k.d_col_cy, col_cy.data(), ncol))) return rc;
   if ((rc = upload(c, &k.d_xact, xs.data(), xs.size()))) return rc;
   if ((rc = upload(c, &k.d_gk2, gk2.data(), ngw))) return rc;
   if ((rc = upload(c, &k.d_gk, gk, 3 * (size_t)ngw))) return rc;

@@ -442,6 +442,126 @@ k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __rest
   plane_vloc_body<T>(W + ((long)bl * n3 + z) * ncol, V, z, col_ij, ncol, xact, nxact, dx, dy, twx, twy);
 }
 
+// 2c. The plane pass for planes whose padded form does not fit in LDS (FP64 at 108^2):
+// only the active x lines are kept, compressed (Yc: nxact lines of n2, stride ys; column c
+// sits at slot col_cy[c] = a*ys + y for x = xact[a]).  The x-transform of each row runs in a
+// per-wavefront row buffer of n1 entries: the row is expanded from Yc (zero at inactive x),
+// transformed (inverse, * V/N, forward) and its active entries gathered back.  rl rows are
+// in flight per wavefront (rl <= gpw, sized to the LDS left after Yc).
+template <typename T>
+__device__ inline void plane_rows_c(cplx<T>* __restrict__ Yc, int ys, cplx<T>* __restrict__ rows,
+                                    const int* __restrict__ xl, int nxact, const FftDim& dx, int n2,
+                                    const cplx<T>* __restrict__ tx, const T* __restrict__ v, int rl,
+                                    double* __restrict__ racc, double wb) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
+  const int n1 = dx.n;
+  const int g = lane / dx.L, q = lane - g * dx.L;
+  for (int t0 = wave * rl; t0 < n2; t0 += nwave * rl) {
+    const int row = t0 + g;
+    const bool active = g < rl && row < n2;
+    const int base = active ? (wave * rl + g) * n1 : 0;
+    if (active)
+      for (int i = q; i < n1; i += dx.L) rows[base + i] = mk((T)0, (T)0);
+    wave_fence();
+    if (active)
+      for (int a = q; a < nxact; a += dx.L) rows[base + xl[a]] = Yc[a * ys + row];
+    wave_fence();
+    wave_line_fft<T, +1, FFT_ME>(rows, base, 1, dx, tx, q, active, nullptr);
+    if (racc) {
+      // density: add w |psi(r)|^2 of this row into the block's global plane
+      if (active)
+        for (int i = q; i < n1; i += dx.L) {
+          const double re = (double)rows[base + i].x, im = (double)rows[base + i].y;
+          racc[(long)row * n1 + i] += wb * (re * re + im * im);
+        }
+      wave_fence();
+      continue;
+    }
+    wave_line_fft<T, -1, FFT_ME>(rows, base, 1, dx, tx, q, active, active ? v + (long)row * n1 : nullptr);
+    if (active)
+      for (int a = q; a < nxact; a += dx.L) Yc[a * ys + row] = rows[base + xl[a]];
+    wave_fence();
+  }
+}
+
+template <typename T>
+__device__ inline void plane_vloc_c_body(cplx<T>* __restrict__ w, const T* __restrict__ V, int z,
+                                         const int* __restrict__ col_cy, int ncol, const int* __restrict__ xact,
+                                         int nxact, FftDim dx, FftDim dy, const cplx<T>* __restrict__ twx,
+                                         const cplx<T>* __restrict__ twy, int rl) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n1 = dx.n, n2 = dy.n, ys = odd_stride(n2);
+  cplx<T>* Yc = lds;
+  cplx<T>* tx = Yc + nxact * ys;
+  cplx<T>* ty = tx + n1;
+  cplx<T>* rows = ty + n2;
+  int* xl = reinterpret_cast<int*>(rows + (blockDim.x >> 6) * rl * n1);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int i = tid; i < nxact * ys; i += nthr) Yc[i] = mk((T)0, (T)0);
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
+  __syncthreads();
+  for (int c = tid; c < ncol; c += nthr) Yc[col_cy[c]] = w[c];
+  __syncthreads();
+  wave_lines<T, +1, FFT_ME>(Yc, nxact, nullptr, ys, 1, dy, ty);
+  __syncthreads();
+  plane_rows_c<T>(Yc, ys, rows, xl, nxact, dx, n2, tx, V + (long)z * n1 * n2, rl, nullptr, 0.0);
+  __syncthreads();
+  wave_lines<T, -1, FFT_ME>(Yc, nxact, nullptr, ys, 1, dy, ty);
+  __syncthreads();
+  for (int c = tid; c < ncol; c += nthr) w[c] = Yc[col_cy[c]];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_plane_vloc_c(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __restrict__ col_cy, int ncol,
+               const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy, const cplx<T>* __restrict__ twx,
+               const cplx<T>* __restrict__ twy, int n3, int rl) {
+  const int z = blockIdx.x, bl = blockIdx.y;
+  plane_vloc_c_body<T>(W + ((long)bl * n3 + z) * ncol, V, z, col_cy, ncol, xact, nxact, dx, dy, twx, twy, rl);
+}
+
+// Density planes in the compressed form: sum_b w_b |psi_b(r)|^2 accumulated straight into
+// the block's partial plane in global memory (each element is owned by one lane per band,
+// and the bands of a group run one after another in the block).
+template <typename T>
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_plane_rhoe_c(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int nb, int bpg,
+               double* __restrict__ rpart, long N, const int* __restrict__ col_cy, int ncol,
+               const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy, const cplx<T>* __restrict__ twx,
+               const cplx<T>* __restrict__ twy, int n3, int rl) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ys = odd_stride(n2);
+  cplx<T>* Yc = lds;
+  cplx<T>* tx = Yc + nxact * ys;
+  cplx<T>* ty = tx + n1;
+  cplx<T>* rows = ty + n2;
+  int* xl = reinterpret_cast<int*>(rows + (blockDim.x >> 6) * rl * n1);
+  const int z = blockIdx.x, g = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  double* out = rpart + (long)g * N + (long)z * np;
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
+  for (int i = tid; i < np; i += nthr) out[i] = 0.0;
+  const int bend = min(nb, (g + 1) * bpg);
+  for (int b = g * bpg; b < bend; ++b) {
+    for (int i = tid; i < nxact * ys; i += nthr) Yc[i] = mk((T)0, (T)0);
+    __syncthreads();
+    const cplx<T>* w = W + ((long)b * n3 + z) * ncol;
+    for (int c = tid; c < ncol; c += nthr) Yc[col_cy[c]] = w[c];
+    __syncthreads();
+    wave_lines<T, +1, FFT_ME>(Yc, nxact, nullptr, ys, 1, dy, ty);
+    __syncthreads();
+    plane_rows_c<T>(Yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, out, wts[b]);
+    __syncthreads();
+  }
+}
+
 // 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi (one band:
 // w its block of W, p / h its psi / Hpsi columns).
 template <typename T>
@@ -635,14 +755,31 @@ static size_t plane_lds(const pw_ctx* c, size_t esz, int nxact) {
 }
 static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
+// Compressed plane (k_plane_vloc_c / k_plane_rhoe_c): rows in flight per wavefront that fit
+// in LDS next to the compressed plane, and the LDS bytes; 0 rows when nothing fits.
+static int plane_c_rows(const pw_ctx* c, size_t esz, int nxact, int nthreads, size_t* bytes) {
+  const size_t fixed = ((size_t)nxact * odd_stride(c->n2) + c->n1 + c->n2) * esz + (size_t)nxact * sizeof(int);
+  const int nwave = nthreads / 64;
+  const size_t per = (size_t)nwave * c->n1 * esz;
+  const size_t lim = 160 * 1024;
+  int rl = fixed < lim ? (int)((lim - fixed) / per) : 0;
+  rl = std::min(rl, c->dim[0].gpw);
+  *bytes = fixed + (size_t)std::max(rl, 0) * per;
+  return rl;
+}
+
 template <typename T>
 static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>* psi, long ldpsi,
                     cplx<double>* hpsi, long ldh, int kin, int acc) {
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
-  const size_t plane_bytes = plane_lds(c, esz, k.nxact);
-  if (plane_bytes > 160 * 1024)
-    return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
+  size_t plane_bytes = plane_lds(c, esz, k.nxact);
+  int rl = 0;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  if (compressed) {
+    rl = plane_c_rows(c, esz, k.nxact, PlaneNT<T>::value, &plane_bytes);
+    if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
+  }
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
   const int bb = std::max(1, std::min(nb, c->band_batch));
@@ -667,8 +804,12 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
     {
       dim3 g2(c->n3, nbb);
       ProfScope ps(c, PW_PROF_PLANE_VLOC, 2.0 * wb + (double)c->N * sizeof(T));
-      hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V, k.d_col_ij,
-                         k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+      if (compressed)
+        hipLaunchKernelGGL(k_plane_vloc_c<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
+                           k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
+      else
+        hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
+                           k.d_col_ij, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     }
     {
       ProfScope ps(c, PW_PROF_COL_FFT, wb + psib * ((kin ? 1 : 0) + (acc ? 1 : 0) + 1) + k.ngw * 16.0);
@@ -785,9 +926,13 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
   const int np = c->n1 * c->n2;
-  const size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
-  if (plane_bytes > 160 * 1024)
-    return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
+  size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
+  int rl = 0;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  if (compressed) {
+    rl = plane_c_rows(c, esz, k.nxact, PlaneNT<T>::value, &plane_bytes);
+    if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
+  }
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
   const int bb = std::max(1, std::min(nb, c->band_batch));
@@ -809,9 +954,14 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
                        c->stream, psi, ldpsi, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb,
                        c->dim[2], tw[2]);
     ProfScope ps(c, PW_PROF_PLANE_RHOE, (double)nbb * c->n3 * k.ncol * esz + (double)ngu * c->N * 8.0);
-    hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
-                       d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,
-                       k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+    if (compressed)
+      hipLaunchKernelGGL(k_plane_rhoe_c<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
+                         d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_cy, k.ncol,
+                         k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
+    else
+      hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
+                         d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,
+                         k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     hipLaunchKernelGGL(k_rho_reduce, dim3(1024), dim3(256), 0, c->stream, c->d_rho,
                        reinterpret_cast<const double*>(c->d_rpart), ngu, c->N, 1.0 / c->omega);
   }

@@ -90,6 +90,7 @@ struct KData {
   int* d_ent_pos = nullptr;   // [ngw]   column * cstride + z (LDS slot in the column tile)
   int* d_col_ptr = nullptr;   // [ncol+1]
   int* d_col_ij = nullptr;    // [ncol]  i + pstride * j (LDS slot inside a padded z-plane)
+  int* d_col_cy = nullptr;    // [ncol]  a * ystride + j, x = xact[a] (slot in a compressed plane)
   int* d_xact = nullptr;      // [nxact] distinct x of the columns
   double* d_gk2 = nullptr;    // [ngw]   |G+k|^2
   double* d_gk = nullptr;     // [3*ngw] G+k (x,y,z per entry, column-major 3 x ngw)
@@ -167,6 +168,7 @@ struct pw_ctx {
   void* d_tmp = nullptr;   size_t tmp_bytes = 0;      // staging for host-pointer entry points
   void* d_wts = nullptr;   size_t wts_bytes = 0;      // band weights of calc_rhoe
   int band_batch = 64;
+  bool plane_compressed = false;  // force the compressed plane kernels (tuning / tests)
   // tuning (pw_tune_set): [0] = A^H B products, [1] = A B products
   int tune_gemm_shape[2] = {0, 0};  // 0: 2x2 wavefronts (64x64), 1: 4x2, 2: 2x4, 3: 4x4
   int tune_gemm_wg[2] = {0, 0};     // split-K target workgroups (0 = 1024)

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from ._lib import DeviceMatrix, PosDefException, PW_TERM_ALL, ptr
-from .solvers import EPS, _Dense, _lobpcg_lock_step
+from .solvers import EPS, MIXED_TOL_FLOOR, _Dense, _lobpcg_lock_step
 
 _ONE = np.array([1.0, 0.0])
 _ZERO = np.array([0.0, 0.0])
@@ -178,6 +178,8 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     psiks: list of DeviceMatrix (Ngw[ik] x Nst), overwritten with the Ritz vectors.
     Returns (evals Nst x nk, stats per k-point {"iter", "ncols"})."""
     mixed = Ham.mixed if mixed is None else mixed
+    if mixed:
+        tol = max(tol, MIXED_TOL_FLOOR)
     iks = list(range(Ham.pw.Nkpt)) if iks is None else list(iks)
     nk = len(iks)
     Nst = psiks[0].cols

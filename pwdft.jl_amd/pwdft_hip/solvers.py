@@ -145,8 +145,12 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
         D.copy(Nb, Nst, t0, w)
         D.copy(Nb, Nst, t1, hw)
         nq = 3 * Nst if it > 1 else 2 * Nst
-        D.gemm(1, nq, nq, Nb, Q.p, Nb, HQ.p, Nb, Sm.p, nq)          # T = Q'HQ
-        D64.gemm(1, nq, nq, Nb, Q.p, Nb, Q.p, Nb, Gm.p, nq)         # G = Q'Q (FP64: factorised)
+        # T = Q'HQ and G = Q'Q: only the upper block triangle, the part Hermitian() reads
+        # (block column j needs rows of blocks 0..j); G in FP64 since it is factorised
+        for j in range(nq // Nst):
+            m = (j + 1) * Nst
+            D.gemm(1, m, Nst, Nb, Q.p, Nb, HQ.col(j * Nst), Nb, ctypes.c_void_p(Sm.p.value + 16 * nq * j * Nst), nq)
+            D64.gemm(1, m, Nst, Nb, Q.p, Nb, Q.col(j * Nst), Nb, ctypes.c_void_p(Gm.p.value + 16 * nq * j * Nst), nq)
         c.call("pw_hegvd_dev", nq, Sm.p, nq, Gm.p, nq, evp)         # Ritz vectors in Sm
         D.gemm(0, Nb, Nst, nq, Q.p, Nb, Sm.p, nq, t0, Nb)
         D.gemm(0, Nb, Nst, nq, HQ.p, Nb, Sm.p, nq, t1, Nb)
@@ -174,6 +178,10 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
 
 
 DENSE = "device"   # where diag_LOBPCG_ solves its small dense problems: "device" or "host"
+# In mixed precision the FP32 products resolve eigenvalue changes to about 1e-7 Ha; below
+# that the |dlambda| test cannot be met and the block vectors turn to noise, so the
+# tolerance is not taken lower (the FP64 iterations that follow reach the final accuracy).
+MIXED_TOL_FLOOR = 1e-6
 
 
 def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, dense=None):
@@ -185,6 +193,8 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     mixed = Ham.mixed if mixed is None else mixed
     if Nstates_conv == 0:
         Nstates_conv = X.cols
+    if mixed:
+        tol = max(tol, MIXED_TOL_FLOOR)
     if (DENSE if dense is None else dense) == "device":
         return _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed)
     D = _Dense(Ham, mixed)

@@ -15,7 +15,7 @@ SHIFT = np.array([[0.041, -0.027, 0.033, -0.012, 0.018, -0.045, 0.006, 0.029],
                   [0.025, 0.047, -0.038, 0.011, -0.023, 0.035, -0.044, -0.007]])
 
 
-def _si8(ecut=10.0, meshk=(2, 2, 2), **kw):
+def _si8(ecut=10.0, meshk=(2, 2, 2), Nstates=16, **kw):
     """Si8 cubic cell (a = 10.2631 bohr), every atom moved by a fixed offset <= 0.05 bohr."""
     import pwdft_hip as ph
     L = ph.gen_lattice_sc(A_SI)
@@ -23,7 +23,7 @@ def _si8(ecut=10.0, meshk=(2, 2, 2), **kw):
     cart = L @ frac.T + SHIFT
     frac = np.linalg.solve(L, cart).T
     atoms = ph.Atoms(["Si"] * 8, frac, L)
-    H = ph.Hamiltonian(atoms, ["Si-q4.lda.gth"], ecut, meshk=meshk, Nstates=16, use_symmetry=False, **kw)
+    H = ph.Hamiltonian(atoms, ["Si-q4.lda.gth"], ecut, meshk=meshk, Nstates=Nstates, use_symmetry=False, **kw)
     return H, frac, L
 
 
@@ -179,14 +179,16 @@ def test_scf_forces_fd():
 
 
 def test_scf_forces_mixed_vs_fp64():
-    """Mixed precision SCF (FP32 FFT/V and projectors, FP64 refinement at the end) gives
-    total energy within 1e-6 Ha/atom and forces within 1e-4 Ha/bohr of the FP64 SCF."""
+    """Si8 (displaced, 2x2x2 k-points, 20 bands): the mixed precision SCF (FP32 FFT/V and
+    projectors, FP64 refinement at the end) gives total energy within 1e-6 Ha/atom and
+    forces within 1e-4 Ha/bohr of the FP64 SCF."""
     import pwdft_hip as ph
     print("case Si8 SCF mixed vs FP64 forces", flush=True)
     res = {}
     for mixed in (False, True):
-        H, _, _ = _si8(meshk=(1, 1, 1), mixed=mixed)
-        out = _scf(H, mixed_until=1e-6 if mixed else None)
+        H, _, _ = _si8(meshk=(2, 2, 2), Nstates=20, mixed=mixed)
+        out = ph.KS_solve_SCF_(H, betamix=0.3, etot_conv_thr=1e-9, ethr_evals_last=1e-10, NiterMax=80,
+                               mixed_until=1e-6 if mixed else None)
         rho = ph.calc_rhoe(H, out["psiks"])
         res[mixed] = (out["energies"]["Total"], ph.calc_forces(H, out["psiks"], rho))
     dE = abs(res[True][0] - res[False][0]) / 8

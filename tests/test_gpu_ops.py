@@ -140,9 +140,12 @@ def test_betaNL_psi(si_pair):
     assert rel(got, O.calc_betaNL_psi(oH.pspotNL.betaNL[3], psi)) < TOL64
 
 
-@pytest.mark.parametrize("case", ["al_fcc_20", "si8_45", "si8_72"])
+@pytest.mark.parametrize("case", ["al_fcc_20", "si8_45", "si8_72", "si8_72_compressed", "si8_108"])
 def test_other_grids(case):
-    """Grids with other radix mixes: 20 = 5*4, 45 = 9*5, 72 = 9*8 (Ns_ override)."""
+    """Grids with other radix mixes: 20 = 5*4, 45 = 9*5, 72 = 9*8, 108 = 9*4*3 (Ns_ override).
+    At 108 the padded FP64 z-plane (188 KB) exceeds LDS and the compressed plane kernels
+    run; "si8_72_compressed" forces them where the padded plane would fit.  calc_rhoe is
+    checked on the same grids."""
     import pwdft_hip as ph
     from oracle import ops as O
     from oracle import setup as S
@@ -155,13 +158,17 @@ def test_other_grids(case):
         Lo, Lp = S.gen_lattice_sc(A_SI), ph.gen_lattice_sc(A_SI)
         psp, ecut = "Si-q4.lda.gth", 20.0
         kw = dict(meshk=(1, 1, 1), extra_states=4)
-        if case == "si8_72":
+        if case.startswith("si8_72"):
             kw["Ns"] = (72, 72, 72)
+        if case == "si8_108":
+            kw["Ns"] = (108, 108, 108)
     kwp = dict(kw)
     if "Ns" in kwp:
         kwp["Ns_"] = kwp.pop("Ns")
     oH = oracle_ham(sym, frac, Lo, psp, ecut, **kw)
     H = product_ham(sym, frac, Lp, psp, ecut, **kwp)
+    if case == "si8_72_compressed":
+        H.ctx.call("pw_tune_set", 5, 1)
     rng = np.random.default_rng(4)
     V = oH.V_Ps_loc + 0.2 * rng.standard_normal(oH.pw.Npoints)
     oH.V_loc[:, 0] = V
@@ -174,3 +181,11 @@ def test_other_grids(case):
         H.mixed = True
         assert rel(ph.op_H(H, psi), O.op_H(oH, psi)) < TOLMIX
         H.mixed = False
+    from oracle.density import calc_rhoe as o_rhoe
+    psiks = [rand_psi(rng, n, 6) for n in H.pw.Ngw]
+    Focc = np.full((6, H.pw.Nkpt), 2.0)
+    ref = o_rhoe(oH, psiks, Focc)
+    assert rel(ph.calc_rhoe(H, psiks, Focc=Focc)[:, 0], ref) < TOL64
+    H.mixed = True
+    assert rel(ph.calc_rhoe(H, psiks, Focc=Focc)[:, 0], ref) < TOLMIX
+    H.mixed = False

@@ -123,10 +123,11 @@ def test_lobpcg_device_dense_vs_host_dense(mixed):
         res[dense] = (ev, X.download(), st)
     ev_d, X_d, st_d = res["device"]
     ev_h, X_h, st_h = res["host"]
-    assert np.abs(ev_d - ev_h).max() < (1e-9 if not mixed else 1e-6), (ev_d, ev_h)
+    # mixed: the solver stops at its FP32 floor (|dlambda| < 1e-6)
+    assert np.abs(ev_d - ev_h).max() < (1e-9 if not mixed else 1e-5), (ev_d, ev_h)
     # same invariant subspace: |X_d^H X_h| has unit singular values
     sv = np.linalg.svd(X_d.conj().T @ X_h, compute_uv=False)
-    assert np.abs(sv - 1).max() < (1e-7 if not mixed else 1e-4), sv
+    assert np.abs(sv - 1).max() < (1e-7 if not mixed else 1e-3), sv
 
 
 def test_scf_si_fcc_pbe_matches_reference_log():
