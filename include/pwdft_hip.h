@@ -210,6 +210,8 @@ int pw_herm_fix_dev(pw_ctx* ctx, int n, void* A, int64_t lda, int mode);
 int pw_potrf_dev(pw_ctx* ctx, int n, void* A, int64_t lda);
 int pw_trtri_dev(pw_ctx* ctx, int n, void* A, int64_t lda);
 int pw_heevd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
+/* eigvals(Hermitian(A)) only (A is destroyed). */
+int pw_heev_values_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
 int pw_hegvd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w);
 
 /* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */

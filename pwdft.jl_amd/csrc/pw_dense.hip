@@ -151,7 +151,15 @@ int pw_trtri_dev(pw_ctx* c, int n, void* A, int64_t lda) {
   return PW_OK;
 }
 
-int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
+static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool vectors);
+
+int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) { return heevd_impl(c, n, A, lda, w, true); }
+
+int pw_heev_values_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
+  return heevd_impl(c, n, A, lda, w, false);
+}
+
+static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool vectors) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   if (n < 0 || !A || !w || lda < (n > 0 ? n : 1)) return set_error(c, PW_ERR_ARG, "pw_heevd_dev: bad arguments");
@@ -163,8 +171,8 @@ int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
   double* D = reinterpret_cast<double*>(c->d_dwork);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
-  rc = st_check(c, rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n,
-                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda, D, D + n, dinfo),
+  rc = st_check(c, rocsolver_zheevd(h, vectors ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
+                                    n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda, D, D + n, dinfo),
                 "zheevd");
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);

@@ -96,6 +96,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     Tmp = DeviceMatrix(c, Nb, 4 * Nst)
     Sm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
     Gm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
+    Cm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
     S2 = DeviceMatrix(c, Nst, Nst)
     S3 = DeviceMatrix(c, Nst, Nst)
     x, w, p = Q.col(0), Q.col(Nst), Q.col(2 * Nst)
@@ -119,13 +120,21 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     evals_old = np.zeros(Nst)
     SMALL = 10.0 * EPS
     it_done = 0
+    rr_evals = None
     for it in range(1, NiterMax + 1):
         it_done = it
         D.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, S2.p, Nst)          # X'HX
         c.call("pw_herm_fix_dev", Nst, S2.p, Nst, 0)
-        c.call("pw_memcpy_d2d", S3.p, S2.p, 16 * Nst * Nst)
-        c.call("pw_heevd_dev", Nst, S3.p, Nst, evp)
-        evals = ev[:Nst].copy()
+        if rr_evals is None:
+            c.call("pw_memcpy_d2d", S3.p, S2.p, 16 * Nst * Nst)
+            c.call("pw_heev_values_dev", Nst, S3.p, Nst, evp)      # eigvals(Hermitian(XHX))
+            evals = ev[:Nst].copy()
+        else:
+            # X and HX are the previous Rayleigh-Ritz vectors, so eigvals(X'HX) are that
+            # step's lowest Ritz values (equal to rounding); they are reused instead of
+            # decomposing X'HX again
+            evals = rr_evals
+        rr_evals = None
         devals = np.abs(evals - evals_old)
         evals_old = evals.copy()
         D.copy(Nb, Nst, hx, w)                                      # R = HX - X*XHX
@@ -151,7 +160,18 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
             m = (j + 1) * Nst
             D.gemm(1, m, Nst, Nb, Q.p, Nb, HQ.col(j * Nst), Nb, ctypes.c_void_p(Sm.p.value + 16 * nq * j * Nst), nq)
             D64.gemm(1, m, Nst, Nb, Q.p, Nb, Q.col(j * Nst), Nb, ctypes.c_void_p(Gm.p.value + 16 * nq * j * Nst), nq)
-        c.call("pw_hegvd_dev", nq, Sm.p, nq, Gm.p, nq, evp)         # Ritz vectors in Sm
+        # eigen(T, G): G = U^H U, C = U^-H T U^-1, C y = l y, x = U^-1 y (the reduction
+        # zhegv performs, with the standard eigenproblem through rocSOLVER's zheevd)
+        c.call("pw_herm_fix_dev", nq, Sm.p, nq, 0)
+        c.call("pw_potrf_dev", nq, Gm.p, nq)
+        c.call("pw_trtri_dev", nq, Gm.p, nq)
+        c.call("pw_herm_fix_dev", nq, Gm.p, nq, 1)                   # Ui = U^-1, upper
+        D64.gemm(0, nq, nq, nq, Sm.p, nq, Gm.p, nq, Cm.p, nq)       # T Ui
+        D64.gemm(1, nq, nq, nq, Gm.p, nq, Cm.p, nq, Sm.p, nq)       # Ui^H T Ui
+        c.call("pw_heevd_dev", nq, Sm.p, nq, evp)
+        rr_evals = ev[:Nst].copy()
+        D64.gemm(0, nq, Nst, nq, Gm.p, nq, Sm.p, nq, Cm.p, nq)      # x = Ui y, lowest Nst
+        c.call("pw_memcpy_d2d", Sm.p, Cm.p, 16 * nq * Nst)            # Ritz vectors in Sm
         D.gemm(0, Nb, Nst, nq, Q.p, Nb, Sm.p, nq, t0, Nb)
         D.gemm(0, Nb, Nst, nq, HQ.p, Nb, Sm.p, nq, t1, Nb)
         if it > 1:
@@ -172,7 +192,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     evals = ev[:Nst].copy()
     D.gemm(0, Nb, Nst, Nst, x, Nb, S2.p, Nst, X.p, Nb)
     c.call("pw_synchronize")
-    for m in (Q, HQ, Tmp, Sm, Gm, S2, S3):
+    for m in (Q, HQ, Tmp, Sm, Gm, Cm, S2, S3):
         m.free()
     return evals, {"iter": it_done, "ncols": ncols}
 
