@@ -137,6 +137,43 @@ int pw_rhoe_accum(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, c
 int pw_rhoe_accum_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, const double* w,
                       unsigned flags);
 int pw_rhoe_get(pw_ctx* ctx, double* rho);
+/* Copy of the accumulated density into a device array (N doubles). */
+int pw_rhoe_get_dev(pw_ctx* ctx, double* rho);
+
+/* ---- density side of the SCF iteration on the device ------------------------------ */
+
+/* Exchange-correlation functionals of update! (Hamiltonian.jl:526-533). */
+enum { PW_XC_VWN = 0, PW_XC_PBE = 1 };
+
+/* Ham.potentials.Ps_loc (Hamiltonian.jl:119-141), N values, host pointer. */
+int pw_set_vps_loc(pw_ctx* ctx, const double* Vps);
+
+/* Symmetry operations of symmetrize_rhoe! (RhoeSymmetrizer.jl:111-236, SymmetryInfo.jl):
+ * for operation s, M[9 s .. 9 s + 8] (row-major 3 x 3 integers) maps the crystal coordinates
+ * m of a G-vector to M m (= R_s^{-T} m), t[3 s .. 3 s + 2] is its fractional translation;
+ * gm[3 g .. 3 g + 2] are the integer crystal coordinates of rho-sphere entry g (ng = the
+ * size given to pw_set_rho_sphere, which must come first).  The sphere must be closed
+ * under every M (PW_ERR_ARG otherwise).  Host pointers. */
+int pw_set_symmetry(pw_ctx* ctx, int nsym, const int* M, const double* t, int ng, const int* gm);
+
+/* update!(Ham, Rhoe) for one spin (Hamiltonian.jl:510-545) on device arrays of N doubles:
+ * V_H by Poisson_solve! (Poisson_solve.jl:22-39), V_xc by Slater + VWN or PBE, and
+ * V_Total = V_Ps_loc + V_H + V_xc written to VH / Vxc / Vtot (each may be NULL); with
+ * install != 0 V_Total also becomes the op_H potential of ispin (as pw_set_vloc).  E (host,
+ * may be NULL) receives {E_Ps_loc, E_Hartree, E_xc} of this density, the local terms of
+ * calc_energies (calc_energies.jl:245-333).  Synchronous when E is given. */
+int pw_update_potential_dev(pw_ctx* ctx, int ispin, int xc, int install, const double* rho, double* VH,
+                            double* Vxc, double* Vtot, double* E);
+
+/* Simple mixing of KS_solve_SCF! (KS_solve_SCF.jl:240-281): rho_out = beta rho_new +
+ * (1 - beta) rho_in, then rho_out = max(rho_out, eps(Float64)); device arrays, rho_out may
+ * alias either input. */
+int pw_mix_rhoe_dev(pw_ctx* ctx, double beta, const double* rho_new, const double* rho_in, double* rho_out);
+
+/* symmetrize_rhoe! in place on a device density: rho(G) on the rho sphere is replaced by
+ * 1/Nsym sum_s rho(M_s m) exp(2 pi i t_s . M_s m); components outside the sphere are kept.
+ * A no-op when pw_set_symmetry has not been called or gave one operation. */
+int pw_symmetrize_rhoe_dev(pw_ctx* ctx, double* rho);
 
 /* ---- forces and energy gradients (the adjoint of the pseudopotential energies) ---- */
 

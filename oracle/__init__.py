@@ -20,4 +20,5 @@ Modules:
   solvers    ortho_sqrt, diag_LOBPCG, diag_davidson
   energy     energies, Ewald E_NN, occupations (Fermi-Dirac), KS_solve_SCF
   forces     calc_forces_Ps_loc, calc_forces_Ps_nloc, calc_forces_NN
+  symmetry   symmetrize_rhoe
 """

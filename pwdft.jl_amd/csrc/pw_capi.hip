@@ -262,6 +262,7 @@ int pw_ctx_destroy(pw_ctx* c) {
   hipFree(c->d_tmp);
   hipFree(c->d_wts);
   free_forces(c);
+  free_scf(c);
   free_desc(c);
   free_dense(c);
   for (void* g : c->graphs)
@@ -325,6 +326,12 @@ int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
   c->kd.clear();
   free_k(c->krho);
   c->krho_g0 = -1;
+  for (auto p : c->d_v64) hipFree(p);
+  for (auto p : c->d_v32) hipFree(p);
+  c->d_v64.clear();
+  c->d_v32.clear();
+  c->nspin = 0;
+  free_scf(c);
   c->n1 = n1;
   c->n2 = n2;
   c->n3 = n3;
@@ -576,7 +583,7 @@ int pw_rhoe_zero(pw_ctx* c) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   if (!c->d_rho) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
-  return hip_check(c, hipMemsetAsync(c->d_rho, 0, c->N * sizeof(double), c->stream), "rho zero");
+  return zero_fill(c, c->d_rho, c->N);
 }
 
 int pw_rhoe_accum_dev(pw_ctx* c, int ik, int nb, const void* psi, int64_t ldpsi, const double* w,

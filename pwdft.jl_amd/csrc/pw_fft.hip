@@ -448,7 +448,9 @@ k_plane_vloc(cplx<T>* __restrict__ W, const T* __restrict__ V, const int* __rest
 // per-wavefront row buffer of n1 entries: the row is expanded from Yc (zero at inactive x),
 // transformed (inverse, * V/N, forward) and its active entries gathered back.  rl rows are
 // in flight per wavefront (rl <= gpw, sized to the LDS left after Yc).
-template <typename T>
+// With racc set the row is not transformed back: RM 0 adds wb |f(r)|^2 into racc (density),
+// RM 1 writes wb Re f(r), RM 2 adds wb Re f(r) (G->R of a real field).
+template <typename T, int RM = 0>
 __device__ inline void plane_rows_c(cplx<T>* __restrict__ Yc, int ys, cplx<T>* __restrict__ rows,
                                     const int* __restrict__ xl, int nxact, const FftDim& dx, int n2,
                                     const cplx<T>* __restrict__ tx, const T* __restrict__ v, int rl,
@@ -469,11 +471,16 @@ __device__ inline void plane_rows_c(cplx<T>* __restrict__ Yc, int ys, cplx<T>* _
     wave_fence();
     wave_line_fft<T, +1, FFT_ME>(rows, base, 1, dx, tx, q, active, nullptr);
     if (racc) {
-      // density: add w |psi(r)|^2 of this row into the block's global plane
       if (active)
         for (int i = q; i < n1; i += dx.L) {
           const double re = (double)rows[base + i].x, im = (double)rows[base + i].y;
-          racc[(long)row * n1 + i] += wb * (re * re + im * im);
+          double& o = racc[(long)row * n1 + i];
+          if (RM == 0)
+            o += wb * (re * re + im * im);
+          else if (RM == 1)
+            o = wb * re;
+          else
+            o += wb * re;
         }
       wave_fence();
       continue;
@@ -560,6 +567,74 @@ k_plane_rhoe_c(const cplx<T>* __restrict__ W, const double* __restrict__ wts, in
     plane_rows_c<T>(Yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, out, wts[b]);
     __syncthreads();
   }
+}
+
+// G->R of real fields on a sphere (FP64), after k_col_ifft: per (z-plane, field) the inverse
+// y-FFT of the active x lines and the inverse x-FFT of every row, then
+// out[f N + r] = scale Re f(r) (added to out when acc is set).
+__global__ void __launch_bounds__(PlaneNT<double>::value)
+k_plane_g2r(const cplx<double>* __restrict__ W, double* __restrict__ out, long N, double scale, int acc,
+            const int* __restrict__ col_ij, int ncol, const int* __restrict__ xact, int nxact, FftDim dx, FftDim dy,
+            const cplx<double>* __restrict__ twx, const cplx<double>* __restrict__ twy, int n3) {
+  extern __shared__ char smem_raw[];
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+  const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
+  cplx<double>* tx = lds + ps * n2;
+  cplx<double>* ty = tx + n1;
+  int* xl = reinterpret_cast<int*>(ty + n2);
+  const int z = blockIdx.x, f = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk(0.0, 0.0);
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
+  __syncthreads();
+  const cplx<double>* w = W + ((long)f * n3 + z) * ncol;
+  for (int c = tid; c < ncol; c += nthr) lds[col_ij[c]] = w[c];
+  __syncthreads();
+  wave_lines<double, +1, FFT_ME>(lds, nxact, xl, 0, ps, dy, ty);
+  __syncthreads();
+  wave_lines<double, +1, FFT_ME>(lds, n2, nullptr, ps, 1, dx, tx);
+  __syncthreads();
+  double* o = out + (long)f * N + (long)z * np;
+  for (int i = tid; i < np; i += nthr) {
+    const int j = i / n1, x = i - j * n1;
+    const double v = scale * lds[x + ps * j].x;
+    o[i] = acc ? o[i] + v : v;
+  }
+}
+
+// The same for planes beyond LDS, in the compressed form of k_plane_vloc_c.
+__global__ void __launch_bounds__(PlaneNT<double>::value)
+k_plane_g2r_c(const cplx<double>* __restrict__ W, double* __restrict__ out, long N, double scale, int acc,
+              const int* __restrict__ col_cy, int ncol, const int* __restrict__ xact, int nxact, FftDim dx,
+              FftDim dy, const cplx<double>* __restrict__ twx, const cplx<double>* __restrict__ twy, int n3,
+              int rl) {
+  extern __shared__ char smem_raw[];
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+  const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ys = odd_stride(n2);
+  cplx<double>* Yc = lds;
+  cplx<double>* tx = Yc + nxact * ys;
+  cplx<double>* ty = tx + n1;
+  cplx<double>* rows = ty + n2;
+  int* xl = reinterpret_cast<int*>(rows + (blockDim.x >> 6) * rl * n1);
+  const int z = blockIdx.x, f = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int i = tid; i < nxact * ys; i += nthr) Yc[i] = mk(0.0, 0.0);
+  stage(tx, twx, n1);
+  stage(ty, twy, n2);
+  stage(xl, xact, nxact);
+  __syncthreads();
+  const cplx<double>* w = W + ((long)f * n3 + z) * ncol;
+  for (int c = tid; c < ncol; c += nthr) Yc[col_cy[c]] = w[c];
+  __syncthreads();
+  wave_lines<double, +1, FFT_ME>(Yc, nxact, nullptr, ys, 1, dy, ty);
+  __syncthreads();
+  double* o = out + (long)f * N + (long)z * np;
+  if (acc)
+    plane_rows_c<double, 2>(Yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, o, scale);
+  else
+    plane_rows_c<double, 1>(Yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, o, scale);
 }
 
 // 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi (one band:
@@ -966,6 +1041,43 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
                        reinterpret_cast<const double*>(c->d_rpart), ngu, c->N, 1.0 / c->omega);
   }
   return hip_check(c, hipGetLastError(), "rhoe kernels");
+}
+
+int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, double* out, double scale,
+              int acc) {
+  const size_t esz = sizeof(cplx<double>);
+  const int C = cols_per_block(c, esz);
+  size_t plane_bytes = plane_lds(c, esz, k.nxact);
+  int rl = 0;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  if (compressed) {
+    rl = plane_c_rows(c, esz, k.nxact, PlaneNT<double>::value, &plane_bytes);
+    if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane too large for the G->R kernel");
+  }
+  if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
+    return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
+  const int bb = std::max(1, std::min(nf, c->band_batch));
+  int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)bb * c->n3 * k.ncol * esz);
+  if (rc) return rc;
+  cplx<double>* W = reinterpret_cast<cplx<double>*>(c->d_work);
+  const cplx<double>* tw[3];
+  for (int d = 0; d < 3; ++d) tw[d] = reinterpret_cast<const cplx<double>*>(c->d_tw64[d]);
+  for (int b0 = 0; b0 < nf; b0 += bb) {
+    const int nbb = std::min(bb, nf - b0);
+    hipLaunchKernelGGL(k_col_ifft<double>, dim3(col_grid(k.ncol, C, nbb)), dim3(COL_NT), col_lds(c, C, esz),
+                       c->stream, fG, ldf, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb,
+                       c->dim[2], tw[2]);
+    double* o = out + (long)b0 * c->N;
+    if (compressed)
+      hipLaunchKernelGGL(k_plane_g2r_c, dim3(c->n3, nbb), dim3(PlaneNT<double>::value), plane_bytes, c->stream, W,
+                         o, c->N, scale, acc, k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0],
+                         tw[1], c->n3, rl);
+    else
+      hipLaunchKernelGGL(k_plane_g2r, dim3(c->n3, nbb), dim3(PlaneNT<double>::value), plane_bytes, c->stream, W, o,
+                         c->N, scale, acc, k.d_col_ij, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0],
+                         tw[1], c->n3);
+  }
+  return hip_check(c, hipGetLastError(), "G->R kernels");
 }
 
 int rhoe_accum(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, const double* d_w,

@@ -216,6 +216,19 @@ struct pw_ctx {
   void* rb = nullptr;
   void* d_dinfo = nullptr; size_t dinfo_bytes = 0;
   void* d_dwork = nullptr; size_t dwork_bytes = 0;
+  // density side of an SCF iteration (pw_scf.hip)
+  double* d_vps = nullptr;                            // V_Ps_loc(r) [N]
+  void* d_sgrid = nullptr; size_t sgrid_bytes = 0;    // grid fields of the potential update
+  void* d_ssph = nullptr;  size_t ssph_bytes = 0;     // rho-sphere fields
+  void* d_spart = nullptr; size_t spart_bytes = 0;    // reduction partials
+  int nsym = 0;                                       // symmetry operations (0: none set)
+  int sym_ng = 0;                                     // rho-sphere size the tables were built for
+  int* d_symM = nullptr;                              // [9 nsym] m' = M m, row-major 3 x 3
+  double* d_symt = nullptr;                           // [3 nsym] fractional translations
+  int* d_gm = nullptr;                                // [3 ng] crystal coordinates of the rho-sphere G
+  int* d_gtab = nullptr;                              // box of m -> sphere entry (-1: not in the sphere)
+  int gtab_lo[3] = {0, 0, 0};
+  int gtab_span[3] = {0, 0, 0};
 };
 
 namespace pw {
@@ -253,6 +266,12 @@ void free_k(KData& k);
 template <typename T>
 int upload(pw_ctx* c, T** dst, const T* src, size_t n);
 void free_forces(pw_ctx* c);
+// Real fields on the sphere k back to the grid (FP64):
+// out[f N + r] = scale Re sum_G fG[f ldf + g] e^{+i G.r} for f < nf (added to out when acc).
+int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, double* out, double scale, int acc);
+void free_scf(pw_ctx* c);
+// out[0..n) = 0 with a kernel (safe inside stream capture, unlike a memset node).
+int zero_fill(pw_ctx* c, double* out, long n);
 
 // Batched op_V_loc (+ kinetic) over the k-points iks (pw_fft.hip); psi / hpsi per k-point.
 int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,

@@ -4,7 +4,7 @@ The compute runs in libpwdft_hip.so (hand-written gfx950 HIP kernels behind a C 
 include/pwdft_hip.h); this package is the host-side mirror of the reference's Julia API
 over that ABI (ctypes), plus the host set-up the reference also does on the host.
 """
-from ._lib import Context, DeviceMatrix, Graph, PWError, PosDefException, load
+from ._lib import Context, DeviceMatrix, DeviceVector, Graph, PWError, PosDefException, load
 from .basis import (Atoms, PsPot_GTH, PsPotNL, PWGrid, diamond_supercell, gen_lattice_fcc,
                     gen_lattice_sc, kpoint_grid)
 from .hamiltonian import (Hamiltonian, Kprec, calc_betaNL_psi, calc_rhoe, calc_rhoe_, op_H, op_H_, op_H_batch,
@@ -18,7 +18,7 @@ from .forces import (E_Ps_loc_grad, calc_forces, calc_forces_NN, calc_forces_Ps_
 
 __all__ = [
     "Atoms", "Context", "Graph", "E_Ps_loc_grad", "calc_forces", "calc_forces_NN", "calc_forces_Ps_loc",
-    "calc_forces_Ps_nloc", "rrule_E_Ps_loc", "rrule_E_Ps_nloc", "DeviceMatrix", "Hamiltonian", "KS_solve_SCF_", "Kprec", "PWError", "PWGrid",
+    "calc_forces_Ps_nloc", "rrule_E_Ps_loc", "rrule_E_Ps_nloc", "DeviceMatrix", "DeviceVector", "Hamiltonian", "KS_solve_SCF_", "Kprec", "PWError", "PWGrid",
     "PosDefException", "PsPot_GTH", "PsPotNL", "calc_betaNL_psi", "calc_energies", "calc_rhoe",
     "calc_rhoe_", "diag_LOBPCG", "diag_LOBPCG_", "diag_LOBPCG_batch", "diag_davidson", "diag_davidson_", "diamond_supercell",
     "gen_lattice_fcc", "gen_lattice_sc", "kpoint_grid", "load", "op_H", "op_H_", "op_H_batch", "op_K", "op_K_",

@@ -19,6 +19,9 @@ PW_TERM_NLOC = 4
 PW_TERM_ALL = 7
 PW_ACCUMULATE = 16
 PW_MIXED = 32
+PW_XC_VWN = 0
+PW_XC_PBE = 1
+PW_TUNE_PLANE_COMPRESSED = 5
 
 PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
 
@@ -80,6 +83,12 @@ _PROTOS = {
     "pw_profile_enable": [_vp, _i],
     "pw_profile_read": [_vp, _i, _c.POINTER(_c.c_double), _c.POINTER(_c.c_long), _c.POINTER(_c.c_double)],
     "pw_event_elapsed_ms": [_vp, _i, _i, _c.POINTER(_c.c_float)],
+    "pw_rhoe_get_dev": [_vp, _vp],
+    "pw_set_vps_loc": [_vp, _vp],
+    "pw_set_symmetry": [_vp, _i, _vp, _vp, _i, _vp],
+    "pw_update_potential_dev": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "pw_mix_rhoe_dev": [_vp, _d, _vp, _vp, _vp],
+    "pw_symmetrize_rhoe_dev": [_vp, _vp],
 }
 
 EXPORTED = sorted(list(_PROTOS) + ["pw_last_error"])
@@ -231,6 +240,53 @@ class DeviceMatrix:
         a = np.empty((self.rows, self.cols), dtype=np.complex128, order="F")
         self.ctx.call("pw_memcpy_d2h", ptr(a), self.p, self.nbytes)
         return a
+
+    def free(self):
+        if self.p is not None and self.p.value:
+            self.ctx.call("pw_free", self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h is not None:
+                self.free()
+        except Exception:
+            pass
+
+
+class DeviceVector:
+    """float64 vector in device memory (a density or a potential on the FFT grid)."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = _vp()
+        ctx.call("pw_malloc", max(16, 8 * self.n), ctypes.byref(p))
+        self.p = p
+
+    @property
+    def nbytes(self):
+        return 8 * self.n
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        v = cls(ctx, a.size)
+        v.upload(a)
+        return v
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == self.n
+        self.ctx.call("pw_memcpy_h2d", self.p, ptr(a), self.nbytes)
+
+    def download(self):
+        a = np.empty(self.n)
+        self.ctx.call("pw_memcpy_d2h", ptr(a), self.p, self.nbytes)
+        return a
+
+    def copy_from(self, other):
+        assert other.n == self.n
+        self.ctx.call("pw_memcpy_d2d", self.p, other.p, self.nbytes)
 
     def free(self):
         if self.p is not None and self.p.value:

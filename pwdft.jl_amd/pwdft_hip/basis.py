@@ -345,40 +345,23 @@ def V_Ps_loc(atoms, pw, pspots):
 
 
 class RhoeSymmetrizer:
-    """symmetrize_rhoe! (RhoeSymmetrizer.jl:111-236) for the operations of crystal_symmetry:
-    rho(G) on the rho sphere is replaced by its average over the operations {R|t},
-        rho_sym(m) = 1/Nsym sum_s rho(R_s^-T m) exp(2 pi i (R_s^-T m).t_s),
-    m = integer crystal coordinates of G; grid components outside the sphere are kept, as
-    the reference keeps them.  Host-side, once per density (O(Nsym Ng))."""
+    """Set-up data of symmetrize_rhoe! (RhoeSymmetrizer.jl:1-105) for the operations of
+    crystal_symmetry, in the form the device kernel takes (pw_set_symmetry):
+        rho_sym(m) = 1/Nsym sum_s rho(M_s m) exp(2 pi i t_s . M_s m),   M_s = R_s^-T,
+    m = integer crystal coordinates of the rho-sphere G (ng x 3), t_s the fractional
+    translation of {R_s|t_s}.  Components outside the sphere are kept, as the reference
+    keeps them.  The library checks that the sphere is closed under every M_s."""
 
     def __init__(self, atoms, pw, ops):
-        self.pw = pw
         self.Nsyms = len(ops)
-        m = np.rint(np.asarray(atoms.LatVecs, dtype=float).T @ pw.G / (2 * np.pi)).astype(np.int64)
-        lo = m.min(axis=1)
-        span = m.max(axis=1) - lo + 1
-        table = np.full(tuple(span), -1, dtype=np.int64)
-        table[tuple(m - lo[:, None])] = np.arange(pw.Ng)
-        self.idx = np.empty((self.Nsyms, pw.Ng), dtype=np.int64)
-        self.phase = np.empty((self.Nsyms, pw.Ng), dtype=complex)
-        for s, (R, t) in enumerate(ops):
-            ms = np.rint(np.linalg.inv(R).T).astype(np.int64) @ m
-            rel = ms - lo[:, None]
-            if (rel < 0).any() or (rel >= span[:, None]).any():
-                raise ValueError("RhoeSymmetrizer: the G sphere is not closed under the operations")
-            j = table[tuple(rel)]
-            if (j < 0).any():
-                raise ValueError("RhoeSymmetrizer: the G sphere is not closed under the operations")
-            self.idx[s] = j
-            self.phase[s] = np.exp(2j * np.pi * (t @ ms))
+        self.m = np.ascontiguousarray(
+            np.rint(np.asarray(atoms.LatVecs, dtype=float).T @ pw.G / (2 * np.pi)).T, dtype=np.int32)
+        self.M = np.ascontiguousarray([np.rint(np.linalg.inv(R).T) for R, _ in ops], dtype=np.int32)
+        self.t = np.ascontiguousarray([t for _, t in ops], dtype=float)
 
-    def __call__(self, rho):
-        pw = self.pw
-        r = np.asarray(rho, dtype=float).reshape(-1)
-        f = np.fft.fftn(r.reshape(pw.Ns, order="F")).reshape(-1, order="F")
-        v = f[pw.idx_g2r]
-        f[pw.idx_g2r] = np.mean(v[self.idx] * self.phase, axis=0)
-        return np.real(np.fft.ifftn(f.reshape(pw.Ns, order="F")).reshape(-1, order="F"))
+    def upload(self, ctx):
+        ctx.call("pw_set_symmetry", self.Nsyms, self.M.ctypes.data, self.t.ctypes.data, self.m.shape[0],
+                 self.m.ctypes.data)
 
 
 def symmetrize_vector(atoms, ops, v):

@@ -20,11 +20,11 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC,
-                   Context, DeviceMatrix, ptr)
+from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC, PW_XC_PBE,
+                   PW_XC_VWN, Context, DeviceMatrix, DeviceVector, ptr)
 from .basis import (PsPot_GTH, PsPotNL, PWGrid, RhoeSymmetrizer, V_Ps_loc, crystal_symmetry,
                     eval_Vloc_G, kpoint_grid, kpoint_rotations)
-from .potentials import calc_E_NN, epsxc_vxc_pbe, epsxc_vxc_vwn, guess_rhoe, poisson_solve
+from .potentials import calc_E_NN, guess_rhoe
 
 
 class Electrons:
@@ -58,13 +58,37 @@ class Electrons:
 
 
 class Potentials:
-    """Potentials.jl:7-14 (Nspin = 1)."""
+    """Potentials.jl:7-14 (Nspin = 1).  Ps_loc is built on the host (once per geometry);
+    Hartree, XC and Total are produced in device memory by Hamiltonian.update and copied to
+    the host the first time they are read after it.  A host copy that has been read (and
+    possibly modified in place, as set_Vloc does) stays authoritative until the next update."""
 
-    def __init__(self, Npoints):
+    _SHAPES = {"Hartree": False, "XC": True, "Total": True}   # name -> stored as N x 1
+
+    def __init__(self, ctx, Npoints):
         self.Ps_loc = np.zeros(Npoints)
-        self.Hartree = np.zeros(Npoints)
-        self.XC = np.zeros((Npoints, 1))
-        self.Total = np.zeros((Npoints, 1))
+        self.dev = {k: DeviceVector(ctx, Npoints) for k in self._SHAPES}
+        self._host = {k: (np.zeros((Npoints, 1)) if col else np.zeros(Npoints)) for k, col in self._SHAPES.items()}
+
+    def _get(self, name):
+        h = self._host[name]
+        if h is None:
+            v = self.dev[name].download()
+            h = v.reshape(-1, 1) if self._SHAPES[name] else v
+            self._host[name] = h
+        return h
+
+    def _set(self, name, v):
+        v = np.array(v, dtype=float)
+        self._host[name] = v.reshape(-1, 1) if self._SHAPES[name] else v.reshape(-1)
+
+    def device_updated(self):
+        """The device arrays hold the current Hartree / XC / Total (after update)."""
+        self._host = dict.fromkeys(self._host)
+
+    Hartree = property(lambda self: self._get("Hartree"), lambda self, v: self._set("Hartree", v))
+    XC = property(lambda self: self._get("XC"), lambda self, v: self._set("XC", v))
+    Total = property(lambda self: self._get("Total"), lambda self, v: self._set("Total", v))
 
 
 class Hamiltonian:
@@ -101,14 +125,15 @@ class Hamiltonian:
         self.electrons = Electrons(Ne, self.pw.Nkpt, Nstates=Nstates, extra_states=extra_states,
                                    use_smearing=use_smearing)
         self.electrons.kT = smearing_kT
-        self.potentials = Potentials(self.pw.Npoints)
-        self.potentials.Ps_loc = V_Ps_loc(atoms, self.pw, self.pspots)
-        self.potentials.Total[:, 0] = self.potentials.Ps_loc
-        self.E_NN = calc_E_NN(atoms, self.Zvals)
         self.ik = 0
         self.ispin = 0
         self.mixed = mixed
         self.ctx = Context(device)
+        self._rho_scratch = None
+        self.potentials = Potentials(self.ctx, self.pw.Npoints)
+        self.potentials.Ps_loc = V_Ps_loc(atoms, self.pw, self.pspots)
+        self.potentials.Total[:, 0] = self.potentials.Ps_loc
+        self.E_NN = calc_E_NN(atoms, self.Zvals)
         self.ctx.call("pw_set_band_batch", int(band_batch))
         self._upload_static()
         self._upload_vloc()
@@ -135,6 +160,9 @@ class Hamiltonian:
         idx = np.ascontiguousarray(pw.idx_g2r, dtype=np.int64)
         G = np.ascontiguousarray(pw.G.T, dtype=float)
         c.call("pw_set_rho_sphere", int(pw.Ng), ptr(idx), 0, ptr(G), self.atoms.Nspecies, ptr(vg))
+        if self.rhoe_symmetrizer is not None:
+            self.rhoe_symmetrizer.upload(c)
+        c.call("pw_set_vps_loc", ptr(np.ascontiguousarray(self.potentials.Ps_loc)))
         self._upload_atoms()
 
     def _upload_atoms(self):
@@ -161,6 +189,7 @@ class Hamiltonian:
             self.sym_ops = [op for op in crystal_symmetry(atoms)
                             if any(np.array_equal(op[0], R) for R, _ in self.sym_ops)]
             self.rhoe_symmetrizer = RhoeSymmetrizer(atoms, self.pw, self.sym_ops)
+            self.rhoe_symmetrizer.upload(self.ctx)
         pw, c = self.pw, self.ctx
         self.pspotNL = PsPotNL(atoms, pw, self.pspots)
         nb = self.pspotNL.NbetaNL
@@ -170,6 +199,7 @@ class Hamiltonian:
                 c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
         self._upload_atoms()
         self.potentials.Ps_loc = V_Ps_loc(atoms, pw, self.pspots)
+        c.call("pw_set_vps_loc", ptr(np.ascontiguousarray(self.potentials.Ps_loc)))
         self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + self.potentials.XC[:, 0]
         self.E_NN = calc_E_NN(atoms, self.Zvals)
         self._upload_vloc()
@@ -183,20 +213,33 @@ class Hamiltonian:
         self.potentials.Total[:, 0] = V
         self._upload_vloc()
 
-    def update(self, rho):
-        """update!(Ham, psiks, Rhoe) for Nspin = 1 (Hamiltonian.jl:510-545)."""
-        rho = np.asarray(rho, dtype=float).reshape(-1)
-        self.potentials.Hartree = poisson_solve(self.pw, rho)
-        _, vxc = self.epsxc_vxc(rho)
-        self.potentials.XC[:, 0] = vxc
-        self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + vxc
-        self._upload_vloc()
+    @property
+    def xc_code(self):
+        return PW_XC_PBE if self.xcfunc == "PBE" else PW_XC_VWN
 
-    def epsxc_vxc(self, rho):
-        """Slater + VWN, or PBE for xcfunc == "PBE" (Hamiltonian.jl:526-533)."""
-        if self.xcfunc == "PBE":
-            return epsxc_vxc_pbe(self.pw, rho)
-        return epsxc_vxc_vwn(rho)
+    def rho_scratch(self):
+        """A device vector of Npoints doubles owned by the Hamiltonian (staging of host
+        densities for the device entry points)."""
+        if self._rho_scratch is None:
+            self._rho_scratch = DeviceVector(self.ctx, self.pw.Npoints)
+        return self._rho_scratch
+
+    def update(self, rho):
+        """update!(Ham, psiks, Rhoe) for Nspin = 1 (Hamiltonian.jl:510-545), on the device:
+        Poisson_solve!, Slater + VWN or PBE (xcfunc, Hamiltonian.jl:526-533) and V_Total,
+        which becomes the op_H potential.  rho is a host array or a DeviceVector.  Returns
+        (E_Ps_loc, E_Hartree, E_xc) of this density, the local terms of calc_energies."""
+        if isinstance(rho, DeviceVector):
+            dev = rho
+        else:
+            dev = self.rho_scratch()
+            dev.upload(np.asarray(rho, dtype=float).reshape(-1))
+        p = self.potentials
+        E = np.zeros(3)
+        self.ctx.call("pw_update_potential_dev", 0, self.xc_code, 1, dev.p, p.dev["Hartree"].p, p.dev["XC"].p,
+                      p.dev["Total"].p, ptr(E))
+        p.device_updated()
+        return tuple(float(x) for x in E)
 
     def guess_rhoe(self):
         return guess_rhoe(self.atoms, self.pw, self.pspots, self.electrons.Nelectrons)
@@ -331,8 +374,9 @@ def calc_betaNL_psi(Ham, psi):
 
 def calc_rhoe_(Ham, psiks, Rhoe, Focc=None):
     """calc_rhoe! (calc_rhoe.jl:9-135, Nspin = 1): Rhoe[:, 0] is overwritten with
-    sum_k wk sum_ist Focc |psi(r)|^2 (psi(r) = N/sqrt(Omega) IFFT psi), symmetrised when the
-    k-points were reduced by crystal symmetry (calc_rhoe.jl:130-132)."""
+    sum_k wk sum_ist Focc |psi(r)|^2 (psi(r) = N/sqrt(Omega) IFFT psi), symmetrised on the
+    device when the k-points were reduced by crystal symmetry (calc_rhoe.jl:130-132).  Rhoe
+    is a host array or a DeviceVector (then the density stays in device memory)."""
     c = Ham.ctx
     Focc = Ham.electrons.Focc if Focc is None else Focc
     c.call("pw_rhoe_zero")
@@ -344,10 +388,13 @@ def calc_rhoe_(Ham, psiks, Rhoe, Focc=None):
         else:
             a = np.asfortranarray(psi, dtype=np.complex128)
             c.call("pw_rhoe_accum", ik, a.shape[1], ptr(a), a.shape[0], ptr(w), Ham.flags_prec)
-    r = np.empty(Ham.pw.Npoints)
-    c.call("pw_rhoe_get", ptr(r))
+    dev = Rhoe if isinstance(Rhoe, DeviceVector) else Ham.rho_scratch()
+    c.call("pw_rhoe_get_dev", dev.p)
     if Ham.rhoe_symmetrizer is not None:  # calc_rhoe.jl:130-132
-        r = Ham.rhoe_symmetrizer(r)
+        c.call("pw_symmetrize_rhoe_dev", dev.p)
+    if dev is Rhoe:
+        return Rhoe
+    r = dev.download()
     Rhoe = np.asarray(Rhoe)
     if Rhoe.ndim == 2:
         Rhoe[:, 0] = r

@@ -2,14 +2,15 @@
 Energies sum (Energies.jl:30-34), calc_Focc / find_E_fermi / calc_entropy
 (occupations.jl:6-245) and KS_solve_SCF! (KS_solve_SCF.jl:24-373, mix_method "simple").
 
-The wave functions stay on the device for the whole SCF: LOBPCG/Davidson, calc_rhoe and
-the psi-dependent energy terms run through libpwdft_hip; the O(N_grid) potential update,
-density mixing and the FP64 energy sums of grid quantities run on the host."""
+The wave functions, the density and the potentials stay on the device for the whole SCF:
+LOBPCG/Davidson, calc_rhoe with the density symmetrisation, simple mixing, the potential
+update (Poisson, exchange-correlation, V_Total) and the energy terms run through
+libpwdft_hip.  Per iteration the host receives the eigenvalues and a few scalars."""
 import time
 
 import numpy as np
 
-from ._lib import DeviceMatrix, PW_TERM_K, ptr
+from ._lib import DeviceMatrix, DeviceVector, PW_TERM_K, ptr
 from .hamiltonian import calc_betaNL_psi, calc_rhoe_
 from .batch import diag_LOBPCG_batch
 from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc, _Dense
@@ -62,20 +63,30 @@ def calc_E_Ps_nloc(Ham, psiks, Focc):
 
 
 def calc_E_local(Ham, rho):
-    """calc_energies.jl:245-333."""
+    """calc_energies.jl:245-333: E_Ps_loc = <V_Ps_loc, rho> dV, E_Hartree = 1/2 <V_H, rho> dV
+    with the Hartree potential of the last update, E_xc = sum eps_xc[rho] rho dV.  The
+    exchange-correlation energy density is evaluated on the device."""
     pw = Ham.pw
     dV = pw.CellVolume / pw.Npoints
-    epsxc, _ = Ham.epsxc_vxc(rho)
-    return (np.dot(Ham.potentials.Ps_loc, rho) * dV, 0.5 * np.dot(Ham.potentials.Hartree, rho) * dV,
-            np.sum(epsxc * rho) * dV)
+    if isinstance(rho, DeviceVector):
+        dev, host = rho, rho.download()
+    else:
+        host = np.asarray(rho, dtype=float).reshape(-1)
+        dev = Ham.rho_scratch()
+        dev.upload(host)
+    E = np.zeros(3)
+    Ham.ctx.call("pw_update_potential_dev", 0, Ham.xc_code, 0, dev.p, None, None, None, ptr(E))
+    return float(E[0]), 0.5 * float(np.dot(Ham.potentials.Hartree, host)) * dV, float(E[2])
 
 
-def calc_energies(Ham, psiks, rho, Focc=None, mTS=0.0):
+def calc_energies(Ham, psiks, rho, Focc=None, mTS=0.0, E_local=None):
+    """calc_energies! (calc_energies.jl:463-492).  E_local: the (E_Ps_loc, E_Hartree, E_xc)
+    that Ham.update(rho) returned for this same density, which saves evaluating them again."""
     Focc = Ham.electrons.Focc if Focc is None else Focc
     e = Energies()
     e.Kinetic = calc_E_kin(Ham, psiks, Focc)
     e.Ps_nloc = calc_E_Ps_nloc(Ham, psiks, Focc)
-    e.Ps_loc, e.Hartree, e.XC = calc_E_local(Ham, rho)
+    e.Ps_loc, e.Hartree, e.XC = calc_E_local(Ham, rho) if E_local is None else E_local
     e.NN = Ham.E_NN
     e.mTS = mTS
     return e
@@ -143,7 +154,8 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
     el = Ham.electrons
     if psiks is None:
         psiks = rand_BlochWavefunc(Ham, seed=seed)
-    rho = Ham.guess_rhoe()
+    rho = DeviceVector.from_host(Ham.ctx, Ham.guess_rhoe())
+    rho_new = DeviceVector(Ham.ctx, pw.Npoints)
     Ham.update(rho)
     Focc = el.Focc.copy()
     Etot_old = 0.0
@@ -179,11 +191,10 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
             mTS = calc_entropy(pw.wk, kT, evals, Ef)
             el.E_fermi = Ef
             el.Focc = Focc.copy()
-        rho_new = calc_rhoe_(Ham, psiks, np.zeros(pw.Npoints), Focc=Focc)
-        rho = betamix * rho_new + (1 - betamix) * rho
-        rho = np.maximum(rho, EPS)
-        Ham.update(rho)
-        e = calc_energies(Ham, psiks, rho, Focc, mTS)
+        calc_rhoe_(Ham, psiks, rho_new, Focc=Focc)
+        Ham.ctx.call("pw_mix_rhoe_dev", float(betamix), rho_new.p, rho.p, rho.p)  # clamps at eps()
+        E_local = Ham.update(rho)
+        e = calc_energies(Ham, psiks, rho, Focc, mTS, E_local=E_local)
         Etot = e.total()
         diffE = abs(Etot - Etot_old)
         history.append(Etot)
@@ -200,5 +211,5 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
         Etot_old = Etot
     el.ebands = evals
     Ham.energies = e
-    return {"energies": e.as_dict(), "evals": evals, "psiks": psiks, "rho": rho, "history": history,
+    return {"energies": e.as_dict(), "evals": evals, "psiks": psiks, "rho": rho.download(), "history": history,
             "ncols": ncols, "seconds": time.perf_counter() - t0, "iterations": len(history)}

@@ -1,0 +1,160 @@
+"""The density side of the SCF iteration on the device, through the C ABI: update!
+(Poisson_solve!, Slater + VWN and PBE exchange-correlation, V_Total and the local energy
+terms), symmetrize_rhoe!, simple mixing and calc_rhoe into device memory, against the
+oracle's restatements (oracle/density.py, oracle/symmetry.py).  FP64 throughout; the
+tolerances are relative to the largest entry."""
+import numpy as np
+import pytest
+
+from _systems import A_SI, PSP, oracle_ham, product_ham, rand_psi, rel, si_fcc_structure
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_pw(H):
+    from oracle.setup import PWGrid
+    return PWGrid(H.pw.ecutwfc, H.atoms.LatVecs, Ns=H.pw.Ns)
+
+
+def _test_density(H, seed=0):
+    """Atomic-superposition density scaled by a ramp (low-density corners where the
+    functionals' thresholds switch terms off), clamped at eps() so that it also carries
+    components outside the rho sphere."""
+    rho = H.guess_rhoe()
+    ramp = np.linspace(0.02, 1.0, rho.size)
+    rng = np.random.default_rng(seed)
+    return np.maximum(rho * ramp * (1.0 + 0.01 * rng.standard_normal(rho.size)), np.finfo(float).eps)
+
+
+def _systems():
+    import pwdft_hip as ph
+    sym, frac = si_fcc_structure()
+    yield "si2_27", lambda xc: product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, xcfunc=xc)
+    yield "si8_45", lambda xc: product_ham(["Si"] * 8, ph.diamond_supercell(1.0, 1), ph.gen_lattice_sc(A_SI),
+                                           "Si-q4.lda.gth", 20.0, xcfunc=xc)
+    yield "si8_108", lambda xc: product_ham(["Si"] * 8, ph.diamond_supercell(1.0, 1), ph.gen_lattice_sc(A_SI),
+                                            "Si-q4.lda.gth", 20.0, xcfunc=xc, Ns_=(108, 108, 108))
+
+
+@pytest.mark.parametrize("xc", ["VWN", "PBE"])
+@pytest.mark.parametrize("name,make", list(_systems()))
+def test_update_potential_vs_oracle(name, make, xc):
+    from oracle.density import Poisson_solve, calc_epsxc_Vxc_PBE, calc_epsxc_Vxc_VWN
+    print(f"case update_potential {name} {xc}", flush=True)
+    H = make(xc)
+    opw = _oracle_pw(H)
+    assert np.array_equal(opw.idx_g2r, H.pw.idx_g2r)
+    rho = _test_density(H)
+    E = H.update(rho)
+    VH = Poisson_solve(opw, rho)
+    eps, Vxc = calc_epsxc_Vxc_PBE(opw, rho) if xc == "PBE" else calc_epsxc_Vxc_VWN(rho)
+    p = H.potentials
+    assert rel(p.Hartree, VH) < 1e-12, rel(p.Hartree, VH)
+    assert rel(p.XC[:, 0], Vxc) < (1e-11 if xc == "PBE" else 1e-13), rel(p.XC[:, 0], Vxc)
+    assert rel(p.Total[:, 0], p.Ps_loc + VH + Vxc) < 1e-12
+    dV = H.pw.CellVolume / H.pw.Npoints
+    ref = (np.dot(p.Ps_loc, rho) * dV, 0.5 * np.dot(VH, rho) * dV, np.sum(eps * rho) * dV)
+    for got, want in zip(E, ref):
+        assert abs(got - want) < 1e-11 * max(abs(want), 1.0), (E, ref)
+
+
+def test_update_potential_compressed_planes():
+    """The same update with the compressed plane kernels forced on a grid whose planes fit
+    in LDS gives the uncompressed result to rounding."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import PW_TUNE_PLANE_COMPRESSED
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, xcfunc="PBE")
+    rho = _test_density(H, 3)
+    E0 = H.update(rho)
+    V0 = H.potentials.Total[:, 0].copy()
+    H.ctx.call("pw_tune_set", PW_TUNE_PLANE_COMPRESSED, 1)
+    try:
+        E1 = H.update(rho)
+    finally:
+        H.ctx.call("pw_tune_set", PW_TUNE_PLANE_COMPRESSED, 0)
+    assert rel(H.potentials.Total[:, 0], V0) < 1e-14
+    assert np.allclose(E0, E1, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_installed_potential_matches_set_vloc(mixed):
+    """op_H after update uses exactly V_Total: the same bits as uploading V_Total from the
+    host with pw_set_vloc, in FP64 and in the mixed path."""
+    import pwdft_hip as ph
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0)
+    H.update(_test_density(H, 1))
+    psi = rand_psi(np.random.default_rng(5), H.pw.Ngw[0], 8)
+    H.mixed = mixed
+    a = ph.op_H(H, psi)
+    H.set_Vloc(H.potentials.Total[:, 0].copy())
+    b = ph.op_H(H, psi)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,symbols,frac,lat,psp", [
+    ("si_fcc", *si_fcc_structure(), "fcc", "Si-q4.lda.gth"),
+    ("al_fcc", ["Al"], [[0, 0, 0]], "fcc_al", "Al-q3.lda.gth"),
+])
+def test_symmetrize_vs_oracle(name, symbols, frac, lat, psp):
+    """symmetrize_rhoe! on the device against the oracle's orbit algorithm, for a density
+    that is not band-limited (its components outside the sphere must be kept), then
+    idempotence on the device."""
+    import pwdft_hip as ph
+    from oracle.symmetry import symmetrize_rhoe
+    print(f"case symmetrize {name}", flush=True)
+    L = ph.gen_lattice_fcc(A_SI) if lat == "fcc" else ph.gen_lattice_fcc(7.6)
+    H = product_ham(symbols, frac, L, psp, 15.0, meshk=(3, 3, 3), use_symmetry=True)
+    ops = H.sym_ops
+    assert len(ops) == 48
+    inv_s = [np.rint(np.linalg.inv(R).T).astype(np.int64) for R, _ in ops]
+    ft = [-t for _, t in ops]
+    rho = np.random.default_rng(2).uniform(size=H.pw.Npoints)
+    ref = symmetrize_rhoe(_oracle_pw(H), inv_s, ft, rho)
+    d = ph.DeviceVector.from_host(H.ctx, rho)
+    H.ctx.call("pw_symmetrize_rhoe_dev", d.p)
+    got = d.download()
+    assert rel(got, ref) < 1e-13, rel(got, ref)
+    H.ctx.call("pw_symmetrize_rhoe_dev", d.p)
+    assert rel(d.download(), got) < 1e-13
+
+
+def test_mix_and_clamp():
+    import pwdft_hip as ph
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0)
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal(H.pw.Npoints)
+    b = rng.standard_normal(H.pw.Npoints)
+    da, db = ph.DeviceVector.from_host(H.ctx, a), ph.DeviceVector.from_host(H.ctx, b)
+    H.ctx.call("pw_mix_rhoe_dev", 0.3, da.p, db.p, db.p)
+    want = np.maximum(0.3 * a + 0.7 * b, np.finfo(float).eps)
+    got = db.download()
+    assert np.abs(got - want).max() < 4e-16 * np.abs(want).max()
+    assert got.min() >= np.finfo(float).eps
+
+
+def test_calc_rhoe_into_device_vector():
+    """calc_rhoe_ into a DeviceVector (symmetrised in place) equals calc_rhoe_ into a host
+    array, and the unsymmetrised sum agrees with the oracle's calc_rhoe symmetrised by the
+    oracle's symmetrize_rhoe."""
+    import pwdft_hip as ph
+    from oracle.density import calc_rhoe as o_rhoe
+    from oracle.symmetry import symmetrize_rhoe
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(3, 3, 3),
+                    use_symmetry=True, extra_states=4)
+    rng = np.random.default_rng(4)
+    host = [rand_psi(rng, n, H.electrons.Nstates) for n in H.pw.Ngw]
+    psiks = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
+    d = ph.DeviceVector(H.ctx, H.pw.Npoints)
+    ph.calc_rhoe_(H, psiks, d)
+    r = ph.calc_rhoe(H, psiks)[:, 0]
+    assert rel(d.download(), r) < 1e-15
+    oH = oracle_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, kpoints=(H.pw.k, H.pw.wk),
+                    extra_states=4)
+    ops = H.sym_ops
+    inv_s = [np.rint(np.linalg.inv(R).T).astype(np.int64) for R, _ in ops]
+    ref = symmetrize_rhoe(oH.pw, inv_s, [-t for _, t in ops], o_rhoe(oH, host, Focc=H.electrons.Focc))
+    assert rel(r, ref) < 1e-12, rel(r, ref)

