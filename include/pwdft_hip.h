@@ -214,6 +214,13 @@ int pw_calc_forces_Ps_nloc_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int
 int pw_calc_forces_Ps_nloc(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi,
                            const double* w, double* F, double* E);
 
+/* The psi-dependent terms of calc_energies for one k-point with band weights
+ * w[st] = wk[ik] * Focc[st, ik] (host, nb values): E[0] += sum_st w_st 1/2 sum_g |G+k|^2
+ * |psi_g,st|^2 (calc_E_kin, calc_energies.jl:342-375) and E[1] += sum_st w_st sum_ij D_ij
+ * Re(conj(B_i,st) B_j,st) with B = betaNL^H psi (calc_E_Ps_nloc for GTH,
+ * calc_energies.jl:7-58).  psi on the device; synchronous. */
+int pw_energies_psi_dev(pw_ctx* ctx, int ik, int nb, const void* psi, int64_t ldpsi, const double* w, double* E);
+
 /* ---- dense products of diag_LOBPCG! / diag_davidson! ------------------------------ */
 
 /* C = alpha * op(A) * B + beta * C with op(A) = A (transa 0, A m x k) or A^H (transa 1,

@@ -155,6 +155,40 @@ __global__ void __launch_bounds__(FNT) k_nloc_reduce(const cplx<double>* __restr
   }
 }
 
+// calc_E_kin (calc_energies.jl:342-375) of one k-point: part[st] = w_st / 2 sum_g |G+k|^2 |psi_g,st|^2,
+// one block per band.
+__global__ void __launch_bounds__(FNT) k_ekin(const cplx<double>* __restrict__ psi, long ld,
+                                              const double* __restrict__ gk2, int ngw, const double* __restrict__ w,
+                                              double* __restrict__ part) {
+  __shared__ double sh[FNT / 64];
+  const int st = blockIdx.x;
+  const cplx<double>* p = psi + (long)st * ld;
+  double e = 0.0;
+  for (int g = threadIdx.x; g < ngw; g += FNT) {
+    const cplx<double> v = p[g];
+    e += gk2[g] * (v.x * v.x + v.y * v.y);
+  }
+  e = block_sum(e, sh);
+  if (threadIdx.x == 0) part[st] = 0.5 * w[st] * e;
+}
+
+// calc_E_Ps_nloc (calc_energies.jl:7-58, GTH) of one k-point: part[st] = w_st sum_j
+// Re(conj((D B)_j) B_j) with B = beta^H psi (nbeta x nb); one thread per band.
+__global__ void k_enl(const cplx<double>* __restrict__ B, int nbeta, int nb, const int* __restrict__ cptr,
+                      const int* __restrict__ rows, const double* __restrict__ vals, const double* __restrict__ w,
+                      double* __restrict__ part) {
+  const int st = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st >= nb) return;
+  const cplx<double>* Bs = B + (size_t)nbeta * st;
+  double e = 0.0;
+  for (int j = 0; j < nbeta; ++j) {
+    cplx<double> db = mk(0.0, 0.0);
+    for (int q = cptr[j]; q < cptr[j + 1]; ++q) db = db + scal(vals[q], Bs[rows[q]]);
+    e += db.x * Bs[j].x + db.y * Bs[j].y;
+  }
+  part[st] = w[st] * e;
+}
+
 void free_forces(pw_ctx* c) {
   free_k(c->krho);
   c->krho_g0 = -1;
@@ -334,6 +368,54 @@ static int nloc_forces(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long 
   return PW_OK;
 }
 
+// E[0] += E_kin and E[1] += E_Ps_nloc contributions of k-point ik with band weights w.
+static int psi_energies(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, const double* w, double* E) {
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
+  if (ik < 0 || ik >= (int)c->kd.size() || c->kd[ik].ngw == 0)
+    return set_error(c, PW_ERR_STATE, "k-point " + std::to_string(ik) + " is not set up");
+  KData& k = c->kd[ik];
+  if (nb < 0 || ldpsi < k.ngw || !E || (nb > 0 && (!psi || !w)))
+    return set_error(c, PW_ERR_ARG, "calc_energies: bad arguments");
+  if (nb == 0) return PW_OK;
+  const int nbeta = k.nbeta;
+  const size_t mB = (size_t)nbeta * nb;
+  int rc = ensure(c, &c->d_fmat, &c->fmat_bytes, mB * sizeof(cplx<double>) + (size_t)nb * sizeof(double));
+  if (!rc) rc = ensure(c, &c->d_fpart, &c->fpart_bytes, 2 * (size_t)nb * sizeof(double));
+  if (!rc && nbeta > 0) rc = ensure_dcsr(c);
+  if (rc) return rc;
+  cplx<double>* Bm = reinterpret_cast<cplx<double>*>(c->d_fmat);
+  double* dw = reinterpret_cast<double*>(Bm + mB);
+  double* part = reinterpret_cast<double*>(c->d_fpart);
+  hipError_t e = hipMemcpyAsync(dw, w, nb * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "band weights upload");
+  hipLaunchKernelGGL(k_ekin, dim3(nb), dim3(FNT), 0, c->stream, psi, ldpsi, k.d_gk2, k.ngw, dw, part);
+  rc = hip_check(c, hipGetLastError(), "k_ekin");
+  if (rc) return rc;
+  if (nbeta > 0) {
+    const cplx<double> one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+    rc = zgemm(c, 1, nbeta, nb, k.ngw, one, reinterpret_cast<const cplx<double>*>(k.d_beta64), k.ngw, psi, ldpsi,
+               zero, Bm, nbeta, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_enl, dim3((nb + 63) / 64), dim3(64), 0, c->stream, Bm, nbeta, nb, c->d_dcol_ptr, c->d_drow,
+                       c->d_dval, dw, part + nb);
+    rc = hip_check(c, hipGetLastError(), "k_enl");
+    if (rc) return rc;
+  }
+  std::vector<double> hp(2 * (size_t)nb, 0.0);
+  e = hipMemcpyAsync(hp.data(), part, (nbeta > 0 ? 2 : 1) * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                     c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "energy download");
+  double ek = 0.0, en = 0.0;
+  for (int st = 0; st < nb; ++st) {
+    ek += hp[st];
+    en += hp[nb + st];
+  }
+  E[0] += ek;
+  E[1] += en;
+  return PW_OK;
+}
+
 }  // namespace pw
 
 using namespace pw;
@@ -413,6 +495,12 @@ int pw_calc_forces_Ps_nloc_dev(pw_ctx* c, int ik, int nb, const void* psi, int64
   CTX_CHECK(c);
   hipSetDevice(c->device);
   return nloc_forces(c, ik, nb, reinterpret_cast<const cplx<double>*>(psi), ldpsi, w, F, E);
+}
+
+int pw_energies_psi_dev(pw_ctx* c, int ik, int nb, const void* psi, int64_t ldpsi, const double* w, double* E) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  return psi_energies(c, ik, nb, reinterpret_cast<const cplx<double>*>(psi), ldpsi, w, E);
 }
 
 int pw_calc_forces_Ps_nloc(pw_ctx* c, int ik, int nb, const void* psi, int64_t ldpsi, const double* w,

@@ -89,6 +89,7 @@ _PROTOS = {
     "pw_update_potential_dev": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "pw_mix_rhoe_dev": [_vp, _d, _vp, _vp, _vp],
     "pw_symmetrize_rhoe_dev": [_vp, _vp],
+    "pw_energies_psi_dev": [_vp, _i, _i, _vp, _i64, _vp, _vp],
 }
 
 EXPORTED = sorted(list(_PROTOS) + ["pw_last_error"])

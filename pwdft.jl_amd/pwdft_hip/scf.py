@@ -10,10 +10,10 @@ import time
 
 import numpy as np
 
-from ._lib import DeviceMatrix, DeviceVector, PW_TERM_K, ptr
-from .hamiltonian import calc_betaNL_psi, calc_rhoe_
+from ._lib import DeviceMatrix, DeviceVector, ptr
+from .hamiltonian import calc_rhoe_
 from .batch import diag_LOBPCG_batch
-from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc, _Dense
+from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc
 
 EPS = np.finfo(float).eps
 
@@ -33,33 +33,26 @@ class Energies:
         return d
 
 
-def calc_E_kin(Ham, psiks, Focc):
-    """calc_energies.jl:342-375: 0.5 sum_k wk sum_ist Focc <psi|G+k|^2|psi>, with the
-    kinetic image K psi and the diagonal of psi' (K psi) formed on the device."""
-    E = 0.0
-    D = _Dense(Ham, False)
+def _psi_energies(Ham, psiks, Focc):
+    """(E_kin, E_Ps_nloc): calc_energies.jl:342-375 (0.5 sum_k wk sum_ist Focc <psi|G+k|^2|psi>)
+    and calc_energies.jl:7-58 (GTH: sum_k wk sum_ist Focc h_ij Re(conj(<b_i|psi>) <b_j|psi>)),
+    each k-point reduced on the device (pw_energies_psi_dev)."""
+    E = np.zeros(2)
     for ik, psi in enumerate(psiks):
-        Ham.ik = ik
-        Kp = DeviceMatrix(Ham.ctx, psi.rows, psi.cols)
-        Ham.ctx.call("pw_op_H_dev", ik, 0, psi.cols, psi.p, psi.rows, Kp.p, Kp.rows, PW_TERM_K)
-        d = np.real(np.diag(D.gram(psi.p, Kp.p, psi.cols, psi.cols, psi.rows)))
-        E += Ham.pw.wk[ik] * np.sum(Focc[:, ik] * d)
-        Kp.free()
-    return E  # (the 0.5 is inside op_K)
+        dev = psi if isinstance(psi, DeviceMatrix) else DeviceMatrix.from_host(Ham.ctx, psi)
+        w = np.ascontiguousarray(Ham.pw.wk[ik] * Focc[:, ik], dtype=float)
+        Ham.ctx.call("pw_energies_psi_dev", ik, dev.cols, dev.p, dev.rows, ptr(w), ptr(E))
+    return float(E[0]), float(E[1])
+
+
+def calc_E_kin(Ham, psiks, Focc):
+    """calc_energies.jl:342-375."""
+    return _psi_energies(Ham, psiks, Focc)[0]
 
 
 def calc_E_Ps_nloc(Ham, psiks, Focc):
-    """calc_energies.jl:7-58 (GTH): sum wk Focc h_ij Re(conj(<b_i|psi>) <b_j|psi>)."""
-    nl = Ham.pspotNL
-    if nl.NbetaNL == 0:
-        return 0.0
-    E = 0.0
-    for ik, psi in enumerate(psiks):
-        Ham.ik = ik
-        bp = calc_betaNL_psi(Ham, psi)          # Nst x Nbeta
-        enl = np.real(np.einsum("si,ij,sj->s", bp.conj(), nl.D, bp))
-        E += Ham.pw.wk[ik] * np.sum(Focc[:, ik] * enl)
-    return E
+    """calc_energies.jl:7-58 (GTH)."""
+    return _psi_energies(Ham, psiks, Focc)[1]
 
 
 def calc_E_local(Ham, rho):
@@ -84,8 +77,7 @@ def calc_energies(Ham, psiks, rho, Focc=None, mTS=0.0, E_local=None):
     that Ham.update(rho) returned for this same density, which saves evaluating them again."""
     Focc = Ham.electrons.Focc if Focc is None else Focc
     e = Energies()
-    e.Kinetic = calc_E_kin(Ham, psiks, Focc)
-    e.Ps_nloc = calc_E_Ps_nloc(Ham, psiks, Focc)
+    e.Kinetic, e.Ps_nloc = _psi_energies(Ham, psiks, Focc)
     e.Ps_loc, e.Hartree, e.XC = calc_E_local(Ham, rho) if E_local is None else E_local
     e.NN = Ham.E_NN
     e.mTS = mTS

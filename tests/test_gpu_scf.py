@@ -158,3 +158,23 @@ def test_calc_rhoe_into_device_vector():
     inv_s = [np.rint(np.linalg.inv(R).T).astype(np.int64) for R, _ in ops]
     ref = symmetrize_rhoe(oH.pw, inv_s, [-t for _, t in ops], o_rhoe(oH, host, Focc=H.electrons.Focc))
     assert rel(r, ref) < 1e-12, rel(r, ref)
+
+
+def test_psi_energies_vs_oracle():
+    """E_kin and E_Ps_nloc reduced on the device (pw_energies_psi_dev) against the oracle's
+    calc_E_kin / calc_E_Ps_nloc, Si fcc on a 2x2x2 mesh, random occupations."""
+    import pwdft_hip as ph
+    from pwdft_hip.scf import _psi_energies
+    from oracle.energy import calc_E_Ps_nloc, calc_E_kin
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(2, 2, 2), extra_states=4)
+    oH = oracle_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, kpoints=(H.pw.k, H.pw.wk),
+                    extra_states=4)
+    rng = np.random.default_rng(12)
+    host = [rand_psi(rng, n, H.electrons.Nstates) for n in H.pw.Ngw]
+    Focc = rng.uniform(0.0, 2.0, size=(H.electrons.Nstates, H.pw.Nkpt))
+    psiks = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
+    ek, en = _psi_energies(H, psiks, Focc)
+    ek0, en0 = calc_E_kin(oH, host, Focc), calc_E_Ps_nloc(oH, host, Focc)
+    assert abs(ek - ek0) < 1e-12 * abs(ek0), (ek, ek0)
+    assert abs(en - en0) < 1e-12 * abs(en0), (en, en0)
