@@ -5,11 +5,13 @@ Julia binding of libpwdft_hip.so (include/pwdft_hip.h): the drop-in for PWDFT.jl
 Hamiltonian application, density accumulation and eigensolver products on MI355X.
 
 Loading this module after `using PWDFT` adds methods for `Hamiltonian{PsPot_GTH}` that
-route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! through the C ABI.  The
-reference's own KS_solve_SCF!, diag_LOBPCG! and diag_davidson! then run unchanged on top
-of them (they call op_H).  A HIP context is attached to each Hamiltonian the first time it
-is used, keyed by object identity; `sync_potential!(Ham)` re-uploads V_loc after
-`update!` (call it from a wrapper of update!, or use `KS_solve_SCF_hip!`).
+route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! (with symmetrize_rhoe!) and
+update! (Poisson, VWN or PBE, V_Total) through the C ABI.  The reference's own
+KS_solve_SCF!, diag_LOBPCG! and diag_davidson! then run unchanged on top of them (they call
+op_H, calc_rhoe! and update!).  A HIP context is attached to each Hamiltonian the first
+time it is used, keyed by object identity; the device update! installs V_Total as the op_H
+potential itself, and `sync_potential!(Ham)` re-uploads Ham.potentials.Total for code that
+writes it directly.
 
 Index conventions: Julia's 1-based idx_gw2r is passed with index_base = 1; Ham.ik and
 Ham.ispin are converted to 0-based at the call.
@@ -18,7 +20,8 @@ module PWDFTHip
 
 using PWDFT
 import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
-              calc_rhoe!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH, BlochWavefunc
+              calc_rhoe!, update!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH,
+              BlochWavefunc
 
 const libpw = joinpath(@__DIR__, "..", "pwdft_hip", "libpwdft_hip.so")
 
@@ -32,6 +35,7 @@ const PW_MIXED = UInt32(32)
 mutable struct HipCtx
     h::Ptr{Cvoid}
     mixed::Bool
+    grid::Vector{Ptr{Cvoid}}   # device rho, V_H, V_xc, V_Total of update! (N doubles each)
 end
 
 const CONTEXTS = IdDict{Any,HipCtx}()
@@ -51,7 +55,7 @@ function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
     r = Ref{Ptr{Cvoid}}(C_NULL)
     rc = ccall((:pw_ctx_create, libpw), Cint, (Ref{Ptr{Cvoid}}, Cint), r, device)
     rc == 0 || error("pw_ctx_create failed ($rc)")
-    ctx = HipCtx(r[], false)
+    ctx = HipCtx(r[], false, Ptr{Cvoid}[])
     pw = Ham.pw
     Ns = pw.Ns
     check(ctx, ccall((:pw_set_grid, libpw), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble),
@@ -86,8 +90,68 @@ function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
     CONTEXTS[Ham] = ctx
     sync_potential!(Ham)
     sync_atoms!(Ham)
+    sync_symmetry!(Ham)
+    check(ctx, ccall((:pw_set_vps_loc, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, Ham.potentials.Ps_loc))
     return ctx
 end
+
+"""symmetrize_rhoe! tables (RhoeSymmetrizer.jl:111-236) from Ham.sym_info: the library's
+operation s maps crystal coordinates m of G to inv_s m with phase exp(2 pi i t.(inv_s m)),
+t = -ft in the reference's sign convention."""
+function sync_symmetry!(Ham::Hamiltonian{PsPot_GTH})
+    ctx = CONTEXTS[Ham]
+    si = Ham.sym_info
+    si.Nsyms > 1 || return nothing
+    M = zeros(Cint, 9 * si.Nsyms)
+    t = zeros(Float64, 3 * si.Nsyms)
+    for s in 1:si.Nsyms, i in 1:3
+        for j in 1:3
+            M[9 * (s - 1) + 3 * (i - 1) + j] = round(Cint, si.inv_s[i, j, s])
+        end
+        t[3 * (s - 1) + i] = -si.ft[i, s]
+    end
+    gm = round.(Cint, (Ham.pw.LatVecs' / (2 * pi)) * Ham.pw.gvec.G)   # 3 x Ng, column-major
+    check(ctx, ccall((:pw_set_symmetry, libpw), Cint, (Ptr{Cvoid}, Cint, Ptr{Cint}, Ptr{Float64}, Cint, Ptr{Cint}),
+                     ctx.h, si.Nsyms, M, t, Ham.pw.gvec.Ng, gm))
+    return nothing
+end
+
+function grid_buffers(ctx::HipCtx, N::Int)
+    if isempty(ctx.grid)
+        for _ in 1:4
+            r = Ref{Ptr{Cvoid}}(C_NULL)
+            check(ctx, ccall((:pw_malloc, libpw), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}), ctx.h, 8 * N, r))
+            push!(ctx.grid, r[])
+        end
+    end
+    return ctx.grid
+end
+
+h2d!(ctx, d, a) = check(ctx, ccall((:pw_memcpy_h2d, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Csize_t),
+                                   ctx.h, d, a, 8 * length(a)))
+d2h!(ctx, a, d) = check(ctx, ccall((:pw_memcpy_d2h, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Csize_t),
+                                   ctx.h, a, d, 8 * length(a)))
+
+# update! (Hamiltonian.jl:510-545, Nspin = 1): Poisson_solve!, calc_Vxc_VWN! / calc_Vxc_PBE!
+# and V_Total on the device; Ham.potentials receives the three fields and V_Total is the op_H
+# potential from here on.
+function update!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::AbstractVector{Float64})
+    ctx = context(Ham)
+    N = length(Rhoe)
+    rho, vh, vxc, vtot = grid_buffers(ctx, N)
+    h2d!(ctx, rho, Vector{Float64}(Rhoe))
+    xc = Ham.xcfunc == "PBE" ? Cint(1) : Cint(0)
+    E = zeros(Float64, 3)
+    check(ctx, ccall((:pw_update_potential_dev, libpw), Cint,
+                     (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
+                     ctx.h, 0, xc, 1, rho, vh, vxc, vtot, E))
+    d2h!(ctx, Ham.potentials.Hartree, vh)
+    d2h!(ctx, view(Ham.potentials.XC, :, 1), vxc)
+    d2h!(ctx, view(Ham.potentials.Total, :, 1), vtot)
+    return
+end
+update!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64}) =
+    update!(Ham, psiks, view(Rhoe, :, 1))
 
 """Upload Ham.potentials.Total (call after update!)."""
 function sync_potential!(Ham::Hamiltonian{PsPot_GTH})
@@ -124,7 +188,7 @@ op_V_Ps_nloc(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham,
 op_V_Ps_nloc!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
     (apply!(Ham, psi, Hpsi, PW_TERM_NLOC | PW_ACCUMULATE); nothing)
 
-# calc_rhoe.jl:9-135 (norm-conserving, Nspin = 1; symmetrisation stays in PWDFT)
+# calc_rhoe.jl:9-135 (norm-conserving, Nspin = 1), symmetrised on the device
 function calc_rhoe!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64})
     ctx = context(Ham)
     pw = Ham.pw
@@ -139,9 +203,13 @@ function calc_rhoe!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Mat
                          ctx.h, ik - 1, size(psi, 2), psi, size(psi, 1), w,
                          ctx.mixed ? PW_MIXED : UInt32(0)))
     end
-    check(ctx, ccall((:pw_rhoe_get, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, view(Rhoe, :, 1)))
     if Ham.sym_info.Nsyms > 1
-        PWDFT.symmetrize_rhoe!(Ham.pw, Ham.sym_info, Ham.rhoe_symmetrizer, Rhoe)
+        rho = grid_buffers(ctx, size(Rhoe, 1))[1]
+        check(ctx, ccall((:pw_rhoe_get_dev, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, rho))
+        check(ctx, ccall((:pw_symmetrize_rhoe_dev, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, rho))
+        d2h!(ctx, view(Rhoe, :, 1), rho)
+    else
+        check(ctx, ccall((:pw_rhoe_get, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, view(Rhoe, :, 1)))
     end
     return
 end
@@ -238,15 +306,10 @@ if isdefined(Main, :ChainRulesCore)
     end
 end
 
-"""KS_solve_SCF! with the potential kept in sync on the device after every update!."""
+"""The reference's KS_solve_SCF! on the device path (op_H, calc_rhoe!, update! above),
+optionally with the mixed-precision Hamiltonian."""
 function KS_solve_SCF_hip!(Ham::Hamiltonian{PsPot_GTH}; mixed::Bool=false, kwargs...)
     context(Ham).mixed = mixed
-    orig = PWDFT.update!
-    # PWDFT calls update!(Ham, psiks, Rhoe); re-upload V_loc right after it
-    @eval PWDFT function update!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64})
-        invoke($orig, Tuple{Hamiltonian, BlochWavefunc, Matrix{Float64}}, Ham, psiks, Rhoe)
-        $(sync_potential!)(Ham)
-    end
     PWDFT.KS_solve_SCF!(Ham; kwargs...)
 end
 
