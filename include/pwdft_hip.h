@@ -170,6 +170,23 @@ int pw_update_potential_dev(pw_ctx* ctx, int ispin, int xc, int install, const d
  * alias either input. */
 int pw_mix_rhoe_dev(pw_ctx* ctx, double beta, const double* rho_new, const double* rho_in, double* rho_out);
 
+/* Vector primitives of the density mixers (mix_broyden.jl, mix_pulay.jl, mix_rpulay.jl,
+ * mix_ppulay.jl, mix_anderson.jl) on device vectors of n doubles; the host keeps only the
+ * small history matrices.  a, b, x are host arrays of device pointers.
+ *   pw_vec_dots_dev:    out[q] = a[q] . b[q] for q < npair <= 64 (host out; synchronous,
+ *                       summation order fixed, so repeated calls give the same bits);
+ *   pw_vec_lincomb_dev: y = ay * y + sum_j coef[j] x[j], m <= 32 (y is not read when
+ *                       ay == 0; y may be one of the x[j]). */
+int pw_vec_dots_dev(pw_ctx* ctx, int64_t n, int npair, const double* const* a, const double* const* b, double* out);
+int pw_vec_lincomb_dev(pw_ctx* ctx, int64_t n, int m, const double* coef, const double* const* x, double ay,
+                       double* y);
+/* y = max(y, lo) element-wise (the eps() clamp of KS_solve_SCF.jl:277-281 after mixing). */
+int pw_vec_clamp_dev(pw_ctx* ctx, int64_t n, double lo, double* y);
+/* mix_adaptive! (mix_adaptive.jl:34-54) on device vectors: mu (the density, updated), nu
+ * (the new density), beta and f (per-point state, n doubles each). */
+int pw_mix_adaptive_dev(pw_ctx* ctx, int64_t n, double beta0, double betamax, double* mu, const double* nu,
+                        double* beta, double* f);
+
 /* symmetrize_rhoe! in place on a device density: rho(G) on the rho sphere is replaced by
  * 1/Nsym sum_s rho(M_s m) exp(2 pi i t_s . M_s m); components outside the sphere are kept.
  * A no-op when pw_set_symmetry has not been called or gave one operation. */

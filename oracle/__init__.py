@@ -21,4 +21,5 @@ Modules:
   energy     energies, Ewald E_NN, occupations (Fermi-Dirac), KS_solve_SCF
   forces     calc_forces_Ps_loc, calc_forces_Ps_nloc, calc_forces_NN
   symmetry   symmetrize_rhoe
+  mixing     mix_broyden, mix_pulay, mix_rpulay, mix_ppulay, mix_anderson, mix_adaptive
 """

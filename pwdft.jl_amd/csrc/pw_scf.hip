@@ -287,6 +287,72 @@ __global__ void __launch_bounds__(SNT) k_dot3(const double* __restrict__ rho, co
   }
 }
 
+// ---- vector primitives of the density mixers (mix_*.jl) -------------------------------------
+
+constexpr int VDOT_MAX = 64;   // dot products per launch
+constexpr int VLIN_MAX = 32;   // vectors of one linear combination
+constexpr int VDOT_BLK = 128;  // blocks per dot product
+
+struct VecPairs {
+  const double* a[VDOT_MAX];
+  const double* b[VDOT_MAX];
+};
+struct VecLin {
+  const double* x[VLIN_MAX];
+  double c[VLIN_MAX];
+};
+
+// part[blk * npair + q] = block blk's share of a_q . b_q (fixed order).
+__global__ void __launch_bounds__(SNT) k_vdots(VecPairs v, long n, int npair, double* __restrict__ part) {
+  __shared__ double sh[SNT / 64];
+  const int q = blockIdx.y;
+  const double* __restrict__ a = v.a[q];
+  const double* __restrict__ b = v.b[q];
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)SNT + threadIdx.x; i < n; i += (long)gridDim.x * SNT) s += a[i] * b[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < SNT / 64; ++i) t += sh[i];
+    part[(long)blockIdx.x * npair + q] = t;
+  }
+}
+
+// y = ay y + sum_j c_j x_j (y is not read when ay == 0; y may be one of the x_j).
+__global__ void k_vlin(VecLin v, int m, double ay, double* y, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double t = ay != 0.0 ? ay * y[i] : 0.0;
+    for (int j = 0; j < m; ++j) t += v.c[j] * v.x[j][i];
+    y[i] = t;
+  }
+}
+
+__global__ void k_vclamp(double* __restrict__ y, long n, double lo) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (y[i] < lo) y[i] = lo;
+}
+
+// mix_adaptive! (mix_adaptive.jl:34-54): per point, beta grows by beta0 (capped at betamax)
+// while the residual keeps its sign and is halved towards beta0 when it flips.
+__global__ void k_mix_adaptive(double* __restrict__ mu, const double* __restrict__ nu, double* __restrict__ beta,
+                               double* __restrict__ f, long n, double beta0, double betamax) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double t = nu[i] - mu[i];
+    double b = beta[i];
+    if (t * f[i] >= 0.0) {
+      b = b + beta0;
+      if (b > betamax) b = betamax;
+    } else {
+      b = 0.5 * (b + beta0);
+    }
+    beta[i] = b;
+    f[i] = t;
+    mu[i] = b * nu[i] + (1.0 - b) * mu[i];
+  }
+}
+
 // ---- drivers -------------------------------------------------------------------------------
 
 void free_scf(pw_ctx* c) {
@@ -553,6 +619,73 @@ int pw_symmetrize_rhoe_dev(pw_ctx* c, double* rho) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   return symmetrize(c, rho);
+}
+
+int pw_vec_dots_dev(pw_ctx* c, int64_t n, int npair, const double* const* a, const double* const* b, double* out) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || npair < 0 || npair > VDOT_MAX || (npair > 0 && (!a || !b || !out)))
+    return set_error(c, PW_ERR_ARG, "pw_vec_dots_dev: bad arguments (at most 64 pairs)");
+  if (npair == 0) return PW_OK;
+  VecPairs v{};
+  for (int q = 0; q < npair; ++q) {
+    if (!a[q] || !b[q]) return set_error(c, PW_ERR_ARG, "pw_vec_dots_dev: null vector");
+    v.a[q] = a[q];
+    v.b[q] = b[q];
+  }
+  int rc = ensure(c, &c->d_spart, &c->spart_bytes,
+                  std::max<size_t>(3 * (size_t)SRED, (size_t)VDOT_BLK * npair) * sizeof(double));
+  if (rc) return rc;
+  double* part = reinterpret_cast<double*>(c->d_spart);
+  hipLaunchKernelGGL(k_vdots, dim3(VDOT_BLK, npair), dim3(SNT), 0, c->stream, v, (long)n, npair, part);
+  rc = hip_check(c, hipGetLastError(), "k_vdots");
+  if (rc) return rc;
+  std::vector<double> hp((size_t)VDOT_BLK * npair);
+  hipError_t e = hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "dot products download");
+  for (int q = 0; q < npair; ++q) {
+    double t = 0.0;
+    for (int b = 0; b < VDOT_BLK; ++b) t += hp[(size_t)b * npair + q];
+    out[q] = t;
+  }
+  return PW_OK;
+}
+
+int pw_vec_lincomb_dev(pw_ctx* c, int64_t n, int m, const double* coef, const double* const* x, double ay, double* y) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || m < 0 || m > VLIN_MAX || !y || (m > 0 && (!coef || !x)))
+    return set_error(c, PW_ERR_ARG, "pw_vec_lincomb_dev: bad arguments (at most 32 vectors)");
+  if (n == 0) return PW_OK;
+  VecLin v{};
+  for (int j = 0; j < m; ++j) {
+    if (!x[j]) return set_error(c, PW_ERR_ARG, "pw_vec_lincomb_dev: null vector");
+    v.x[j] = x[j];
+    v.c[j] = coef[j];
+  }
+  hipLaunchKernelGGL(k_vlin, dim3(grid_blocks(n)), dim3(SNT), 0, c->stream, v, m, ay, y, (long)n);
+  return hip_check(c, hipGetLastError(), "k_vlin");
+}
+
+int pw_vec_clamp_dev(pw_ctx* c, int64_t n, double lo, double* y) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !y) return set_error(c, PW_ERR_ARG, "pw_vec_clamp_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  hipLaunchKernelGGL(k_vclamp, dim3(grid_blocks(n)), dim3(SNT), 0, c->stream, y, (long)n, lo);
+  return hip_check(c, hipGetLastError(), "k_vclamp");
+}
+
+int pw_mix_adaptive_dev(pw_ctx* c, int64_t n, double beta0, double betamax, double* mu, const double* nu,
+                        double* beta, double* f) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || !mu || !nu || !beta || !f) return set_error(c, PW_ERR_ARG, "pw_mix_adaptive_dev: bad arguments");
+  if (n == 0) return PW_OK;
+  hipLaunchKernelGGL(k_mix_adaptive, dim3(grid_blocks(n)), dim3(SNT), 0, c->stream, mu, nu, beta, f, (long)n, beta0,
+                     betamax);
+  return hip_check(c, hipGetLastError(), "k_mix_adaptive");
 }
 
 int pw_rhoe_get_dev(pw_ctx* c, double* rho) {

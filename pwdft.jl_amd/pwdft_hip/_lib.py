@@ -90,6 +90,10 @@ _PROTOS = {
     "pw_mix_rhoe_dev": [_vp, _d, _vp, _vp, _vp],
     "pw_symmetrize_rhoe_dev": [_vp, _vp],
     "pw_energies_psi_dev": [_vp, _i, _i, _vp, _i64, _vp, _vp],
+    "pw_vec_dots_dev": [_vp, _i64, _i, _vp, _vp, _vp],
+    "pw_vec_lincomb_dev": [_vp, _i64, _i, _vp, _vp, _d, _vp],
+    "pw_mix_adaptive_dev": [_vp, _i64, _d, _d, _vp, _vp, _vp, _vp],
+    "pw_vec_clamp_dev": [_vp, _i64, _d, _vp],
 }
 
 EXPORTED = sorted(list(_PROTOS) + ["pw_last_error"])

@@ -13,6 +13,7 @@ import numpy as np
 from ._lib import DeviceMatrix, DeviceVector, ptr
 from .hamiltonian import calc_rhoe_
 from .batch import diag_LOBPCG_batch
+from .mixing import DensityMixer
 from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc
 
 EPS = np.finfo(float).eps
@@ -132,8 +133,10 @@ def calc_entropy(wk, kT, evals, Ef):
 
 def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6,
                   ethr_evals_last=1e-5, update_psi="LOBPCG", use_smearing=False, kT=1e-3,
-                  seed=1234, verbose=False, mixed_until=None, batched=False, mixed_ethr_floor=1e-6):
-    """KS_solve_SCF! (KS_solve_SCF.jl:24-373), mix_method = "simple", Nspin = 1.
+                  seed=1234, verbose=False, mixed_until=None, batched=False, mixed_ethr_floor=1e-6,
+                  mix_method="simple", mixdim=5):
+    """KS_solve_SCF! (KS_solve_SCF.jl:24-373), Nspin = 1, with the reference's mix_method
+    choices (simple, linear_adaptive, broyden, pulay, rpulay, ppulay, anderson; mixing.py).
 
     `mixed_until`: when Ham.mixed is set, switch the Hamiltonian back to FP64 once the
     energy change falls below this threshold (FP64 refinement of the final iterations).
@@ -148,6 +151,7 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
         psiks = rand_BlochWavefunc(Ham, seed=seed)
     rho = DeviceVector.from_host(Ham.ctx, Ham.guess_rhoe())
     rho_new = DeviceVector(Ham.ctx, pw.Npoints)
+    mixer = DensityMixer(Ham, mix_method, betamix, mixdim)
     Ham.update(rho)
     Focc = el.Focc.copy()
     Etot_old = 0.0
@@ -184,7 +188,7 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
             el.E_fermi = Ef
             el.Focc = Focc.copy()
         calc_rhoe_(Ham, psiks, rho_new, Focc=Focc)
-        Ham.ctx.call("pw_mix_rhoe_dev", float(betamix), rho_new.p, rho.p, rho.p)  # clamps at eps()
+        mixer.mix(rho, rho_new, it)       # then clamps at eps()
         E_local = Ham.update(rho)
         e = calc_energies(Ham, psiks, rho, Focc, mTS, E_local=E_local)
         Etot = e.total()
