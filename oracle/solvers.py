@@ -175,3 +175,72 @@ def diag_davidson(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
             break
         evals_old = evals.copy()
     return evals, it_done, ncols
+
+
+# ---------------------------------------------------------------------------- CheFSI
+
+
+def chebyfilt(opH, X, degree, lb, ub):
+    """CheFSI.jl:36-62: the scaled three-term Chebyshev recurrence, which returns
+    T_m((H - c)/e) X / T_m(sigma1^-1) with e = (ub - lb)/2, c = (ub + lb)/2 and
+    sigma1 = e / (lb - ub) = -1/2."""
+    ee = (ub - lb) / 2
+    c = (ub + lb) / 2
+    sigma = ee / (lb - ub)
+    sigma1 = sigma
+    Y = (opH(X) - X * c) * sigma1 / ee
+    for _ in range(2, degree + 1):
+        sigma2 = 1 / (2 / sigma1 - sigma)
+        Y1 = (opH(Y) - Y * c) * 2 * sigma2 / ee - X * (sigma * sigma2)
+        X = Y
+        Y = Y1
+        sigma = sigma2
+    return Y
+
+
+def get_ub_lb_lanczos(opH, v0, nlancz):
+    """CheFSI.jl:65-130: nlancz Lanczos steps from v0 with one reorthogonalisation per step;
+    returns (lb, ub) = (the nlancz/2-th eigenvalue of T, |T 1/sqrt(n)| + |f|).  T is the
+    reference's upper-Hessenberg T (sub-diagonal beta, upper part real(h)); its eigenvalues
+    are real to rounding and are sorted by real part as Julia's eigvals sorts them."""
+    n = v0.size
+    V = np.zeros((n, nlancz), dtype=complex)
+    HV = np.zeros((n, nlancz), dtype=complex)
+    T = np.zeros((nlancz, nlancz))
+    h = np.zeros(nlancz, dtype=complex)
+    s = np.zeros(nlancz, dtype=complex)
+    V[:, 0] = v0 / np.linalg.norm(v0)
+    HV[:, 0] = opH(V[:, :1])[:, 0]
+    h[0] = np.real(np.vdot(V[:, 0], HV[:, 0]))
+    T[0, 0] = h[0].real
+    f = HV[:, 0] - V[:, 0] * h[0]
+    s[0] = np.vdot(V[:, 0], f)
+    h[0] += s[0]
+    f = f - V[:, 0] * s[0]
+    for j in range(1, nlancz):
+        beta = np.linalg.norm(f)
+        T[j, j - 1] = beta
+        V[:, j] = f / beta
+        HV[:, j] = opH(V[:, j:j + 1])[:, 0]
+        h[:j + 1] = V[:, :j + 1].conj().T @ HV[:, j]
+        f = HV[:, j] - V[:, :j + 1] @ h[:j + 1]
+        s[:j + 1] = V[:, :j + 1].conj().T @ f
+        h[:j + 1] += s[:j + 1]
+        f = f - V[:, :j + 1] @ s[:j + 1]
+        T[:j + 1, j] = np.real(h[:j + 1])
+    ev = np.linalg.eigvals(T)
+    ev = ev[np.lexsort((ev.imag, ev.real))]
+    lb = ev[nlancz // 2 - 1].real
+    ub = np.linalg.norm(T @ (np.ones(nlancz) / np.sqrt(nlancz))) + np.linalg.norm(f)
+    return lb, ub
+
+
+def diag_CheFSI(opH, X, degree, v0):
+    """diag_CheFSI! (CheFSI.jl:1-31) for one k-point; returns (evals, filtered X).  The
+    reference binds `ub, lb = get_ub_lb_lanczos(...)` to a function returning (lb, ub), so
+    the filter receives the two bounds exchanged; that is kept (with sigma1 = -1/2 it only
+    changes the filtered block's overall sign, which ortho_sqrt and eigvals do not see)."""
+    ub, lb = get_ub_lb_lanczos(opH, v0, 2 * X.shape[1])
+    Y = ortho_sqrt(chebyfilt(opH, X, degree, lb, ub))
+    Hr = Y.conj().T @ opH(Y)
+    return np.linalg.eigvalsh(herm_u(Hr)), Y

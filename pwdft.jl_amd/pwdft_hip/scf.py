@@ -14,6 +14,7 @@ from ._lib import DeviceMatrix, DeviceVector, ptr
 from .hamiltonian import calc_rhoe_
 from .batch import diag_LOBPCG_batch
 from .mixing import DensityMixer
+from .chefsi import diag_CheFSI_
 from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc
 
 EPS = np.finfo(float).eps
@@ -134,7 +135,7 @@ def calc_entropy(wk, kT, evals, Ef):
 def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6,
                   ethr_evals_last=1e-5, update_psi="LOBPCG", use_smearing=False, kT=1e-3,
                   seed=1234, verbose=False, mixed_until=None, batched=False, mixed_ethr_floor=1e-6,
-                  mix_method="simple", mixdim=5):
+                  mix_method="simple", mixdim=5, cheby_degree=8):
     """KS_solve_SCF! (KS_solve_SCF.jl:24-373), Nspin = 1, with the reference's mix_method
     choices (simple, linear_adaptive, broyden, pulay, rpulay, ppulay, anderson; mixing.py).
 
@@ -162,7 +163,10 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
     history = []
     ncols = 0
     t0 = time.perf_counter()
+    if update_psi not in ("LOBPCG", "davidson", "CheFSI"):
+        raise ValueError(f"Unknown method for update_psi = {update_psi}")
     solver = diag_LOBPCG_ if update_psi == "LOBPCG" else diag_davidson_
+    chefsi_rng = np.random.default_rng(seed + 1)
     e = None
     for it in range(1, NiterMax + 1):
         if it == 1:
@@ -172,7 +176,11 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
         else:
             ethr = max(ethr / 5.0, ethr_evals_last)
         tol = max(ethr, mixed_ethr_floor) if Ham.mixed else ethr
-        if batched and update_psi == "LOBPCG":
+        if update_psi == "CheFSI":
+            ev, st = diag_CheFSI_(Ham, psiks, cheby_degree, rng=chefsi_rng)
+            evals[:, :] = ev
+            ncols += st["ncols"]
+        elif batched and update_psi == "LOBPCG":
             ev, sts = diag_LOBPCG_batch(Ham, psiks, tol=tol, Nstates_conv=el.Nstates_occ)
             evals[:, :] = ev
             ncols += sum(st["ncols"] for st in sts)
