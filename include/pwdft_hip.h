@@ -192,6 +192,20 @@ int pw_mix_adaptive_dev(pw_ctx* ctx, int64_t n, double beta0, double betamax, do
  * A no-op when pw_set_symmetry has not been called or gave one operation. */
 int pw_symmetrize_rhoe_dev(pw_ctx* ctx, double* rho);
 
+/* ---- per-geometry set-up on the device (relaxation / MD rebuilds) ------------------- */
+
+/* betaNL[ik] of PsPotNL (PsPotNL.jl:16-77) built on the device for the current atoms
+ * (pw_set_atoms) and k-point ik (pw_set_kpoint's G+k): column j is projector desc[4j..4j+3]
+ * = (atom 0-based, l, m, iprj 1-based), beta = (-i)^l Ylm_real(l, m, g) eval_proj_G(l, iprj,
+ * |g|) exp(-i g.tau) (Ylm_real.jl:5-107, PsPot_GTH.jl:338-391) with rc[4 s + l] the GTH rc of
+ * species s.  Replaces pw_set_projectors (also derives beta D and the FP32 copies); nbeta
+ * must equal pw_set_dij's. */
+int pw_build_projectors_dev(pw_ctx* ctx, int ik, int nbeta, const int* desc, int nsp, const double* rc);
+/* V_Ps_loc (Hamiltonian.jl:119-141) for the current atoms from the rho sphere and V_sp(|G|)
+ * of pw_set_rho_sphere: V(r) = Re sum_G sum_a V_sp(a)(G) exp(-i G.tau_a) / Omega e^{iG.r};
+ * becomes the V_Ps_loc of pw_update_potential_dev; copied to Vps (host, N) unless NULL. */
+int pw_build_vps_loc_dev(pw_ctx* ctx, double* Vps);
+
 /* ---- forces and energy gradients (the adjoint of the pseudopotential energies) ---- */
 
 /* Atoms for the force kernels: positions tau (3 x natoms, bohr, column-major), 0-based

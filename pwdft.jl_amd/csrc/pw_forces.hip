@@ -215,7 +215,7 @@ void free_forces(pw_ctx* c) {
 }
 
 // Non-zeros of each column of D, on the device.
-static int ensure_dcsr(pw_ctx* c) {
+int ensure_dcsr(pw_ctx* c) {
   if (c->dcsr_valid) return PW_OK;
   const int nb = c->nbeta;
   std::vector<int> ptr(nb + 1, 0), rows;

@@ -270,6 +270,8 @@ void free_forces(pw_ctx* c);
 // out[f N + r] = scale Re sum_G fG[f ldf + g] e^{+i G.r} for f < nf (added to out when acc).
 int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, double* out, double scale, int acc);
 void free_scf(pw_ctx* c);
+// Non-zeros of each column of D (pw_set_dij) on the device: d_dcol_ptr / d_drow / d_dval.
+int ensure_dcsr(pw_ctx* c);
 // out[0..n) = 0 with a kernel (safe inside stream capture, unlike a memset node).
 int zero_fill(pw_ctx* c, double* out, long n);
 

@@ -94,6 +94,8 @@ _PROTOS = {
     "pw_vec_lincomb_dev": [_vp, _i64, _i, _vp, _vp, _d, _vp],
     "pw_mix_adaptive_dev": [_vp, _i64, _d, _d, _vp, _vp, _vp, _vp],
     "pw_vec_clamp_dev": [_vp, _i64, _d, _vp],
+    "pw_build_projectors_dev": [_vp, _i, _i, _vp, _i, _vp],
+    "pw_build_vps_loc_dev": [_vp, _vp],
 }
 
 EXPORTED = sorted(list(_PROTOS) + ["pw_last_error"])

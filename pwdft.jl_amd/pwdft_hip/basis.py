@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 PSP_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pseudopotentials")
-ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0}
+ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0, "Ce": 58.0}
 
 
 def gen_lattice_fcc(a):
@@ -300,6 +300,8 @@ class PsPotNL:
     block-diagonal D[ib, jb] = h[l, iprj, jprj] that op_V_Ps_nloc.jl:59-76 sums with."""
 
     def __init__(self, atoms, pw, pspots):
+        self._src = (atoms, pw, pspots, np.array(atoms.positions))
+        self._beta = None
         self.blocks = []
         nb = 0
         for ia in range(atoms.Natoms):
@@ -314,7 +316,24 @@ class PsPotNL:
         for ia, l, m, ids in self.blocks:
             h = pspots[atoms.atm2species[ia]].h[l]
             self.D[np.ix_(ids, ids)] = h[: len(ids), : len(ids)]
-        self.betaNL = []
+
+    def projector_table(self):
+        """(atom, l, m, iprj) of every beta column, iprj 1-based (pw_build_projectors_dev)."""
+        return np.array([(ia, l, m, i + 1) for ia, l, m, ids in self.blocks for i in range(len(ids))],
+                        dtype=np.int32).reshape(-1, 4)
+
+    @property
+    def betaNL(self):
+        """Host copy of the projectors for the positions this object was built with,
+        computed on first access (the Hamiltonian builds its device copy itself)."""
+        if self._beta is None:
+            self._beta = self._build_host()
+        return self._beta
+
+    def _build_host(self):
+        atoms, pw, pspots, pos = self._src
+        nb = self.NbetaNL
+        out = []
         for ik in range(pw.Nkpt):
             g = pw.gk(ik)
             Gm = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
@@ -322,7 +341,7 @@ class PsPotNL:
             ylm_cache = {}
             for ia, l, m, ids in self.blocks:
                 psp = pspots[atoms.atm2species[ia]]
-                t = atoms.positions[:, ia]
+                t = pos[:, ia]
                 GX = t[0] * g[0] + t[1] * g[1] + t[2] * g[2]
                 Sf = np.cos(GX) - 1j * np.sin(GX)
                 if (l, m) not in ylm_cache:
@@ -330,7 +349,8 @@ class PsPotNL:
                 ang = ((-1j) ** l) * ylm_cache[(l, m)] * Sf
                 for a, ib in enumerate(ids):
                     beta[:, ib] = ang * eval_proj_G(psp, l, a + 1, Gm, pw.CellVolume)
-            self.betaNL.append(beta)
+            out.append(beta)
+        return out
 
 
 def V_Ps_loc(atoms, pw, pspots):

@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC, PW_XC_PBE,
                    PW_XC_VWN, Context, DeviceMatrix, DeviceVector, ptr)
-from .basis import (PsPot_GTH, PsPotNL, PWGrid, RhoeSymmetrizer, V_Ps_loc, crystal_symmetry,
+from .basis import (PsPot_GTH, PsPotNL, PWGrid, RhoeSymmetrizer, crystal_symmetry,
                     eval_Vloc_G, kpoint_grid, kpoint_rotations)
 from .potentials import calc_E_NN, guess_rhoe
 
@@ -131,11 +131,10 @@ class Hamiltonian:
         self.ctx = Context(device)
         self._rho_scratch = None
         self.potentials = Potentials(self.ctx, self.pw.Npoints)
-        self.potentials.Ps_loc = V_Ps_loc(atoms, self.pw, self.pspots)
-        self.potentials.Total[:, 0] = self.potentials.Ps_loc
         self.E_NN = calc_E_NN(atoms, self.Zvals)
         self.ctx.call("pw_set_band_batch", int(band_batch))
-        self._upload_static()
+        self._upload_static()          # also builds V_Ps_loc and betaNL on the device
+        self.potentials.Total[:, 0] = self.potentials.Ps_loc
         self._upload_vloc()
 
     # ---- device state ---------------------------------------------------------------
@@ -150,9 +149,6 @@ class Hamiltonian:
             idx = np.ascontiguousarray(pw.idx_gw2r[ik], dtype=np.int64)
             gk = np.ascontiguousarray(pw.gk(ik).T, dtype=float)  # ngw x 3 row-major == 3 x ngw col-major
             c.call("pw_set_kpoint", ik, int(pw.Ngw[ik]), ptr(idx), 0, ptr(gk))
-            if nb > 0:
-                b = np.asfortranarray(self.pspotNL.betaNL[ik])
-                c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
         # rho sphere and V_sp(|G|) of every species for the force / energy-gradient kernels
         vg = np.empty((pw.Ng, self.atoms.Nspecies), order="F")
         for isp, psp in enumerate(self.pspots):
@@ -162,8 +158,8 @@ class Hamiltonian:
         c.call("pw_set_rho_sphere", int(pw.Ng), ptr(idx), 0, ptr(G), self.atoms.Nspecies, ptr(vg))
         if self.rhoe_symmetrizer is not None:
             self.rhoe_symmetrizer.upload(c)
-        c.call("pw_set_vps_loc", ptr(np.ascontiguousarray(self.potentials.Ps_loc)))
         self._upload_atoms()
+        self._build_geometry()
 
     def _upload_atoms(self):
         atoms = self.atoms
@@ -173,10 +169,24 @@ class Hamiltonian:
         self.ctx.call("pw_set_atoms", atoms.Natoms, ptr(tau), ptr(sp), len(ba),
                       ptr(ba) if len(ba) else None)
 
+    def _build_geometry(self):
+        """Per-geometry set-up on the device for the current positions: betaNL of every
+        k-point (PsPotNL.jl:16-77) and V_Ps_loc (Hamiltonian.jl:119-141), whose host copy
+        becomes potentials.Ps_loc."""
+        c, nl = self.ctx, self.pspotNL
+        if nl.NbetaNL > 0:
+            desc = np.ascontiguousarray(nl.projector_table())
+            rc = np.ascontiguousarray([p.rc[:4] for p in self.pspots], dtype=float)
+            for ik in range(self.pw.Nkpt):
+                c.call("pw_build_projectors_dev", ik, nl.NbetaNL, ptr(desc), len(self.pspots), ptr(rc))
+        V = np.empty(self.pw.Npoints)
+        c.call("pw_build_vps_loc_dev", ptr(V))
+        self.potentials.Ps_loc = V
+
     def set_positions(self, positions):
         """Move the atoms (Cartesian bohr, 3 x Natoms): rebuild the structure-factor
-        dependent set-up (betaNL, V_Ps_loc, E_NN) on the host as Hamiltonian() does and
-        upload it; V_loc is rebuilt from V_Ps_loc + the current Hartree and XC parts."""
+        dependent set-up as Hamiltonian() does -- betaNL and V_Ps_loc on the device, E_NN on
+        the host; V_loc is rebuilt from V_Ps_loc + the current Hartree and XC parts."""
         atoms = self.atoms
         old = atoms.positions
         atoms.positions = np.array(positions, dtype=float).reshape(3, atoms.Natoms)
@@ -192,14 +202,8 @@ class Hamiltonian:
             self.rhoe_symmetrizer.upload(self.ctx)
         pw, c = self.pw, self.ctx
         self.pspotNL = PsPotNL(atoms, pw, self.pspots)
-        nb = self.pspotNL.NbetaNL
-        for ik in range(pw.Nkpt):
-            if nb > 0:
-                b = np.asfortranarray(self.pspotNL.betaNL[ik])
-                c.call("pw_set_projectors", ik, nb, ptr(b), int(pw.Ngw[ik]))
         self._upload_atoms()
-        self.potentials.Ps_loc = V_Ps_loc(atoms, pw, self.pspots)
-        c.call("pw_set_vps_loc", ptr(np.ascontiguousarray(self.potentials.Ps_loc)))
+        self._build_geometry()
         self.potentials.Total[:, 0] = self.potentials.Ps_loc + self.potentials.Hartree + self.potentials.XC[:, 0]
         self.E_NN = calc_E_NN(atoms, self.Zvals)
         self._upload_vloc()

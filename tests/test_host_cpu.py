@@ -67,6 +67,26 @@ def test_projectors_and_local_potential_identical():
     assert abs(P.calc_E_NN(at, [4.0]) - calc_E_NN(oat.LatVecs, oat, np.array([4.0]))) < 1e-12
 
 
+def test_host_projectors_d_f_channels_match_oracle():
+    """The host projector builder (kept for the lazy PsPotNL.betaNL) against the oracle for
+    a species with d and f channels (Ce GTH-PADE-q12, l = 0..3)."""
+    L = B.gen_lattice_fcc(A_SI)
+    ko, wko = S.gen_kpoint_grid(S.gen_lattice_fcc(A_SI), (2, 2, 2), (0, 0, 0))
+    pw = B.PWGrid(12.0, L, kpoints=(ko, wko))
+    opw = S.PWGrid(12.0, S.gen_lattice_fcc(A_SI), kpoints=(ko, wko))
+    at = B.Atoms(["Ce", "Si"], [[0, 0, 0], [0.25, 0.25, 0.25]], L)
+    oat = S.Atoms(["Ce", "Si"], [[0, 0, 0], [0.25, 0.25, 0.25]], S.gen_lattice_fcc(A_SI))
+    files = ["Ce-q12.lda.gth", "Si-q4.lda.gth"]
+    psp = [B.PsPot_GTH(f) for f in files]
+    opsp = [S.PsPot_GTH(os.path.join(ROOT, "pwdft.jl_amd", "pwdft_hip", "pseudopotentials", f)) for f in files]
+    assert psp[0].lmax == 3
+    nl, onl = B.PsPotNL(at, pw, psp), S.PsPotNL(oat, opw, opsp)
+    assert np.array_equal(nl.D, onl.D)
+    for ik in range(pw.Nkpt):
+        assert np.abs(nl.betaNL[ik] - onl.betaNL[ik]).max() < 1e-13
+    assert len(nl.projector_table()) == nl.NbetaNL
+
+
 def test_electrons_counts():
     from pwdft_hip.hamiltonian import Electrons
     e = Electrons(8.0, 4, extra_states=4)
