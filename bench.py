@@ -245,6 +245,9 @@ def main():
     ap.add_argument("--scf-iters", type=int, default=3, help="SCF iterations timed per precision (0 = skip)")
     ap.add_argument("--no-batched-k", dest="batched_k", action="store_false",
                     help="skip the 182-k-point batched op_H leg")
+    ap.add_argument("--no-overlap", dest="overlap", action="store_false",
+                    help="run op_H's nonlocal products on the main stream (kernels in sequence), "
+                         "e.g. under a kernel-trace profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported"}))
@@ -266,7 +269,14 @@ def main():
     res = {}
     from pwdft_hip.solvers import _Dense
     for mixed in (False, True):
-        sec, wall, prof = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=True)
+        # headline: nonlocal products on the second stream beside the FFT passes, no
+        # per-kernel events; then the per-kernel durations for the roofline with the
+        # kernels in sequence on one stream (overlapped kernels share the CUs)
+        H.ctx.call("pw_tune_set", 6, 1 if args.overlap else 0)
+        sec, wall, _ = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=False)
+        H.ctx.call("pw_tune_set", 6, 0)
+        _, _, prof = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=True)
+        H.ctx.call("pw_tune_set", 6, 1 if args.overlap else 0)
         D = _Dense(H, False)
         e = np.real(np.diag(D.gram(X.p, Y.p, NBANDS, NBANDS, X.rows)))
         res["mixed" if mixed else "fp64"] = {"sec": sec, "wall": wall, "eband": e, "prof": prof}
@@ -315,6 +325,8 @@ def main():
         "fp64": {"value": NBANDS / t_64, "ms_per_step": t_64 * 1e3},
         "mixed_over_fp64": t_64 / t_mix,
         "dE_per_atom_mixed_vs_fp64": dE,
+        "op_H_streams": "nonlocal products on a second stream beside the FFT passes" if args.overlap
+                        else "one stream",
         "roofline": roofline(res["mixed"]["prof"], H, True),
         "roofline_fp64": roofline(res["fp64"]["prof"], H, False),
         "lobpcg": lob or None,

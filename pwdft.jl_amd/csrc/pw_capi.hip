@@ -132,6 +132,43 @@ static int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>*
   if (nb == 0) return PW_OK;
   const unsigned terms = flags & PW_TERM_ALL;
   bool have = (flags & PW_ACCUMULATE) != 0;
+  if ((terms & PW_TERM_VLOC) && (terms & PW_TERM_NLOC) && k.nbeta > 0 && c->overlap && c->stream2) {
+    // The nonlocal products (MFMA) run on stream2 beside the column / plane FFT passes
+    // (VALU / LDS): stream2 writes Hpsi = [Hpsi +] (beta D) (beta^H psi) first, and the gather
+    // pass, which waits for it, adds the kinetic and local terms.
+    if (ispin < 0 || ispin >= c->nspin || !c->d_v64[ispin])
+      return set_error(c, PW_ERR_STATE, "op_V_loc: potential of spin " + std::to_string(ispin) + " not set");
+    rc = ensure(c, &c->d_bnl, &c->bnl_bytes, (size_t)k.nbeta * nb * sizeof(cplx<double>));
+    if (rc) return rc;
+    hipError_t e = hipEventRecord(c->ev_fork, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+    if (e != hipSuccess) return hip_check(c, e, "stream fork");
+    cplx<double>* bnl = reinterpret_cast<cplx<double>*>(c->d_bnl);
+    const cplx<double> one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+    hipStream_t main = c->stream;
+    c->stream = c->stream2;  // the GEMM drivers launch on c->stream
+    if (flags & PW_MIXED) {
+      rc = zgemm_f32in(c, 1, k.nbeta, nb, k.ngw, one, reinterpret_cast<const cplx<float>*>(k.d_beta32), k.ngw,
+                       psi, ldpsi, zero, bnl, k.nbeta, 1);
+      if (!rc)
+        rc = zgemm_f32in(c, 0, k.ngw, nb, k.nbeta, one, reinterpret_cast<const cplx<float>*>(k.d_betaD32), k.ngw,
+                         bnl, k.nbeta, have ? one : zero, hpsi, ldh, 1);
+    } else {
+      rc = zgemm(c, 1, k.nbeta, nb, k.ngw, one, reinterpret_cast<const cplx<double>*>(k.d_beta64), k.ngw, psi,
+                 ldpsi, zero, bnl, k.nbeta, 0);
+      if (!rc)
+        rc = zgemm(c, 0, k.ngw, nb, k.nbeta, one, reinterpret_cast<const cplx<double>*>(k.d_betaD64), k.ngw, bnl,
+                   k.nbeta, have ? one : zero, hpsi, ldh, 0);
+    }
+    e = hipEventRecord(c->ev_join, c->stream2);
+    c->stream = main;
+    if (!rc && e != hipSuccess) rc = hip_check(c, e, "stream join record");
+    if (rc) {
+      hipStreamWaitEvent(c->stream, c->ev_join, 0);  // keep the streams ordered on failure
+      return rc;
+    }
+    return vloc_apply(c, ik, ispin, nb, psi, ldpsi, hpsi, ldh, flags | PW_ACCUMULATE, c->ev_join);
+  }
   if (terms & PW_TERM_VLOC) {
     if (ispin < 0 || ispin >= c->nspin || !c->d_v64[ispin])
       return set_error(c, PW_ERR_STATE, "op_V_loc: potential of spin " + std::to_string(ispin) + " not set");
@@ -235,6 +272,9 @@ int pw_ctx_create(pw_ctx** out, int device) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return PW_ERR_HIP;
@@ -272,6 +312,9 @@ int pw_ctx_destroy(pw_ctx* c) {
   for (void* p : c->retired) hipFree(p);
   drain_events(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->stream2) hipStreamDestroy(c->stream2);
   hipStreamDestroy(c->stream);
   delete c;
   return PW_OK;
@@ -303,6 +346,9 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       return pw_set_band_batch(c, value);
     case PW_TUNE_PLANE_COMPRESSED:
       c->plane_compressed = value != 0;
+      return PW_OK;
+    case PW_TUNE_OVERLAP:
+      c->overlap = value != 0;
       return PW_OK;
     default:
       return set_error(c, PW_ERR_ARG, "pw_tune_set: unknown key");

@@ -845,7 +845,7 @@ static int plane_c_rows(const pw_ctx* c, size_t esz, int nxact, int nthreads, si
 
 template <typename T>
 static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>* psi, long ldpsi,
-                    cplx<double>* hpsi, long ldh, int kin, int acc) {
+                    cplx<double>* hpsi, long ldh, int kin, int acc, hipEvent_t join = nullptr) {
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
   size_t plane_bytes = plane_lds(c, esz, k.nxact);
@@ -886,6 +886,10 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
         hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
                            k.d_col_ij, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     }
+    if (join && b0 == 0) {
+      hipError_t e = hipStreamWaitEvent(c->stream, join, 0);
+      if (e != hipSuccess) return hip_check(c, e, "stream join");
+    }
     {
       ProfScope ps(c, PW_PROF_COL_FFT, wb + psib * ((kin ? 1 : 0) + (acc ? 1 : 0) + 1) + k.ngw * 16.0);
       hipLaunchKernelGGL(k_col_fft<T>, g1, dim3(COL_NT), col_lds(c, C, esz), c->stream, W, psi, ldpsi,
@@ -897,11 +901,12 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
 }
 
 int vloc_apply(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
-               cplx<double>* hpsi, long ldh, unsigned flags) {
+               cplx<double>* hpsi, long ldh, unsigned flags, hipEvent_t join) {
   const int kin = (flags & PW_TERM_K) ? 1 : 0;
   const int acc = (flags & PW_ACCUMULATE) ? 1 : 0;
-  if (flags & PW_MIXED) return vloc_run<float>(c, c->kd[ik], c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
-  return vloc_run<double>(c, c->kd[ik], c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc);
+  if (flags & PW_MIXED)
+    return vloc_run<float>(c, c->kd[ik], c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc, join);
+  return vloc_run<double>(c, c->kd[ik], c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, kin, acc, join);
 }
 
 int vloc_apply_k(pw_ctx* c, KData& k, const double* V64, int nb, const cplx<double>* psi, long ldpsi,

@@ -146,6 +146,11 @@ struct EBatch {
 struct pw_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream for the nonlocal products of op_H, which run beside the FFT passes;
+  // forked from and joined back into `stream` with the two events (default priority)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;
   int n1 = 0, n2 = 0, n3 = 0;
   long N = 0;
   double omega = 0.0;
@@ -248,8 +253,10 @@ int hip_check(pw_ctx* c, hipError_t e, const char* what);
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need);
 
 // FFT drivers (pw_fft.hip)
+// join: when set, the stream waits for this event before the first gather pass (the event
+// marks work on another stream that writes hpsi first).
 int vloc_apply(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
-               cplx<double>* hpsi, long ldh, unsigned flags);
+               cplx<double>* hpsi, long ldh, unsigned flags, hipEvent_t join = nullptr);
 int kinetic_apply(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,
                   long ldh, unsigned flags);
 int rhoe_accum(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldpsi, const double* d_w,

@@ -189,3 +189,40 @@ def test_other_grids(case):
     H.mixed = True
     assert rel(ph.calc_rhoe(H, psiks, Focc=Focc)[:, 0], ref) < TOLMIX
     H.mixed = False
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_overlapped_nonlocal_stream(mixed):
+    """op_H with the nonlocal products on the second stream (default) against the
+    single-stream order, with and without accumulation, over several band batches; and a
+    captured op_H (fork and join inside the graph) replayed against the eager result."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL
+    atoms = ph.Atoms(["Si"] * 8, ph.diamond_supercell(1.0, 1), ph.gen_lattice_sc(10.2631))
+    H = ph.Hamiltonian(atoms, ["Si-q4.lda.gth"], 20.0, use_symmetry=False, band_batch=7)
+    H.update(H.guess_rhoe())
+    rng = np.random.default_rng(17)
+    n = H.pw.Ngw[0]
+    psi = ph.DeviceMatrix.from_host(H.ctx, rand_psi(rng, n, 20))
+    h0 = rand_psi(rng, n, 20)
+    flags = PW_TERM_ALL | (PW_MIXED if mixed else 0)
+    res = {}
+    for ov in (0, 1):
+        H.ctx.call("pw_tune_set", 6, ov)
+        for acc in (0, PW_ACCUMULATE):
+            out = ph.DeviceMatrix.from_host(H.ctx, h0)
+            H.ctx.call("pw_op_H_dev", 0, 0, 20, psi.p, n, out.p, n, flags | acc)
+            res[(ov, acc)] = out.download()
+    for acc in (0, PW_ACCUMULATE):
+        assert rel(res[(1, acc)], res[(0, acc)]) < 1e-13, acc
+    out = ph.DeviceMatrix(H.ctx, n, 20)
+    H.ctx.call("pw_op_H_dev", 0, 0, 20, psi.p, n, out.p, n, flags)   # warm workspaces
+    H.ctx.call("pw_synchronize")
+    with ph.Graph(H.ctx) as g:
+        H.ctx.call("pw_op_H_dev", 0, 0, 20, psi.p, n, out.p, n, flags)
+    for rep in range(3):
+        out.upload(np.full((n, 20), np.nan + 0j))
+        g.launch()
+        H.ctx.call("pw_synchronize")
+        assert np.array_equal(out.download(), res[(1, 0)]), rep
+    g.free()
