@@ -132,7 +132,10 @@ static int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>*
   if (nb == 0) return PW_OK;
   const unsigned terms = flags & PW_TERM_ALL;
   bool have = (flags & PW_ACCUMULATE) != 0;
-  if ((terms & PW_TERM_VLOC) && (terms & PW_TERM_NLOC) && k.nbeta > 0 && c->overlap && c->stream2) {
+  // (small problems -- a few bands of a small sphere -- keep one stream: the fork and join
+  // would cost more than the overlap saves)
+  if ((terms & PW_TERM_VLOC) && (terms & PW_TERM_NLOC) && k.nbeta > 0 && c->overlap && c->stream2 &&
+      (long)k.ngw * nb >= (1L << 18)) {
     // The nonlocal products (MFMA) run on stream2 beside the column / plane FFT passes
     // (VALU / LDS): stream2 writes Hpsi = [Hpsi +] (beta D) (beta^H psi) first, and the gather
     // pass, which waits for it, adds the kinetic and local terms.
