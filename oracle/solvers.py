@@ -244,3 +244,75 @@ def diag_CheFSI(opH, X, degree, v0):
     Y = ortho_sqrt(chebyfilt(opH, X, degree, lb, ub))
     Hr = Y.conj().T @ opH(Y)
     return np.linalg.eigvalsh(herm_u(Hr)), Y
+
+
+# ---------------------------------------------------------------------------- PCG
+
+
+def calc_grad_evals(opH, psi):
+    """calc_grad.jl:60-82: H psi - psi (psi' H psi) (the gradient of the band-energy sum)."""
+    Hpsi = opH(psi)
+    return Hpsi - psi @ (psi.conj().T @ Hpsi)
+
+
+def diag_Emin_PCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0, tol_ebands=1e-4, alpha_t=3e-5,
+                  I_CG_BETA=2):
+    """diag_Emin_PCG! (diag_Emin_PCG.jl:56-195): all-band preconditioned conjugate gradient
+    on the band-energy sum, trial step alpha_t and the line-minimising step from the gradient
+    at the trial point; convergence on |d evals| < tol for Nstates_conv bands or on the
+    band-energy change.  X (orthonormal) is overwritten with the Ritz vectors; returns
+    (evals, iterations, number of opH columns applied)."""
+    Nstates = X.shape[1]
+    if Nstates_conv == 0:
+        Nstates_conv = Nstates
+    rdot = lambda a, b: float(np.real(np.sum(np.conj(a) * b)))
+    ncols = 0
+    g_old = d_old = Kg_old = d = None
+    beta = 0.0
+    Ebands_old = 0.0
+    Hr = herm_u(X.conj().T @ opH(X))
+    ncols += Nstates
+    evals = np.linalg.eigvalsh(Hr)
+    evals_old = evals.copy()
+    it_done = 0
+    for it in range(1, NiterMax + 1):
+        it_done = it
+        g = calc_grad_evals(opH, X)
+        Kg = prec(g)
+        ncols += Nstates
+        if it != 1:
+            if I_CG_BETA == 1:
+                beta = rdot(g, Kg) / rdot(g_old, Kg_old)
+            elif I_CG_BETA == 2:
+                beta = rdot(g - g_old, Kg) / rdot(g_old, Kg_old)
+            elif I_CG_BETA == 3:
+                beta = rdot(g - g_old, Kg) / rdot(g - g_old, d)
+            else:
+                beta = rdot(g, Kg) / float(np.real(np.sum((g - g_old) * np.conj(d_old))))
+        if beta < 0.0:
+            beta = 0.0
+        d = -Kg + beta * d_old if it != 1 else -Kg + beta * np.zeros_like(Kg)
+        Xc = ortho_sqrt(X + alpha_t * d)
+        gt = calc_grad_evals(opH, Xc)
+        ncols += Nstates
+        denum = rdot(g - gt, d)
+        alpha = abs(alpha_t * rdot(g, d) / denum) if denum != 0.0 else 0.0
+        X[:] = ortho_sqrt(X + alpha * d)
+        Hr = herm_u(X.conj().T @ opH(X))
+        ncols += Nstates
+        evals = np.linalg.eigvalsh(Hr)
+        Ebands = float(np.sum(evals))
+        devals = np.abs(evals - evals_old)
+        evals_old = evals.copy()
+        nconv = int(np.sum(devals < tol))
+        diffE = abs(Ebands - Ebands_old)
+        if nconv >= Nstates_conv or diffE <= tol_ebands * Nstates:
+            break
+        g_old, d_old, Kg_old = g.copy(), d.copy(), Kg.copy()
+        Ebands_old = Ebands
+    X[:] = ortho_sqrt(X)
+    Hr = herm_u(X.conj().T @ opH(X))
+    ncols += Nstates
+    evals, evecs = np.linalg.eigh(Hr)
+    X[:] = X @ evecs
+    return evals, it_done, ncols

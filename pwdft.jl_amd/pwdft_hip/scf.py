@@ -15,6 +15,7 @@ from .hamiltonian import calc_rhoe_
 from .batch import diag_LOBPCG_batch
 from .mixing import DensityMixer
 from .chefsi import diag_CheFSI_
+from .pcg import diag_Emin_PCG_
 from .solvers import diag_LOBPCG_, diag_davidson_, rand_BlochWavefunc
 
 EPS = np.finfo(float).eps
@@ -163,9 +164,9 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
     history = []
     ncols = 0
     t0 = time.perf_counter()
-    if update_psi not in ("LOBPCG", "davidson", "CheFSI"):
+    if update_psi not in ("LOBPCG", "davidson", "PCG", "CheFSI"):
         raise ValueError(f"Unknown method for update_psi = {update_psi}")
-    solver = diag_LOBPCG_ if update_psi == "LOBPCG" else diag_davidson_
+    solver = {"LOBPCG": diag_LOBPCG_, "davidson": diag_davidson_, "PCG": diag_Emin_PCG_}.get(update_psi)
     chefsi_rng = np.random.default_rng(seed + 1)
     e = None
     for it in range(1, NiterMax + 1):

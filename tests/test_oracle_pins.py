@@ -12,7 +12,7 @@ from oracle import ops as O
 from oracle import setup as S
 from oracle.density import calc_epsxc_Vxc_VWN, calc_rhoe, Poisson_solve
 from oracle.energy import calc_E_NN, Hamiltonian, KS_solve_SCF
-from oracle.solvers import diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
+from oracle.solvers import diag_Emin_PCG, diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
 
 
 def test_vwn_table():
@@ -200,6 +200,31 @@ def test_free_electron(solver):
         X = rand_wavefunc(rng, H.pw.Ngw[ik], 8)
         ev, _, _ = solver(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-12, NiterMax=300)
         assert np.abs(ev - np.sort(0.5 * H.pw.gk2(ik))[:8]).max() < 1e-9
+
+
+@pytest.mark.parametrize("cg_beta", [1, 2, 3, 4])
+def test_pcg_free_electron_and_dense(si_small, cg_beta):
+    """diag_Emin_PCG! with each conjugate-direction formula converges to the free-electron
+    closed form (V = 0, no projectors) and to dense eigh of the full H."""
+    sym, frac = si_fcc_structure()
+    H = oracle_ham(sym, frac, S.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 8.0, meshk=(2, 2, 2), extra_states=4)
+    H.V_loc[:] = 0.0
+    H.pspotNL.NbetaNL = 0
+    H.ik = 1
+    X = rand_wavefunc(np.random.default_rng(6), H.pw.Ngw[1], 8)
+    ev, _, _ = diag_Emin_PCG(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-11, NiterMax=2000,
+                             tol_ebands=0.0, I_CG_BETA=cg_beta)
+    assert np.abs(ev - np.sort(0.5 * H.pw.gk2(1))[:8]).max() < 1e-8
+    H = si_small
+    H.ik = 1
+    n = H.pw.Ngw[1]
+    Hd = O.op_H(H, np.eye(n, dtype=complex))
+    exact = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))[:8]
+    X = rand_wavefunc(np.random.default_rng(7), n, 8)
+    ev, _, _ = diag_Emin_PCG(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-11, NiterMax=2000,
+                             tol_ebands=0.0, I_CG_BETA=cg_beta)
+    assert np.abs(ev - exact).max() < 1e-8
+    assert np.allclose(X.conj().T @ X, np.eye(8), atol=1e-10)
 
 
 def test_lobpcg_vs_dense(si_small):
