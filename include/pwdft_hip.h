@@ -167,6 +167,11 @@ int pw_set_symmetry(pw_ctx* ctx, int nsym, const int* M, const double* t, int ng
 int pw_update_potential_dev(pw_ctx* ctx, int ispin, int xc, int install, const double* rho, double* VH,
                             double* Vxc, double* Vtot, double* E);
 
+/* V_Total = V_Ps_loc + V_H + V_xc from device arrays (N doubles), installed as the op_H
+ * potential of ispin, for drivers that mix the potentials themselves
+ * (KS_solve_SCF_potmix.jl:236-266); Vtot (device, may be NULL) receives V_Total. */
+int pw_set_vloc_parts_dev(pw_ctx* ctx, int ispin, const double* VH, const double* Vxc, double* Vtot);
+
 /* Simple mixing of KS_solve_SCF! (KS_solve_SCF.jl:240-281): rho_out = beta rho_new +
  * (1 - beta) rho_in, then rho_out = max(rho_out, eps(Float64)); device arrays, rho_out may
  * alias either input. */

@@ -606,6 +606,24 @@ int pw_update_potential_dev(pw_ctx* c, int ispin, int xc, int install, const dou
   return update_potential(c, ispin, xc, install, rho, VH, Vxc, Vtot, E);
 }
 
+int pw_set_vloc_parts_dev(pw_ctx* c, int ispin, const double* VH, const double* Vxc, double* Vtot) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
+  if (!c->d_vps) return set_error(c, PW_ERR_STATE, "pw_set_vloc_parts_dev: V_Ps_loc has not been set");
+  if (ispin < 0 || ispin > 3 || !VH || !Vxc) return set_error(c, PW_ERR_ARG, "pw_set_vloc_parts_dev: bad arguments");
+  int rc = ensure_vloc(c, ispin);
+  if (rc) return rc;
+  if (!Vtot) {
+    rc = ensure(c, &c->d_sgrid, &c->sgrid_bytes, (size_t)SG_COUNT * c->N * sizeof(double));
+    if (rc) return rc;
+    Vtot = reinterpret_cast<double*>(c->d_sgrid) + (long)SG_VTOT * c->N;
+  }
+  hipLaunchKernelGGL(k_vtot, dim3(grid_blocks(c->N)), dim3(SNT), 0, c->stream, c->d_vps, VH, Vxc, Vtot, c->d_v64[ispin],
+                     c->d_v32[ispin], c->N, 1.0 / (double)c->N);
+  return hip_check(c, hipGetLastError(), "k_vtot");
+}
+
 int pw_mix_rhoe_dev(pw_ctx* c, double beta, const double* rho_new, const double* rho_in, double* rho_out) {
   CTX_CHECK(c);
   hipSetDevice(c->device);

@@ -221,6 +221,14 @@ class Hamiltonian:
     def xc_code(self):
         return PW_XC_PBE if self.xcfunc == "PBE" else PW_XC_VWN
 
+    def install_potentials(self):
+        """V_Total = V_Ps_loc + Hartree + XC from the device arrays becomes the op_H
+        potential (after the Hartree / XC arrays were changed on the device, as potential
+        mixing does)."""
+        p = self.potentials
+        self.ctx.call("pw_set_vloc_parts_dev", 0, p.dev["Hartree"].p, p.dev["XC"].p, p.dev["Total"].p)
+        p.device_updated()
+
     def rho_scratch(self):
         """A device vector of Npoints doubles owned by the Hamiltonian (staging of host
         densities for the device entry points)."""

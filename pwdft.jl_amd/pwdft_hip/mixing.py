@@ -29,13 +29,14 @@ def _arr(vs):
 class DensityMixer:
     """State of one mix_method for densities of n points on the context of Ham."""
 
-    def __init__(self, Ham, method="simple", betamix=0.2, mixdim=5):
+    def __init__(self, Ham, method="simple", betamix=0.2, mixdim=5, clamp=True):
         if method not in MIX_METHODS:
             raise ValueError(f"Unknown mix_method = {method}")
         if method == "broyden" and mixdim > 8:
             raise ValueError("mix_broyden: mixdim must not exceed 8 (maxter of mix_broyden.jl)")
         self.ctx, self.n = Ham.ctx, Ham.pw.Npoints
         self.method, self.beta, self.mixdim = method, float(betamix), int(mixdim)
+        self.clamp = clamp     # densities are clamped at eps() after mixing, potentials are not
         new = lambda: self._zeros(DeviceVector(self.ctx, self.n))
         self.t = [DeviceVector(self.ctx, self.n) for _ in range(3)]
         if method in ("anderson", "broyden"):
@@ -105,7 +106,8 @@ class DensityMixer:
             self._rpulay(rho, rho_new, it)
         else:
             self._anderson(rho, rho_new, it)
-        self._clamp(rho)
+        if self.clamp:
+            self._clamp(rho)
 
     def _broyden(self, deltain, deltaout_, it):
         """mix_broyden.jl:4-70."""

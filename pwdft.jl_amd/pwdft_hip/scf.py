@@ -133,6 +133,28 @@ def calc_entropy(wk, kT, evals, Ef):
     return float(np.sum(2 * wk[None, :] * kT * w1))
 
 
+def _solve_bands(Ham, psiks, update_psi, tol, batched, cheby_degree, chefsi_rng, evals):
+    """The update_psi dispatch of KS_solve_SCF.jl:187-212 over all k-points; fills evals
+    (Nstates x Nkpt) and returns the number of op_H columns applied."""
+    el = Ham.electrons
+    ncols = 0
+    if update_psi == "CheFSI":
+        ev, st = diag_CheFSI_(Ham, psiks, cheby_degree, rng=chefsi_rng)
+        evals[:, :] = ev
+        return st["ncols"]
+    if batched and update_psi == "LOBPCG":
+        ev, sts = diag_LOBPCG_batch(Ham, psiks, tol=tol, Nstates_conv=el.Nstates_occ)
+        evals[:, :] = ev
+        return sum(st["ncols"] for st in sts)
+    solver = {"LOBPCG": diag_LOBPCG_, "davidson": diag_davidson_, "PCG": diag_Emin_PCG_}[update_psi]
+    for ik in range(Ham.pw.Nkpt):
+        Ham.ik, Ham.ispin = ik, 0
+        ev, st = solver(Ham, psiks[ik], tol=tol, Nstates_conv=el.Nstates_occ)
+        evals[:, ik] = ev
+        ncols += st["ncols"]
+    return ncols
+
+
 def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6,
                   ethr_evals_last=1e-5, update_psi="LOBPCG", use_smearing=False, kT=1e-3,
                   seed=1234, verbose=False, mixed_until=None, batched=False, mixed_ethr_floor=1e-6,
@@ -166,7 +188,6 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
     t0 = time.perf_counter()
     if update_psi not in ("LOBPCG", "davidson", "PCG", "CheFSI"):
         raise ValueError(f"Unknown method for update_psi = {update_psi}")
-    solver = {"LOBPCG": diag_LOBPCG_, "davidson": diag_davidson_, "PCG": diag_Emin_PCG_}.get(update_psi)
     chefsi_rng = np.random.default_rng(seed + 1)
     e = None
     for it in range(1, NiterMax + 1):
@@ -177,20 +198,7 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
         else:
             ethr = max(ethr / 5.0, ethr_evals_last)
         tol = max(ethr, mixed_ethr_floor) if Ham.mixed else ethr
-        if update_psi == "CheFSI":
-            ev, st = diag_CheFSI_(Ham, psiks, cheby_degree, rng=chefsi_rng)
-            evals[:, :] = ev
-            ncols += st["ncols"]
-        elif batched and update_psi == "LOBPCG":
-            ev, sts = diag_LOBPCG_batch(Ham, psiks, tol=tol, Nstates_conv=el.Nstates_occ)
-            evals[:, :] = ev
-            ncols += sum(st["ncols"] for st in sts)
-        else:
-            for ik in range(pw.Nkpt):
-                Ham.ik, Ham.ispin = ik, 0
-                ev, st = solver(Ham, psiks[ik], tol=tol, Nstates_conv=el.Nstates_occ)
-                evals[:, ik] = ev
-                ncols += st["ncols"]
+        ncols += _solve_bands(Ham, psiks, update_psi, tol, batched, cheby_degree, chefsi_rng, evals)
         if use_smearing:
             Focc, Ef = calc_Focc(el.Nelectrons, pw.wk, kT, evals)
             mTS = calc_entropy(pw.wk, kT, evals, Ef)
@@ -214,6 +222,74 @@ def KS_solve_SCF_(Ham, psiks=None, betamix=0.2, NiterMax=100, etot_conv_thr=1e-6
         if nconv >= 2:
             break
         Etot_old = Etot
+    el.ebands = evals
+    Ham.energies = e
+    return {"energies": e.as_dict(), "evals": evals, "psiks": psiks, "rho": rho.download(), "history": history,
+            "ncols": ncols, "seconds": time.perf_counter() - t0, "iterations": len(history)}
+
+
+def KS_solve_SCF_potmix_(Ham, psiks=None, NiterMax=150, betamix=0.2, use_smearing=False, mix_method="simple",
+                         mixdim=5, kT=1e-3, update_psi="LOBPCG", cheby_degree=8, etot_conv_thr=1e-6,
+                         ethr_evals_last=1e-5, seed=1234, verbose=False, batched=False):
+    """KS_solve_SCF_potmix! (KS_solve_SCF_potmix.jl:16-309), Nspin = 1: the Hartree and XC
+    potentials are mixed separately (simple, linear_adaptive or broyden) after each
+    update! of the new density, then V_Total is rebuilt; energies are those of the new
+    density.  All grid fields stay on the device.  With use_smearing the entropy term of
+    calc_entropy is used (the reference's driver refers to an undefined `Entropy` there)."""
+    if mix_method not in ("simple", "linear_adaptive", "broyden"):
+        raise ValueError(f"KS_solve_SCF_potmix: mix_method {mix_method} is not implemented there")
+    if update_psi not in ("LOBPCG", "davidson", "PCG", "CheFSI"):
+        raise ValueError(f"Unknown method for update_psi = {update_psi}")
+    pw, el, ctx = Ham.pw, Ham.electrons, Ham.ctx
+    if psiks is None:
+        psiks = rand_BlochWavefunc(Ham, seed=seed)
+    rho = DeviceVector.from_host(ctx, Ham.guess_rhoe())
+    VH_inp = DeviceVector(ctx, pw.Npoints)
+    Vxc_inp = DeviceVector(ctx, pw.Npoints)
+    mixH = DensityMixer(Ham, mix_method, betamix, mixdim, clamp=False)
+    mixX = DensityMixer(Ham, mix_method, betamix, mixdim, clamp=False)
+    Ham.update(rho)
+    Focc = el.Focc.copy()
+    evals = np.zeros((el.Nstates, pw.Nkpt))
+    chefsi_rng = np.random.default_rng(seed + 1)
+    Etot_old, nconv, ethr, mTS, ncols = 0.0, 0, 0.1, 0.0, 0
+    history = []
+    e = None
+    t0 = time.perf_counter()
+    pot = Ham.potentials.dev
+    for it in range(1, NiterMax + 1):
+        ethr = 0.1 if it == 1 else 0.01 if it == 2 else max(ethr / 5.0, ethr_evals_last)
+        ncols += _solve_bands(Ham, psiks, update_psi, ethr, batched, cheby_degree, chefsi_rng, evals)
+        if use_smearing:
+            Focc, Ef = calc_Focc(el.Nelectrons, pw.wk, kT, evals)
+            mTS = calc_entropy(pw.wk, kT, evals, Ef)
+            el.E_fermi = Ef
+            el.Focc = Focc.copy()
+        calc_rhoe_(Ham, psiks, rho, Focc=Focc)
+        ctx.call("pw_vec_clamp_dev", pw.Npoints, float(EPS), rho.p)
+        Vxc_inp.copy_from(pot["XC"])
+        VH_inp.copy_from(pot["Hartree"])
+        E_local = Ham.update(rho)
+        e = calc_energies(Ham, psiks, rho, Focc, mTS, E_local=E_local)
+        Etot = e.total()
+        diffE = abs(Etot - Etot_old)
+        history.append(Etot)
+        if verbose:
+            print(f"SCF_potmix: {it:5d} {Etot:18.10f} {diffE:12.5e}", flush=True)
+        nconv = nconv + 1 if diffE < etot_conv_thr else 0
+        if nconv >= 2:
+            break
+        Etot_old = Etot
+        if mix_method == "simple":
+            # V = betamix V_new + (1 - betamix) V_inp
+            mixH._lin(pot["Hartree"], betamix, [(1.0 - betamix, VH_inp)])
+            mixX._lin(pot["XC"], betamix, [(1.0 - betamix, Vxc_inp)])
+        else:
+            # mix_broyden!(V_new, V_inp, ...) / mix_adaptive!(V_new, V_inp, ...): the new
+            # potential is the first argument and receives the mix
+            mixH.mix(pot["Hartree"], VH_inp, it)
+            mixX.mix(pot["XC"], Vxc_inp, it)
+        Ham.install_potentials()
     el.ebands = evals
     Ham.energies = e
     return {"energies": e.as_dict(), "evals": evals, "psiks": psiks, "rho": rho.download(), "history": history,

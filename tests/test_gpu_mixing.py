@@ -94,3 +94,18 @@ def test_scf_mix_methods_si_fcc(method, betamix):
     E = out["energies"]["Total"]
     assert out["iterations"] < 100
     assert abs(E - (-7.9110091083)) < 2e-6, (method, E, out["iterations"])
+
+
+@pytest.mark.parametrize("method,betamix,ref", [("simple", 0.5, -7.9110091083), ("linear_adaptive", 0.1, -7.9110091083),
+                                                ("broyden", 0.5, -7.9110091083)])
+def test_scf_potmix_si_fcc(method, betamix, ref):
+    """KS_solve_SCF_potmix! on Si fcc with the reference's validation settings
+    (test/validations_old/Si_fcc.jl:114-140) reaches the self-consistent energy."""
+    import pwdft_hip as ph
+    print(f"case SCF potmix {method}", flush=True)
+    sym, frac = si_fcc_structure()
+    H = product_ham(sym, frac, ph.gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(3, 3, 3), use_symmetry=True)
+    out = ph.KS_solve_SCF_potmix_(H, betamix=betamix, mix_method=method, etot_conv_thr=1e-8,
+                                  ethr_evals_last=1e-9, NiterMax=150)
+    assert out["iterations"] < 150
+    assert abs(out["energies"]["Total"] - ref) < 2e-6, (method, out["energies"]["Total"], out["iterations"])
