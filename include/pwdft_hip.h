@@ -187,6 +187,9 @@ int pw_mix_rhoe_dev(pw_ctx* ctx, double beta, const double* rho_new, const doubl
 int pw_vec_dots_dev(pw_ctx* ctx, int64_t n, int npair, const double* const* a, const double* const* b, double* out);
 int pw_vec_lincomb_dev(pw_ctx* ctx, int64_t n, int m, const double* coef, const double* const* x, double ay,
                        double* y);
+/* A[:, j] *= s[j] (s: host, ncols values) for a column-major complex device block: the
+ * occupation weights wk Focc of calc_grad (calc_grad.jl:1-57). */
+int pw_scale_cols_dev(pw_ctx* ctx, int rows, int ncols, const double* s, void* A, int64_t ld);
 /* y = max(y, lo) element-wise (the eps() clamp of KS_solve_SCF.jl:277-281 after mixing). */
 int pw_vec_clamp_dev(pw_ctx* ctx, int64_t n, double lo, double* y);
 /* mix_adaptive! (mix_adaptive.jl:34-54) on device vectors: mu (the density, updated), nu

@@ -316,3 +316,16 @@ def diag_Emin_PCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0, tol_eban
     evals, evecs = np.linalg.eigh(Hr)
     X[:] = X @ evecs
     return evals, it_done, ncols
+
+
+def calc_grad(opH, psi, Focc_k, wk):
+    """calc_grad.jl:1-57: wk Focc_i (H psi - psi psi' H psi)_i, plus psi Q with
+    Q = (Hr F - F Hr)/2 (Hr = psi' H psi, F = diag(Focc)) unless the occupations are all 2
+    or all 1."""
+    Hpsi = opH(psi)
+    grad = (Hpsi - psi @ (psi.conj().T @ Hpsi)) * (np.asarray(Focc_k) * wk)[None, :]
+    if np.all(Focc_k == 2.0) or np.all(Focc_k == 1.0):
+        return grad
+    F = np.diag(Focc_k)
+    Hr = psi.conj().T @ Hpsi
+    return grad + psi @ (0.5 * (Hr @ F - F @ Hr))

@@ -329,6 +329,16 @@ __global__ void k_vlin(VecLin v, int m, double ay, double* y, long n) {
   }
 }
 
+// A[:, j] *= s[j] for a column-major complex block (rows x ncols, leading dimension ld).
+__global__ void k_scale_cols(cplx<double>* __restrict__ A, long ld, int rows, int ncols, const double* __restrict__ s) {
+  const long tot = (long)rows * ncols;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t / rows), i = (int)(t - (long)j * rows);
+    cplx<double>& a = A[(long)j * ld + i];
+    a = scal(s[j], a);
+  }
+}
+
 __global__ void k_vclamp(double* __restrict__ y, long n, double lo) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     if (y[i] < lo) y[i] = lo;
@@ -684,6 +694,24 @@ int pw_vec_lincomb_dev(pw_ctx* c, int64_t n, int m, const double* coef, const do
   }
   hipLaunchKernelGGL(k_vlin, dim3(grid_blocks(n)), dim3(SNT), 0, c->stream, v, m, ay, y, (long)n);
   return hip_check(c, hipGetLastError(), "k_vlin");
+}
+
+int pw_scale_cols_dev(pw_ctx* c, int rows, int ncols, const double* s, void* A, int64_t ld) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (rows < 0 || ncols < 0 || ld < rows || (ncols > 0 && (!s || !A)))
+    return set_error(c, PW_ERR_ARG, "pw_scale_cols_dev: bad arguments");
+  if (rows == 0 || ncols == 0) return PW_OK;
+  int rc = ensure(c, &c->d_spart, &c->spart_bytes,
+                  std::max<size_t>(3 * (size_t)SRED, (size_t)VDOT_BLK * VDOT_MAX) * sizeof(double) +
+                      (size_t)ncols * sizeof(double));
+  if (rc) return rc;
+  double* ds = reinterpret_cast<double*>(c->d_spart) + std::max<size_t>(3 * (size_t)SRED, (size_t)VDOT_BLK * VDOT_MAX);
+  hipError_t e = hipMemcpyAsync(ds, s, ncols * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "column scales upload");
+  hipLaunchKernelGGL(k_scale_cols, dim3(grid_blocks((long)rows * ncols)), dim3(SNT), 0, c->stream,
+                     reinterpret_cast<cplx<double>*>(A), (long)ld, rows, ncols, ds);
+  return hip_check(c, hipGetLastError(), "k_scale_cols");
 }
 
 int pw_vec_clamp_dev(pw_ctx* c, int64_t n, double lo, double* y) {

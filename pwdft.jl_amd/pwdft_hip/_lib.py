@@ -97,6 +97,7 @@ _PROTOS = {
     "pw_build_projectors_dev": [_vp, _i, _i, _vp, _i, _vp],
     "pw_build_vps_loc_dev": [_vp, _vp],
     "pw_set_vloc_parts_dev": [_vp, _i, _vp, _vp, _vp],
+    "pw_scale_cols_dev": [_vp, _i, _i, _vp, _vp, _i64],
 }
 
 EXPORTED = sorted(list(_PROTOS) + ["pw_last_error"])

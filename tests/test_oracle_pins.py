@@ -12,7 +12,7 @@ from oracle import ops as O
 from oracle import setup as S
 from oracle.density import calc_epsxc_Vxc_VWN, calc_rhoe, Poisson_solve
 from oracle.energy import calc_E_NN, Hamiltonian, KS_solve_SCF
-from oracle.solvers import diag_Emin_PCG, diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
+from oracle.solvers import calc_grad, diag_Emin_PCG, diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
 
 
 def test_vwn_table():
@@ -225,6 +225,28 @@ def test_pcg_free_electron_and_dense(si_small, cg_beta):
                              tol_ebands=0.0, I_CG_BETA=cg_beta)
     assert np.abs(ev - exact).max() < 1e-8
     assert np.allclose(X.conj().T @ X, np.eye(8), atol=1e-10)
+
+
+def test_calc_grad_properties(si_small):
+    """calc_grad: with constant occupations the gradient is orthogonal to psi and equals
+    wk f (H psi - psi psi'H psi); it is the derivative of E = wk sum_i f_i <psi_i|H|psi_i>
+    along directions orthogonal to psi (central differences, 1e-7)."""
+    from oracle.solvers import calc_grad_evals
+    H = si_small
+    H.ik = 0
+    n = H.pw.Ngw[0]
+    rng = np.random.default_rng(21)
+    psi = rand_wavefunc(rng, n, 4)
+    opH = lambda Y: O.op_H(H, Y)
+    g = calc_grad(opH, psi, np.full(4, 2.0), 0.25)
+    assert np.abs(psi.conj().T @ g).max() < 1e-12 * np.abs(g).max()
+    assert np.abs(g - 0.5 * calc_grad_evals(opH, psi)).max() < 1e-14 * np.abs(g).max()
+    d = rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4))
+    d -= psi @ (psi.conj().T @ d)
+    E = lambda X: 0.25 * 2.0 * np.real(np.trace(X.conj().T @ opH(X)))
+    h = 1e-4
+    fd = (E(psi + h * d) - E(psi - h * d)) / (2 * h)
+    assert abs(fd - 2 * np.real(np.vdot(g, d))) < 1e-7 * abs(fd)
 
 
 def test_lobpcg_vs_dense(si_small):
