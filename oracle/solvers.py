@@ -177,6 +177,100 @@ def diag_davidson(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
     return evals, it_done, ncols
 
 
+def diag_davidson_qe(opH, prec, evc, EBANDS_THR=1e-8, NiterMax=40, opS=None):
+    """diag_davidson_qe! (diag_davidson_qe_v2.jl:40-216) for one k-point: QE's cegterg.
+    The search space psi grows by the preconditioned, normalised corrections
+    (H - e S) psi vc of the unconverged roots only, up to Nvecx = 2 Nvec columns; the
+    reduced Hc, Sc are extended by their new columns; the basis is refreshed to the Ritz
+    vectors when all roots have converged (|ew - evals| < EBANDS_THR), when it would
+    overflow, or at iteration NiterMax.  opS defaults to the identity (op_S! of a GTH or
+    norm-conserving Hamiltonian, op_S.jl:82-89).  evc is overwritten in place; returns
+    (evals, iterations, number of opH columns applied)."""
+    if opS is None:
+        opS = lambda x: x.copy()
+    N, Nvec = evc.shape
+    Nvecx = 2 * Nvec
+    evals = np.zeros(Nvec)
+    psi = np.zeros((N, Nvecx), dtype=complex)
+    Hpsi = np.zeros((N, Nvecx), dtype=complex)
+    Spsi = np.zeros((N, Nvecx), dtype=complex)
+    Sc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    Hc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    vc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    ew = np.zeros(Nvecx)
+    is_conv = np.zeros(Nvec, dtype=bool)
+    notcnv = Nvec
+    nbase = Nvec
+    psi[:, :Nvec] = evc
+    Hpsi[:, :Nvec] = opH(psi[:, :Nvec])
+    Spsi[:, :Nvec] = opS(psi[:, :Nvec])
+    ncols = Nvec
+    Hc[:nbase, :nbase] = psi[:, :nbase].conj().T @ Hpsi[:, :nbase]
+    Sc[:nbase, :nbase] = psi[:, :nbase].conj().T @ Spsi[:, :nbase]
+    w, v = sla.eigh(herm_u(Hc[:Nvec, :Nvec]), herm_u(Sc[:Nvec, :Nvec]), lower=False)
+    ew[:Nvec] = w
+    vc[:Nvec, :Nvec] = v
+    evals[:] = w
+    kter = 1
+    dav_iter = 0
+    while kter <= NiterMax or notcnv == 0:
+        dav_iter = kter
+        # unconverged roots first, with their eigenvalue estimates beyond nbase
+        np_ = 0
+        for ist in range(Nvec):
+            if not is_conv[ist]:
+                if np_ != ist:
+                    vc[:, np_] = vc[:, ist]
+                ew[nbase + np_] = evals[ist]
+                np_ += 1
+        nb1 = nbase
+        new = slice(nb1, nb1 + notcnv)
+        # correction vectors (H - e S) psi vc, preconditioned and normalised
+        psi[:, new] = Spsi[:, :nbase] @ vc[:nbase, :notcnv]
+        psi[:, new] *= -ew[nbase:nbase + notcnv][None, :]
+        psi[:, new] += Hpsi[:, :nbase] @ vc[:nbase, :notcnv]
+        psi[:, new] = prec(psi[:, new])
+        ew[:notcnv] = np.real(np.einsum("gi,gi->i", psi[:, new].conj(), psi[:, new]))
+        psi[:, new] /= np.sqrt(ew[:notcnv])[None, :]
+        Hpsi[:, new] = opH(psi[:, new])
+        Spsi[:, new] = opS(psi[:, new])
+        ncols += notcnv
+        Hc[:nbase + notcnv, new] = psi[:, :nbase + notcnv].conj().T @ Hpsi[:, new]
+        Sc[:nbase + notcnv, new] = psi[:, :nbase + notcnv].conj().T @ Spsi[:, new]
+        nbase += notcnv
+        for n in range(nbase):
+            Hc[n, n] = Hc[n, n].real
+            Sc[n, n] = Sc[n, n].real
+            Hc[n + 1:nbase, n] = Hc[n, n + 1:nbase].conj()
+            Sc[n + 1:nbase, n] = Sc[n, n + 1:nbase].conj()
+        w, v = sla.eigh(herm_u(Hc[:nbase, :nbase]), herm_u(Sc[:nbase, :nbase]), lower=False)
+        ew[:nbase] = w
+        vc[:nbase, :nbase] = v
+        vc[:, Nvec:] = 0.0
+        is_conv = np.abs(ew[:Nvec] - evals) < EBANDS_THR
+        notcnv = int(np.count_nonzero(~is_conv))
+        evals[:] = ew[:Nvec]
+        if notcnv == 0 or nbase + notcnv > Nvecx or dav_iter == NiterMax:
+            evc[:, :] = psi[:, :nbase] @ vc[:nbase, :Nvec]
+            if notcnv == 0 or dav_iter == NiterMax:
+                break
+            psi[:, :Nvec] = evc
+            Snew = Spsi[:, :nbase] @ vc[:nbase, :Nvec]
+            Hnew = Hpsi[:, :nbase] @ vc[:nbase, :Nvec]
+            Spsi[:, :Nvec] = Snew
+            Hpsi[:, :Nvec] = Hnew
+            nbase = Nvec
+            Hc[:, :nbase] = 0.0
+            Sc[:, :nbase] = 0.0
+            vc[:, :nbase] = 0.0
+            for n in range(nbase):
+                Hc[n, n] = evals[n]
+                Sc[n, n] = 1.0
+                vc[n, n] = 1.0
+        kter += 1
+    return evals, dav_iter, ncols
+
+
 # ---------------------------------------------------------------------------- CheFSI
 
 

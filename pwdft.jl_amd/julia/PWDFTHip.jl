@@ -6,7 +6,9 @@ Hamiltonian application, density accumulation and eigensolver products on MI355X
 
 Loading this module after `using PWDFT` adds methods for `Hamiltonian{PsPot_GTH}` that
 route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! (with symmetrize_rhoe!) and
-update! (Poisson, VWN or PBE, V_Total) through the C ABI.  The reference's own
+update! (Poisson, VWN or PBE, V_Total) through the C ABI.  The op_H family also runs on the
+device for norm-conserving `Hamiltonian{PsPot_UPF}` (single grid, collinear), whose
+nonlocal term couples each atom's projectors through its dense Deeq block.  The reference's own
 KS_solve_SCF!, diag_LOBPCG! and diag_davidson! then run unchanged on top of them (they call
 op_H, calc_rhoe! and update!).  A HIP context is attached to each Hamiltonian the first
 time it is used, keyed by object identity; the device update! installs V_Total as the op_H
@@ -21,7 +23,11 @@ module PWDFTHip
 using PWDFT
 import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
               calc_rhoe!, update!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH,
-              BlochWavefunc
+              PsPot_UPF, BlochWavefunc
+
+# Hamiltonians whose op_H family runs on the device: GTH, and norm-conserving UPF (the
+# nonlocal term then couples each atom's projectors through its dense Deeq block)
+const HamHip = Union{Hamiltonian{PsPot_GTH},Hamiltonian{PsPot_UPF}}
 
 const libpw = joinpath(@__DIR__, "..", "pwdft_hip", "libpwdft_hip.so")
 
@@ -50,8 +56,9 @@ function check(ctx::HipCtx, rc::Cint)
 end
 
 """Attach (once) a HIP context holding the grid, k-point spheres, projectors and V_loc."""
-function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
+function context(Ham::HamHip; device::Int=0)
     haskey(CONTEXTS, Ham) && return CONTEXTS[Ham]
+    check_supported(Ham)
     r = Ref{Ptr{Cvoid}}(C_NULL)
     rc = ccall((:pw_ctx_create, libpw), Cint, (Ref{Ptr{Cvoid}}, Cint), r, device)
     rc == 0 || error("pw_ctx_create failed ($rc)")
@@ -60,18 +67,8 @@ function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
     Ns = pw.Ns
     check(ctx, ccall((:pw_set_grid, libpw), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble),
                      ctx.h, Ns[1], Ns[2], Ns[3], pw.CellVolume))
-    # D matrix from prj2beta blocks (PsPotNL.jl:147-195)
     nb = Ham.pspotNL.NbetaNL
-    D = zeros(Float64, nb, nb)
-    atoms = Ham.atoms
-    for ia in 1:atoms.Natoms
-        psp = Ham.pspots[atoms.atm2species[ia]]
-        for l in 0:psp.lmax, m in -l:l, i in 1:psp.Nproj_l[l+1], j in 1:psp.Nproj_l[l+1]
-            ib = Ham.pspotNL.prj2beta[i, ia, l+1, m+psp.lmax+1]
-            jb = Ham.pspotNL.prj2beta[j, ia, l+1, m+psp.lmax+1]
-            D[ib, jb] = psp.h[l+1, i, j]
-        end
-    end
+    D = nonlocal_D(Ham)
     check(ctx, ccall((:pw_set_dij, libpw), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), ctx.h, nb, D))
     G = pw.gvec.G
     for ik in 1:pw.gvecw.kpoints.Nkpt
@@ -89,10 +86,54 @@ function context(Ham::Hamiltonian{PsPot_GTH}; device::Int=0)
     end
     CONTEXTS[Ham] = ctx
     sync_potential!(Ham)
-    sync_atoms!(Ham)
-    sync_symmetry!(Ham)
-    check(ctx, ccall((:pw_set_vps_loc, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, Ham.potentials.Ps_loc))
+    if Ham isa Hamiltonian{PsPot_GTH}
+        sync_atoms!(Ham)
+        sync_symmetry!(Ham)
+        check(ctx, ccall((:pw_set_vps_loc, libpw), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, Ham.potentials.Ps_loc))
+    end
     return ctx
+end
+
+check_supported(Ham::Hamiltonian{PsPot_GTH}) = nothing
+function check_supported(Ham::Hamiltonian{PsPot_UPF})
+    Ham.need_overlap && error("PWDFTHip: ultrasoft and PAW pseudopotentials (need_overlap) are not supported")
+    Ham.options.noncollinear && error("PWDFTHip: noncollinear Hamiltonians are not supported")
+    Ham.pw.using_dual_grid && error("PWDFTHip: dual (smooth + dense) grids are not supported")
+    Deeq = Ham.pspotNL.Deeq
+    all(s -> Deeq[:, :, :, s] == Deeq[:, :, :, 1], 1:size(Deeq, 4)) ||
+        error("PWDFTHip: spin-dependent Deeq is not supported")
+    return nothing
+end
+
+"""GTH: D[ib, jb] = h[l, i, j] on the prj2beta blocks (PsPotNL.jl:147-195), what
+op_V_Ps_nloc.jl:39-78 sums term by term."""
+function nonlocal_D(Ham::Hamiltonian{PsPot_GTH})
+    nb = Ham.pspotNL.NbetaNL
+    D = zeros(Float64, nb, nb)
+    atoms = Ham.atoms
+    for ia in 1:atoms.Natoms
+        psp = Ham.pspots[atoms.atm2species[ia]]
+        for l in 0:psp.lmax, m in -l:l, i in 1:psp.Nproj_l[l+1], j in 1:psp.Nproj_l[l+1]
+            ib = Ham.pspotNL.prj2beta[i, ia, l+1, m+psp.lmax+1]
+            jb = Ham.pspotNL.prj2beta[j, ia, l+1, m+psp.lmax+1]
+            D[ib, jb] = psp.h[l+1, i, j]
+        end
+    end
+    return D
+end
+
+"""Norm-conserving UPF: Deeq[1:nh, 1:nh, ia] on the diagonal block of atom ia starting at
+indv_ijkb0[ia] + 1 (op_V_Ps_nloc.jl:82-126)."""
+function nonlocal_D(Ham::Hamiltonian{PsPot_UPF})
+    nl = Ham.pspotNL
+    D = zeros(Float64, nl.NbetaNL, nl.NbetaNL)
+    atoms = Ham.atoms
+    for ia in 1:atoms.Natoms
+        n = nl.nh[atoms.atm2species[ia]]
+        r = (nl.indv_ijkb0[ia]+1):(nl.indv_ijkb0[ia]+n)
+        D[r, r] .= nl.Deeq[1:n, 1:n, ia, 1]
+    end
+    return D
 end
 
 """symmetrize_rhoe! tables (RhoeSymmetrizer.jl:111-236) from Ham.sym_info: the library's
@@ -154,7 +195,7 @@ update!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc, Rhoe::Matrix{Float64}
     update!(Ham, psiks, view(Rhoe, :, 1))
 
 """Upload Ham.potentials.Total (call after update!)."""
-function sync_potential!(Ham::Hamiltonian{PsPot_GTH})
+function sync_potential!(Ham::HamHip)
     ctx = CONTEXTS[Ham]
     V = Ham.potentials.Total
     for ispin in 1:size(V, 2)
@@ -163,7 +204,7 @@ function sync_potential!(Ham::Hamiltonian{PsPot_GTH})
     end
 end
 
-function apply!(Ham::Hamiltonian{PsPot_GTH}, psi::AbstractMatrix{ComplexF64},
+function apply!(Ham::HamHip, psi::AbstractMatrix{ComplexF64},
                 Hpsi::AbstractMatrix{ComplexF64}, flags::UInt32)
     ctx = context(Ham)
     flags |= ctx.mixed ? PW_MIXED : UInt32(0)
@@ -175,17 +216,17 @@ function apply!(Ham::Hamiltonian{PsPot_GTH}, psi::AbstractMatrix{ComplexF64},
 end
 
 # op_H.jl:4-27 and :53-63 (EXX is skipped when Ham.exx === nothing, see op_H.jl:61)
-op_H(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_ALL)
-op_H!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+op_H(Ham::HamHip, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_ALL)
+op_H!(Ham::HamHip, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
     (apply!(Ham, psi, Hpsi, PW_TERM_ALL); nothing)
-op_K(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_K)
-op_K!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+op_K(Ham::HamHip, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_K)
+op_K!(Ham::HamHip, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
     (apply!(Ham, psi, Hpsi, PW_TERM_K | PW_ACCUMULATE); nothing)
-op_V_loc(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_VLOC)
-op_V_loc!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+op_V_loc(Ham::HamHip, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_VLOC)
+op_V_loc!(Ham::HamHip, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
     (apply!(Ham, psi, Hpsi, PW_TERM_VLOC | PW_ACCUMULATE); nothing)
-op_V_Ps_nloc(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_NLOC)
-op_V_Ps_nloc!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
+op_V_Ps_nloc(Ham::HamHip, psi::Matrix{ComplexF64}) = apply!(Ham, psi, similar(psi), PW_TERM_NLOC)
+op_V_Ps_nloc!(Ham::HamHip, psi::Matrix{ComplexF64}, Hpsi::Matrix{ComplexF64}) =
     (apply!(Ham, psi, Hpsi, PW_TERM_NLOC | PW_ACCUMULATE); nothing)
 
 # calc_rhoe.jl:9-135 (norm-conserving, Nspin = 1), symmetrised on the device

@@ -183,10 +183,52 @@ class Hamiltonian:
         c.call("pw_build_vps_loc_dev", ptr(V))
         self.potentials.Ps_loc = V
 
+    def set_nonlocal_deeq(self, Deeq, nh, indv_ijkb0, betaNL=None):
+        """Nonlocal data in the form of a norm-conserving UPF Hamiltonian (PsPotNL_UPF):
+        Deeq (nhm x nhm x Natoms, real symmetric) couples the nh[isp] projectors of each
+        atom, whose first projector is column indv_ijkb0[ia] (0-based) of betaNL[ik]
+        (Ngw[ik] x NbetaNL; default: the projectors already in place).  op_V_Ps_nloc then
+        applies Hpsi += beta (D (beta' psi)) with D the block-diagonal matrix of the Deeq
+        blocks (op_V_Ps_nloc.jl:82-126), and the nonlocal energy and forces use the same D
+        (calc_energies.jl:61-128, calc_forces_Ps_nloc.jl:31-160 with Deff = Deeq)."""
+        atoms, pw, c = self.atoms, self.pw, self.ctx
+        nbeta = self.pspotNL.NbetaNL if betaNL is None else int(np.shape(betaNL[0])[1])
+        Deeq = np.asarray(Deeq, dtype=float)
+        if Deeq.ndim != 3 or Deeq.shape[2] != atoms.Natoms or Deeq.shape[0] != Deeq.shape[1]:
+            raise ValueError("Deeq must be nhm x nhm x Natoms")
+        D = np.zeros((nbeta, nbeta), order="F")
+        owner = np.full(nbeta, -1, dtype=np.int32)
+        for ia in range(atoms.Natoms):
+            n = int(nh[atoms.atm2species[ia]])
+            o = int(indv_ijkb0[ia])
+            if n > Deeq.shape[0] or o < 0 or o + n > nbeta or np.any(owner[o:o + n] >= 0):
+                raise ValueError(f"projector block of atom {ia} does not fit NbetaNL = {nbeta}")
+            blk = Deeq[:n, :n, ia]
+            if np.abs(blk - blk.T).max() > 1e-12 * max(1.0, np.abs(blk).max()):
+                raise ValueError(f"Deeq block of atom {ia} is not symmetric")
+            D[o:o + n, o:o + n] = blk
+            owner[o:o + n] = ia
+        if np.any(owner < 0):
+            raise ValueError("every projector must belong to one atom block")
+        betas = self.pspotNL.betaNL if betaNL is None else betaNL
+        c.call("pw_set_dij", nbeta, ptr(D))
+        for ik in range(pw.Nkpt):
+            b = np.asfortranarray(betas[ik], dtype=np.complex128)
+            if b.shape != (pw.Ngw[ik], nbeta):
+                raise ValueError(f"betaNL[{ik}] must be Ngw x NbetaNL")
+            c.call("pw_set_projectors", ik, nbeta, ptr(b), int(pw.Ngw[ik]))
+        tau = np.asfortranarray(atoms.positions, dtype=float)
+        sp = np.ascontiguousarray(atoms.atm2species, dtype=np.int32)
+        c.call("pw_set_atoms", atoms.Natoms, ptr(tau), ptr(sp), nbeta, ptr(owner) if nbeta else None)
+        self._deeq_nonlocal = True
+
     def set_positions(self, positions):
         """Move the atoms (Cartesian bohr, 3 x Natoms): rebuild the structure-factor
         dependent set-up as Hamiltonian() does -- betaNL and V_Ps_loc on the device, E_NN on
         the host; V_loc is rebuilt from V_Ps_loc + the current Hartree and XC parts."""
+        if getattr(self, "_deeq_nonlocal", False):
+            raise ValueError("set_positions rebuilds GTH projectors; after set_nonlocal_deeq pass the moved "
+                             "projectors to set_nonlocal_deeq instead")
         atoms = self.atoms
         old = atoms.positions
         atoms.positions = np.array(positions, dtype=float).reshape(3, atoms.Natoms)

@@ -1,5 +1,6 @@
-"""Block eigensolvers of PWDFT.jl on the device: diag_LOBPCG! (diag_LOBPCG.jl:1-225) and
-diag_davidson! (diag_davidson.jl:1-191), plus ortho_sqrt (ortho_sqrt.jl:1-9).
+"""Block eigensolvers of PWDFT.jl on the device: diag_LOBPCG! (diag_LOBPCG.jl:1-225),
+diag_davidson! (diag_davidson.jl:1-191) and diag_davidson_qe! (diag_davidson_qe_v2.jl:1-216),
+plus ortho_sqrt (ortho_sqrt.jl:1-9).
 
 Everything Ngw-sized (X, W, P, their H-images, residuals, the Gram/overlap products and
 the Q*U updates) stays in device memory and runs through libpwdft_hip; only the small
@@ -421,6 +422,119 @@ def diag_davidson_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None):
     for m in (Q, HQ, Tmp):
         m.free()
     return evals, {"iter": it_done, "ncols": ncols}
+
+
+def diag_davidson_qe_(Ham, X, EBANDS_THR=1e-8, NiterMax=40, mixed=None):
+    """diag_davidson_qe! for the current Ham.ik (diag_davidson_qe_v2.jl:40-216), QE's
+    cegterg.  The search space P (Ngw x 2Nst) grows by the preconditioned, normalised
+    corrections (H - e) P vc of the unconverged roots only; HP holds its H-image.  The
+    reduced Hc, Sc (at most 2Nst square) and their Ritz vectors vc live on the host, as the
+    reference diagonalises them with LAPACK.  op_S! of a GTH Hamiltonian is the identity
+    (op_S.jl:82-89), so S P is P itself: the products with S P use P, which gives the same
+    numbers as the reference's separate Spsi copy.  X holds the Ritz vectors on return.
+    Returns (evals, {"iter", "ncols"})."""
+    mixed = Ham.mixed if mixed is None else mixed
+    D = _Dense(Ham, mixed)
+    c = Ham.ctx
+    Nb, Nvec = X.rows, X.cols
+    Nvecx = 2 * Nvec
+    P = DeviceMatrix(c, Nb, Nvecx)
+    HP = DeviceMatrix(c, Nb, Nvecx)
+    T = DeviceMatrix(c, Nb, Nvec)
+    Hc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    Sc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    vc = np.zeros((Nvecx, Nvecx), dtype=complex)
+    ew = np.zeros(Nvecx)
+    is_conv = np.zeros(Nvec, dtype=bool)
+    notcnv = nbase = Nvec
+    D.copy(Nb, Nvec, X.p, P.p)
+    D.opH(P.p, HP.p, Nvec, Nb)
+    ncols = Nvec
+    Hc[:Nvec, :Nvec] = D.gram(P.p, HP.p, Nvec, Nvec, Nb)
+    Sc[:Nvec, :Nvec] = D.gram(P.p, P.p, Nvec, Nvec, Nb)
+    w, v = sla.eigh(herm_u(Hc[:Nvec, :Nvec]), herm_u(Sc[:Nvec, :Nvec]), lower=False)
+    ew[:Nvec] = w
+    vc[:Nvec, :Nvec] = v
+    evals = w.copy()
+    kter = 1
+    dav_iter = 0
+    small = []
+    while kter <= NiterMax or notcnv == 0:
+        dav_iter = kter
+        k = 0
+        for ist in range(Nvec):
+            if not is_conv[ist]:
+                if k != ist:
+                    vc[:, k] = vc[:, ist]
+                ew[nbase + k] = evals[ist]
+                k += 1
+        # T = P vc (-e) + HP vc over the unconverged roots, then P_new = K T
+        V = vc[:nbase, :notcnv]
+        dVe = D.upload_small(V * (-ew[nbase:nbase + notcnv])[None, :])
+        dV = D.upload_small(V)
+        small += [dVe, dV]
+        D.gemm(0, Nb, notcnv, nbase, P.p, Nb, dVe.p, nbase, T.p, Nb)
+        D.gemm(0, Nb, notcnv, nbase, HP.p, Nb, dV.p, nbase, T.p, Nb, beta=_ONE)
+        new = P.col(nbase)
+        D.kprec(T.p, new, notcnv, Nb)
+        nrm = np.real(np.diag(D.gram(new, new, notcnv, notcnv, Nb)))
+        ew[:notcnv] = nrm
+        inv = np.ascontiguousarray(1.0 / np.sqrt(nrm))
+        c.call("pw_scale_cols_dev", Nb, notcnv, ptr(inv), new, Nb)
+        D.opH(new, HP.col(nbase), notcnv, Nb)
+        ncols += notcnv
+        m = nbase + notcnv
+        Hc[:m, nbase:m] = D.gram(P.p, HP.col(nbase), m, notcnv, Nb)
+        Sc[:m, nbase:m] = D.gram(P.p, new, m, notcnv, Nb)
+        nbase = m
+        Hc[:nbase, :nbase] = herm_u(Hc[:nbase, :nbase])
+        Sc[:nbase, :nbase] = herm_u(Sc[:nbase, :nbase])
+        w, v = sla.eigh(Hc[:nbase, :nbase], Sc[:nbase, :nbase])
+        ew[:nbase] = w
+        vc[:nbase, :nbase] = v
+        vc[:, Nvec:] = 0.0
+        is_conv = np.abs(ew[:Nvec] - evals) < EBANDS_THR
+        notcnv = int(np.count_nonzero(~is_conv))
+        evals = ew[:Nvec].copy()
+        if notcnv == 0 or nbase + notcnv > Nvecx or dav_iter == NiterMax:
+            dU = D.upload_small(vc[:nbase, :Nvec])
+            small.append(dU)
+            D.gemm(0, Nb, Nvec, nbase, P.p, Nb, dU.p, nbase, X.p, Nb)
+            if notcnv == 0 or dav_iter == NiterMax:
+                break
+            D.gemm(0, Nb, Nvec, nbase, HP.p, Nb, dU.p, nbase, T.p, Nb)
+            D.copy(Nb, Nvec, X.p, P.p)
+            D.copy(Nb, Nvec, T.p, HP.p)
+            nbase = Nvec
+            Hc[:, :nbase] = 0.0
+            Sc[:, :nbase] = 0.0
+            vc[:, :nbase] = 0.0
+            Hc[np.arange(nbase), np.arange(nbase)] = evals
+            Sc[np.arange(nbase), np.arange(nbase)] = 1.0
+            vc[np.arange(nbase), np.arange(nbase)] = 1.0
+        if len(small) > 8:
+            c.call("pw_synchronize")
+            for s in small:
+                s.free()
+            small = []
+        kter += 1
+    c.call("pw_synchronize")
+    for m_ in [P, HP, T] + small:
+        m_.free()
+    return evals, {"iter": dav_iter, "ncols": ncols}
+
+
+def diag_davidson_qe(Ham, psiks, tol=1e-5, NiterMax=100):
+    """diag_davidson_qe!(Ham, psiks::BlochWavefunc) (diag_davidson_qe_v2.jl:6-37): per
+    k-point with EBANDS_THR = tol; evals Nst x Nkpt and the per-k statistics."""
+    Nk = Ham.pw.Nkpt
+    evals = np.zeros((psiks[0].cols, Nk))
+    stats = []
+    for ik in range(Nk):
+        Ham.ik, Ham.ispin = ik, 0
+        evals[:, ik], st = diag_davidson_qe_(Ham, psiks[ik], EBANDS_THR=tol, NiterMax=NiterMax)
+        stats.append(st)
+    return evals, stats
 
 
 def diag_LOBPCG(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0):

@@ -19,7 +19,17 @@ def _si(meshk=(1, 1, 1), **kw):
                        extra_states=4, **kw)
 
 
-@pytest.mark.parametrize("solver", ["LOBPCG", "davidson"])
+def _blochwise(ph, solver):
+    return {"LOBPCG": ph.diag_LOBPCG, "davidson": ph.diag_davidson, "davidson_qe": ph.diag_davidson_qe}[solver]
+
+
+def _per_k(ph, solver):
+    if solver == "davidson_qe":
+        return lambda H, X, tol, NiterMax: ph.diag_davidson_qe_(H, X, EBANDS_THR=tol, NiterMax=NiterMax)
+    return ph.diag_LOBPCG_ if solver == "LOBPCG" else ph.diag_davidson_
+
+
+@pytest.mark.parametrize("solver", ["LOBPCG", "davidson", "davidson_qe"])
 def test_free_electron(solver):
     """V = 0 and no projectors: eigenvalues are the sorted 1/2|G+k|^2
     (timing/time_diag_LOBPCG.jl:5-24)."""
@@ -30,7 +40,7 @@ def test_free_electron(solver):
     for ik in range(H.pw.Nkpt):
         H.ctx.call("pw_set_projectors", ik, 0, None, 0)
     psiks = ph.rand_BlochWavefunc(H, seed=5)
-    fn = ph.diag_LOBPCG if solver == "LOBPCG" else ph.diag_davidson
+    fn = _blochwise(ph, solver)
     print(f"case free electron {solver}", flush=True)
     evals, _ = fn(H, psiks, tol=1e-12, NiterMax=300)
     for ik in range(H.pw.Nkpt):
@@ -38,7 +48,7 @@ def test_free_electron(solver):
         assert np.abs(evals[:, ik] - exact).max() < 1e-8, (ik, evals[:, ik], exact)
 
 
-@pytest.mark.parametrize("solver", ["LOBPCG", "davidson"])
+@pytest.mark.parametrize("solver", ["LOBPCG", "davidson", "davidson_qe"])
 @pytest.mark.parametrize("mixed", [False, True])
 def test_eigs_vs_dense(solver, mixed):
     """Lowest 8 eigenvalues of the Si fcc Hamiltonian at one k-point against the dense
@@ -60,7 +70,7 @@ def test_eigs_vs_dense(solver, mixed):
     X = ph.DeviceMatrix.from_host(H.ctx, rand_psi(np.random.default_rng(1), n, 8))
     ph.ortho_sqrt_(H, X)
     H.mixed = mixed
-    fn = ph.diag_LOBPCG_ if solver == "LOBPCG" else ph.diag_davidson_
+    fn = _per_k(ph, solver)
     print(f"case eigs {solver} mixed={mixed}", flush=True)
     try:
         ev, st = fn(H, X, tol=1e-11 if not mixed else 1e-6, NiterMax=300)

@@ -12,7 +12,8 @@ from oracle import ops as O
 from oracle import setup as S
 from oracle.density import calc_epsxc_Vxc_VWN, calc_rhoe, Poisson_solve
 from oracle.energy import calc_E_NN, Hamiltonian, KS_solve_SCF
-from oracle.solvers import calc_grad, diag_Emin_PCG, diag_LOBPCG, diag_davidson, rand_wavefunc, ortho_sqrt
+from oracle.solvers import (calc_grad, diag_Emin_PCG, diag_LOBPCG, diag_davidson, diag_davidson_qe, rand_wavefunc,
+                            ortho_sqrt)
 
 
 def test_vwn_table():
@@ -187,7 +188,11 @@ def test_poisson_plane_wave(si_small):
     assert np.abs(Poisson_solve(pw, rho) - 4 * np.pi / (g @ g) * rho).max() < 1e-10
 
 
-@pytest.mark.parametrize("solver", [diag_LOBPCG, diag_davidson])
+def _davidson_qe(opH, prec, X, tol, NiterMax):
+    return diag_davidson_qe(opH, prec, X, EBANDS_THR=tol, NiterMax=NiterMax)
+
+
+@pytest.mark.parametrize("solver", [diag_LOBPCG, diag_davidson, _davidson_qe])
 def test_free_electron(solver):
     """timing/time_diag_LOBPCG.jl:5-24: V = 0, no projectors -> sorted 1/2|G+k|^2."""
     sym, frac = si_fcc_structure()
@@ -259,6 +264,49 @@ def test_lobpcg_vs_dense(si_small):
     ev, _, _ = diag_LOBPCG(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, tol=1e-12, NiterMax=300)
     assert np.abs(ev - exact).max() < 1e-9
     assert np.allclose(X.conj().T @ X, np.eye(8), atol=1e-10)
+
+
+def test_davidson_qe_vs_dense(si_small):
+    """diag_davidson_qe! converges to dense eigh of the full H; the returned block is
+    orthonormal and diagonalises H (Rayleigh-Ritz: X'HX = diag(evals))."""
+    H = si_small
+    H.ik = 1
+    n = H.pw.Ngw[1]
+    Hd = O.op_H(H, np.eye(n, dtype=complex))
+    exact = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))[:8]
+    X = rand_wavefunc(np.random.default_rng(7), n, 8)
+    ev, it, ncols = diag_davidson_qe(lambda Y: O.op_H(H, Y), lambda R: O.Kprec(H, R), X, EBANDS_THR=1e-12,
+                                     NiterMax=300)
+    assert np.abs(ev - exact).max() < 1e-9
+    assert np.allclose(X.conj().T @ X, np.eye(8), atol=1e-10)
+    assert np.abs(X.conj().T @ O.op_H(H, X) - np.diag(ev)).max() < 1e-9
+    assert it < 300 and ncols < 8 * (it + 1)     # converged roots stop expanding
+
+
+@pytest.mark.parametrize("nvec", [1, 3])
+def test_davidson_qe_one_step_rayleigh_ritz(nvec):
+    """One diag_davidson_qe! iteration on a dense Hermitian H with a diagonal preconditioner
+    K: the Ritz values of span{X, K r_i}, r_i = (H - e_i) X v_i for the initial Ritz pairs
+    (e_i, v_i), computed here through an orthonormal basis (QR) and eigh.  A sign error in
+    the correction, a missing K or a wrong shift e_i changes that span."""
+    rng = np.random.default_rng(31)
+    n = 24
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Hm = 0.5 * (A + A.conj().T) + np.diag(np.arange(n, dtype=float))
+    kdiag = 1.0 / (1.0 + np.arange(n, dtype=float))
+    X0 = np.linalg.qr(rng.standard_normal((n, nvec)) + 1j * rng.standard_normal((n, nvec)))[0]
+    e, v = np.linalg.eigh(X0.conj().T @ Hm @ X0)
+    Y = X0 @ v
+    R = kdiag[:, None] * (Hm @ Y - Y * e[None, :])
+    Q = np.linalg.qr(np.hstack([X0, R]))[0]
+    expect = np.linalg.eigvalsh(Q.conj().T @ Hm @ Q)[:nvec]
+    X = X0.copy()
+    ev, it, ncols = diag_davidson_qe(lambda Y_: Hm @ Y_, lambda R_: kdiag[:, None] * R_, X, EBANDS_THR=1e-30,
+                                     NiterMax=1)
+    assert it == 1 and ncols == 2 * nvec
+    assert np.abs(ev - expect).max() < 1e-12 * np.abs(expect).max()
+    # the returned block is the Ritz block of that subspace
+    assert np.abs(X.conj().T @ Hm @ X - np.diag(ev)).max() < 1e-11 * np.abs(ev).max()
 
 
 @pytest.mark.slow
