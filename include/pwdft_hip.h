@@ -62,10 +62,11 @@ int pw_set_band_batch(pw_ctx* ctx, int nb);
 
 /* Tuning knobs set by the autotuner (pwdft_hip/autotune.py); no reference counterpart. */
 enum {
-  PW_TUNE_GEMM_CN_SHAPE = 0,  /* A^H B products: 0 = 64x64 tile, 1 = 128x64, 2 = 64x128, 3 = 128x128 */
+  PW_TUNE_GEMM_CN_SHAPE = 0,  /* A^H B products: -1 = chosen by N (default), 0 = 64x64 tile, 1 = 128x64,
+                                 2 = 64x128, 3 = 128x128, 4 = 128x32, 5 = 64x32 */
   PW_TUNE_GEMM_NN_SHAPE = 1,  /* A B products: same codes */
   PW_TUNE_GEMM_CN_WG = 2,     /* A^H B split-K target workgroup count (0 = default 1024) */
-  PW_TUNE_GEMM_NN_WG = 3,     /* A B split-K target workgroup count */
+  PW_TUNE_GEMM_NN_WG = 3,     /* A B split-K target workgroup count (0 = default 512) */
   PW_TUNE_BAND_BATCH = 4,     /* bands per fused FFT launch (same as pw_set_band_batch) */
   PW_TUNE_PLANE_COMPRESSED = 5, /* 1: always use the compressed z-plane kernels (they are used
                                    anyway when the padded plane does not fit in LDS) */

@@ -337,7 +337,7 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
   switch (key) {
     case PW_TUNE_GEMM_CN_SHAPE:
     case PW_TUNE_GEMM_NN_SHAPE:
-      if (value < 0 || value > 3) return set_error(c, PW_ERR_ARG, "pw_tune_set: tile shape must be 0..3");
+      if (value < -1 || value > 5) return set_error(c, PW_ERR_ARG, "pw_tune_set: tile shape must be -1..5");
       c->tune_gemm_shape[key - PW_TUNE_GEMM_CN_SHAPE] = value;
       return PW_OK;
     case PW_TUNE_GEMM_CN_WG:

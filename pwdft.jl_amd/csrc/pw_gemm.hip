@@ -10,11 +10,12 @@
 // At is a signed copy of one element of A, so one MFMA stream computes 8 real flops per
 // complex multiply-add with no wasted lanes.
 //
-// Tiles: a 256-thread workgroup owns 64 complex rows (128 real rows) x 64 columns of C; each
-// of its 4 waves owns 64 real rows x 32 columns as 4 x 2 MFMA 16x16 accumulators.  K is
-// consumed in stages of 16 complex (32 real) staged through k-contiguous, padded LDS rows;
-// the next stage is loaded into registers while the current one feeds the MFMAs.  Long K
-// (the Ngw-long inner products) is split into chunks; partial tiles are reduced by a
+// Tiles: a workgroup of WM x WN waves owns 32 WM complex rows (64 WM real rows) x 32 WN
+// columns of C; each wave owns 64 real rows x 32 columns as 4 x 2 MFMA 16x16 accumulators.
+// K is consumed in stages of 16 (FP32) or 8 (FP64) complex k staged through k-contiguous,
+// padded LDS rows in two buffers: the next stage's global loads are issued before the
+// current stage's MFMAs and written to the other buffer after them, one barrier per stage.
+// Long K (the Ngw-long inner products) is split into chunks; partial tiles are reduced by a
 // second kernel in a fixed order, so results are bitwise repeatable.
 #include "pw_internal.h"
 
@@ -49,8 +50,7 @@ struct Mfma<float> {
 // Workgroup of WM x WN wavefronts; each wavefront owns 64 real rows (32 complex) x 32
 // columns of C as 4 x 2 MFMA 16x16 accumulators, so the workgroup tile is
 // (32 WM complex rows) x (32 WN columns) while per-lane registers stay fixed.
-constexpr int GBK = 16;           // complex k per LDS stage (32 real k)
-constexpr int GKS = 2 * GBK + 4;  // LDS row stride (reals): k-contiguous rows, padded
+constexpr int GBK = 16;           // split-K chunks are multiples of this (and of every stage)
 
 // Workgroup id -> (m tile, n tile, k chunk).  Consecutive workgroup ids land on different
 // XCDs (round-robin over 8), so the mapping sends every tile of one k chunk (which share
@@ -82,9 +82,14 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
   constexpr int NT = 64 * WM * WN;
   constexpr int BMC = 32 * WM;   // complex rows per workgroup
   constexpr int BNC = 32 * WN;   // columns per workgroup
-  constexpr int NA = (BMC * GBK) / NT, NB = (BNC * GBK) / NT;
-  __shared__ TC As[BMC][GKS];   // [complex row m][real k'] raw (re, im) pairs along k
-  __shared__ TC Bs[BNC][GKS];   // [column n][real k']
+  // complex k per LDS stage: two stages (double buffer) of 16 (FP32) or 8 (FP64) complex k
+  // keep the LDS of a 64 x 64 tile at 36 KiB, so four workgroups fit a CU
+  constexpr int SBK = sizeof(TC) == 8 ? 8 : 16;
+  constexpr int SKS = 2 * SBK + 4;  // LDS row stride (reals): k-contiguous rows, padded
+  constexpr int NA = (BMC * SBK) / NT, NB = (BNC * SBK) / NT;
+  static_assert(NA >= 1 && NB >= 1 && (BMC * SBK) % NT == 0 && (BNC * SBK) % NT == 0, "tile");
+  __shared__ TC As[2][BMC][SKS];   // [buffer][complex row m][real k'] raw (re, im) pairs along k
+  __shared__ TC Bs[2][BNC][SKS];   // [buffer][column n][real k']
   typedef typename Mfma<TC>::acc_t acc_t;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -104,50 +109,69 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
       for (int r = 0; r < 4; ++r) acc[i][j][r] = (TC)0;
   const int li = lane & 15, lg = lane >> 4;
 
-  // global -> register staging (NA complex of A and NB of B per thread per stage)
-  cplx<TC> ra[NA], rb[NB];
+  // global -> register staging: NA complex of A and NB of B per thread per stage, held in
+  // their storage type and converted only when written to LDS, so the loads of the next
+  // stage stay in flight during the MFMA phase.  Out-of-range elements load a valid
+  // address (element 0) and are zeroed at the LDS write through the valid mask.
+  cplx<TA> ra[NA];
+  cplx<TB> rb[NB];
+  unsigned va = 0, vb = 0;
   auto a_idx = [&](int s, int& mm, int& kk) {
     if (TRANSA == 0) { mm = tid % BMC; kk = tid / BMC + (NT / BMC) * s; }
-    else { kk = tid & 15; mm = (tid >> 4) + (NT / 16) * s; }
+    else { kk = tid % SBK; mm = tid / SBK + (NT / SBK) * s; }
   };
+  auto b_idx = [&](int s, int& kb, int& nn) { kb = tid % SBK; nn = tid / SBK + (NT / SBK) * s; };
   auto load_stage = [&](int k0) {
+    va = 0;
+    vb = 0;
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
       int mm, kk;
       a_idx(s, mm, kk);
       const int gm = m0 + mm, gk = k0 + kk;
-      ra[s] = mk((TC)0, (TC)0);
-      if (gm < M && gk < kend) ra[s] = cvt<TC>(TRANSA == 0 ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]);
+      const bool ok = gm < M && gk < kend;
+      const long off = ok ? (TRANSA == 0 ? (long)gk * lda + gm : (long)gm * lda + gk) : 0;
+      ra[s] = A[off];
+      va |= (unsigned)ok << s;
     }
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
-      const int kb = tid & 15, nn = (tid >> 4) + (NT / 16) * s;
+      int kb, nn;
+      b_idx(s, kb, nn);
       const int gkb = k0 + kb, gn = n0 + nn;
-      rb[s] = mk((TC)0, (TC)0);
-      if (gkb < kend && gn < N) rb[s] = cvt<TC>(B[(long)gn * ldb + gkb]);
+      const bool ok = gkb < kend && gn < N;
+      rb[s] = B[ok ? (long)gn * ldb + gkb : 0];
+      vb |= (unsigned)ok << s;
     }
   };
-  auto store_stage = [&]() {
+  auto store_stage = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
       int mm, kk;
       a_idx(s, mm, kk);
-      *reinterpret_cast<cplx<TC>*>(&As[mm][2 * kk]) = ra[s];
+      const cplx<TC> v = ((va >> s) & 1u) ? cvt<TC>(ra[s]) : mk((TC)0, (TC)0);
+      *reinterpret_cast<cplx<TC>*>(&As[buf][mm][2 * kk]) = v;
     }
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
-      const int kb = tid & 15, nn = (tid >> 4) + (NT / 16) * s;
-      *reinterpret_cast<cplx<TC>*>(&Bs[nn][2 * kb]) = rb[s];
+      int kb, nn;
+      b_idx(s, kb, nn);
+      const cplx<TC> v = ((vb >> s) & 1u) ? cvt<TC>(rb[s]) : mk((TC)0, (TC)0);
+      *reinterpret_cast<cplx<TC>*>(&Bs[buf][nn][2 * kb]) = v;
     }
   };
 
-  if (kbeg < kend) load_stage(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
-    store_stage();
-    __syncthreads();
-    if (k0 + GBK < kend) load_stage(k0 + GBK);  // in flight during the MFMA phase
+  int buf = 0;
+  if (kbeg < kend) {
+    load_stage(kbeg);
+    store_stage(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += SBK) {
+    const bool more = k0 + SBK < kend;
+    if (more) load_stage(k0 + SBK);  // in flight during the MFMA phase
 #pragma unroll
-    for (int q = 0; q < (2 * GBK) / 4; ++q) {
+    for (int q = 0; q < (2 * SBK) / 4; ++q) {
       const int kp = 4 * q + lg;
       const int kc = kp >> 1, t = kp & 1;
       TC a[4], b[2];
@@ -155,18 +179,22 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
       for (int rt = 0; rt < 4; ++rt) {
         const int ip = wm * 64 + rt * 16 + Mfma<TC>::row(li);
         const int sgn = ip & 1, ml = ip >> 1;
-        const TC av = As[ml][2 * kc + (sgn ^ t)];
+        const TC av = As[buf][ml][2 * kc + (sgn ^ t)];
         const bool neg = (TRANSA == 0) ? (sgn == 0 && t == 1) : (sgn == 1 && t == 0);
         a[rt] = neg ? -av : av;
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) b[ct] = Bs[wn * 32 + ct * 16 + li][kp];
+      for (int ct = 0; ct < 2; ++ct) b[ct] = Bs[buf][wn * 32 + ct * 16 + li][kp];
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = Mfma<TC>::run(a[rt], b[ct], acc[rt][ct]);
     }
+    // one barrier per stage: the other buffer is written while this one is read, and no
+    // wave writes this buffer again before every wave has passed this barrier
+    if (more) store_stage(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
   // with the row maps above, registers (0, 1) and (2, 3) of lane group lg hold the
   // (re, im) pairs of complex rows 2*lg and 2*lg + 1 of each 16-row tile.
@@ -264,10 +292,23 @@ static int zgemm_t(pw_ctx* c, int transa, int M, int N, int K, cplx<double> alph
   if (M <= 0 || N <= 0) return PW_OK;
   // tuning knobs (pw_tune_set): tile shape and the split-K target workgroup count
   const int slot = transa ? 0 : 1;
-  const int shape = c->tune_gemm_shape[slot];
-  const int target = c->tune_gemm_wg[slot] > 0 ? c->tune_gemm_wg[slot] : 1024;
+  int shape = c->tune_gemm_shape[slot];
+  if (shape < 0) {
+    // 32-column tiles when they cover N with less padding than 64-column ones (N = 160
+    // bands: 160 columns instead of 192, a fifth of the MFMA work saved).  Measured on the
+    // Si64 projector products of op_H, which run beside the FFT passes: 64 x 32 for A^H B
+    // (split-K fills the chip); for A B 128 x 32 in FP64, while in FP32 the 64 x 64 tile
+    // (less LDS per workgroup) shares the CUs with the FFT kernels better
+    const int pad64 = (N + 63) / 64 * 64, pad32 = (N + 31) / 32 * 32;
+    if (pad32 < pad64) shape = transa ? 5 : (sizeof(TC) == 8 ? 4 : 0);
+    else shape = 0;
+  }
+  // A B with a short K splits only when the tiles leave most of the chip idle
+  const int target = c->tune_gemm_wg[slot] > 0 ? c->tune_gemm_wg[slot] : (transa ? 1024 : 512);
   switch (shape) {
     case 1: return zgemm_launch<TC, TA, TB, 4, 2>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+    case 4: return zgemm_launch<TC, TA, TB, 4, 1>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
+    case 5: return zgemm_launch<TC, TA, TB, 2, 1>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
     case 2: return zgemm_launch<TC, TA, TB, 2, 4>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
     case 3: return zgemm_launch<TC, TA, TB, 4, 4>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);
     default: return zgemm_launch<TC, TA, TB, 2, 2>(c, transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, target);

@@ -175,7 +175,9 @@ struct pw_ctx {
   int band_batch = 64;
   bool plane_compressed = false;  // force the compressed plane kernels (tuning / tests)
   // tuning (pw_tune_set): [0] = A^H B products, [1] = A B products
-  int tune_gemm_shape[2] = {0, 0};  // 0: 2x2 wavefronts (64x64), 1: 4x2, 2: 2x4, 3: 4x4
+  // -1: by N (pw_gemm.hip: zgemm_t); 0: 2x2 wavefronts (64x64 tile), 1: 4x2, 2: 2x4, 3: 4x4,
+  // 4: 4x1 (128 x 32), 5: 2x1 (64 x 32)
+  int tune_gemm_shape[2] = {-1, -1};
   int tune_gemm_wg[2] = {0, 0};     // split-K target workgroups (0 = 1024)
   std::string err;
   // per-kernel timing (pw_profile_*): event pairs recorded around launches on `stream`

@@ -8,7 +8,7 @@ import itertools
 from ._lib import PW_MIXED, PW_TERM_ALL
 
 TUNE_GEMM_CN_SHAPE, TUNE_GEMM_NN_SHAPE, TUNE_GEMM_CN_WG, TUNE_GEMM_NN_WG, TUNE_BAND_BATCH = range(5)
-SHAPES = {0: "64x64", 1: "128x64", 2: "64x128", 3: "128x128"}
+SHAPES = {-1: "auto", 0: "64x64", 1: "128x64", 2: "64x128", 3: "128x128", 4: "128x32", 5: "64x32"}
 
 
 def _profile_step(H, X, Y, flags, steps):
@@ -22,7 +22,7 @@ def _profile_step(H, X, Y, flags, steps):
     return {k: v[0] / steps for k, v in prof.items()}
 
 
-def tune_op_H(H, X, Y, mixed=None, steps=3, shapes=(0, 1, 2, 3), targets=(256, 512, 1024, 2048),
+def tune_op_H(H, X, Y, mixed=None, steps=3, shapes=(0, 1, 2, 3, 4, 5), targets=(256, 512, 1024, 2048),
               batches=None):
     """Sweep GEMM shapes/targets (both products at once, timed separately) and optionally
     the FFT band batch; apply and return the best settings with the timings (ms/step)."""

@@ -119,6 +119,39 @@ def test_zgemm(si_pair, transa, shape, mixed):
     assert err < (1e-5 if mixed else 1e-14), err
 
 
+@pytest.mark.parametrize("tile", [-1, 0, 1, 2, 3, 4, 5])
+def test_zgemm_every_tile(si_pair, tile):
+    """Each workgroup tile shape (pw_tune_set PW_TUNE_GEMM_*_SHAPE) on ragged sizes, with
+    split-K (K = 3001) and without, both precisions and both op(A)."""
+    from pwdft_hip._lib import DeviceMatrix, PW_MIXED, ptr
+    H, _ = si_pair
+    c = H.ctx
+    try:
+        c.call("pw_tune_set", 0, tile)
+        c.call("pw_tune_set", 1, tile)
+        for transa in (0, 1):
+            for M, N, K in ((37, 19, 100), (100, 130, 3001), (300, 160, 70), (1, 1, 1)):
+                rng = np.random.default_rng(M + 5 * N + 11 * K + transa)
+                A = rand_psi(rng, K, M) if transa else rand_psi(rng, M, K)
+                B = rand_psi(rng, K, N)
+                C0 = rand_psi(rng, M, N)
+                alpha = np.array([0.3, 0.9])
+                beta = np.array([-0.5, 0.1])
+                opA = A.conj().T if transa else A
+                ref = (0.3 + 0.9j) * (opA @ B) + (-0.5 + 0.1j) * C0
+                scale = np.abs(opA).max() * np.abs(B).max() * np.sqrt(K) + np.abs(C0).max()
+                for mixed in (False, True):
+                    dA, dB, dC = (DeviceMatrix.from_host(c, X) for X in (A, B, C0))
+                    print(f"case zgemm tile={tile} transa={transa} shape={(M, N, K)} mixed={mixed}", flush=True)
+                    c.call("pw_zgemm_dev", transa, M, N, K, ptr(alpha), dA.p, dA.rows, dB.p, dB.rows, ptr(beta),
+                           dC.p, M, PW_MIXED if mixed else 0)
+                    err = np.abs(dC.download() - ref).max() / scale
+                    assert err < (1e-5 if mixed else 1e-14), (tile, transa, (M, N, K), mixed, err)
+    finally:
+        c.call("pw_tune_set", 0, -1)
+        c.call("pw_tune_set", 1, -1)
+
+
 def test_kprec(si_pair):
     import pwdft_hip as ph
     from oracle import ops as O
