@@ -299,6 +299,11 @@ int pw_heevd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
 /* eigvals(Hermitian(A)) only (A is destroyed). */
 int pw_heev_values_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
 int pw_hegvd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w);
+/* eigen(Hermitian(A_i)) for nbatch matrices A_i = A + i*strideA (elements), each n x n with
+ * leading dimension lda: the per-k-point Rayleigh-Ritz problems of diag_LOBPCG! run over
+ * many k-points (diag_LOBPCG.jl:176-203, 217-222).  Eigenvectors into A_i (ascending),
+ * eigenvalues into w (host, n x nbatch, column i for matrix i). */
+int pw_heevd_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA, double* w);
 
 /* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */
 

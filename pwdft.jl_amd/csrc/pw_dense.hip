@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "pw_internal.h"
 
@@ -181,6 +182,41 @@ static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool ve
   rc = read_info(c, dinfo, &info);
   if (rc) return rc;
   if (info != 0) return set_error(c, PW_ERR_ARG, "eigen: no convergence (" + std::to_string(info) + ")");
+  return PW_OK;
+}
+
+int pw_heevd_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, double* w) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || nbatch < 0 || (n > 0 && nbatch > 0 && (!A || !w)) || lda < (n > 0 ? n : 1) ||
+      strideA < lda * (int64_t)n)
+    return set_error(c, PW_ERR_ARG, "pw_heevd_batch_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  // eigenvalues, off-diagonals and one info per matrix
+  const size_t nd = 2 * (size_t)n * nbatch;
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, nd * sizeof(double) + (size_t)nbatch * sizeof(int));
+  if (rc) return rc;
+  double* D = reinterpret_cast<double*>(c->d_dwork);
+  double* E = D + (size_t)n * nbatch;
+  int* dinfo = reinterpret_cast<int*>(D + nd);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  rc = st_check(c, rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n,
+                                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
+                                                    (rocblas_stride)strideA, D, n, E, n, dinfo, nbatch),
+                "zheevd_strided_batched");
+  if (rc) return rc;
+  std::vector<int> info(nbatch);
+  hipError_t e = hipMemcpyAsync(w, D, (size_t)n * nbatch * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(info.data(), dinfo, (size_t)nbatch * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "batched eigenvalues download");
+  for (int i = 0; i < nbatch; ++i)
+    if (info[i] != 0)
+      return set_error(c, PW_ERR_ARG, "eigen: no convergence for matrix " + std::to_string(i) + " (" +
+                                          std::to_string(info[i]) + ")");
   return PW_OK;
 }
 

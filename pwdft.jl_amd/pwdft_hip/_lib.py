@@ -58,6 +58,7 @@ _PROTOS = {
     "pw_potrf_dev": [_vp, _i, _vp, _i64],
     "pw_trtri_dev": [_vp, _i, _vp, _i64],
     "pw_heevd_dev": [_vp, _i, _vp, _i64, _vp],
+    "pw_heevd_batch_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp],
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

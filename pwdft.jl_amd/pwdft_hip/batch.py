@@ -5,7 +5,9 @@ k-points of a few hundred plane waves and a handful of bands) every step of that
 tiny launch.  Here every step of one LOBPCG iteration is one batched launch for all k-points
 that have not converged yet (pw_op_H_batch_dev, pw_zgemm_batch_dev, pw_kprec_batch_dev,
 pw_copy_cols_batch_dev), and the small dense problems of all k-points (Nst..3Nst) are
-solved together as stacked numpy arrays.  A k-point leaves the batch when it meets the
+solved together: the Cholesky factors of the Gram matrices as stacked numpy arrays, the
+Hermitian eigenproblems of the Rayleigh-Ritz steps in one batched device call
+(pw_heevd_batch_dev).  A k-point leaves the batch when it meets the
 reference's convergence test; one that reaches the reference's lock branch finishes that
 iteration through the per-k code (solvers._lobpcg_lock_step).  The arithmetic per k-point is
 that of solvers.diag_LOBPCG_."""
@@ -160,8 +162,19 @@ def _chol_upper_inv(C, iks):
     return U, np.linalg.inv(U)
 
 
-def _geneig_lowest(T, G, n):
-    """Lowest n eigenvectors of T x = l G x for every stacked pair (eigen(T, G))."""
+def _eigh_dev(small, A):
+    """eigh of every stacked Hermitian matrix A (nk x n x n), upper triangle read, in one
+    batched device call (pw_heevd_batch_dev): (eigenvalues nk x n, eigenvectors nk x n x n)."""
+    nk, n, _ = A.shape
+    p = small.upload(A)
+    w = np.empty((nk, n))
+    small.ctx.call("pw_heevd_batch_dev", n, nk, ctypes.c_void_p(p[0]), n, n * n, ptr(w))
+    return w, small.download(nk, n, n)
+
+
+def _geneig_lowest(T, G, n, small):
+    """Lowest n eigenvectors of T x = l G x for every stacked pair (eigen(T, G)): G = L L^H
+    on the host, the standard problems L^-1 T L^-H of all k-points on the device."""
     try:
         L = np.linalg.cholesky(G)
     except np.linalg.LinAlgError as e:
@@ -169,7 +182,7 @@ def _geneig_lowest(T, G, n):
     Li = np.linalg.inv(L)
     A = Li @ T @ np.conj(np.swapaxes(Li, -1, -2))
     A = 0.5 * (A + np.conj(np.swapaxes(A, -1, -2)))
-    _, Y = np.linalg.eigh(A)
+    _, Y = _eigh_dev(small, A)
     return np.conj(np.swapaxes(Li, -1, -2)) @ Y[..., :n]
 
 
@@ -191,6 +204,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     Q = _Slab(c, rows, 3 * Nst)
     HQ = _Slab(c, rows, 3 * Nst)
     Tmp = _Slab(c, rows, 4 * Nst)
+    eig_small = _Small(c)
     allk = list(range(nk))
     cols = lambda S, ks, j: [S.col(k, j) for k in ks]
     rws = lambda ks: [rows[k] for k in ks]
@@ -252,7 +266,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         qa, hqa = cols(Q, a, 0), cols(HQ, a, 0)
         T = _herm_u(B.gram(qa, hqa, nq, nq, ra))
         G = _herm_u(B.gram(qa, qa, nq, nq, ra))
-        Um = _geneig_lowest(T, G, Nst)
+        Um = _geneig_lowest(T, G, Nst, eig_small)
         dU = B.small2.upload(Um)
         B.gemm(0, ra, Nst, nq, qa, ra, dU, nq, t0, ra)
         B.gemm(0, ra, Nst, nq, hqa, ra, dU, nq, t1, ra)
@@ -277,11 +291,11 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     x, hx = cols(Q, allk, 0), cols(HQ, allk, 0)
     S = B.gram(x, hx, Nst, Nst, rws(allk))
     S = _herm_u(0.5 * (S + np.conj(np.swapaxes(S, -1, -2))))
-    ev, Qm = np.linalg.eigh(S)
+    ev, Qm = _eigh_dev(eig_small, S)
     dQ = B.small2.upload(Qm)
     B.gemm(0, rws(allk), Nst, Nst, x, rws(allk), dQ, Nst, [psiks[k].p.value for k in allk], rws(allk))
     c.call("pw_synchronize")
-    for s in (Q, HQ, Tmp):
+    for s in (Q, HQ, Tmp, eig_small):
         s.free()
     B.free()
     stats = [{"iter": int(iters[k]), "ncols": int(ncols[k])} for k in allk]

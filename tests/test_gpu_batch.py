@@ -119,6 +119,39 @@ def test_zgemm_batch_vs_numpy():
             assert np.abs(Cd.download() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (transa, s)
 
 
+@pytest.mark.parametrize("n,nbatch", [(1, 3), (8, 182), (24, 182), (40, 5)])
+def test_heevd_batch_vs_numpy(n, nbatch):
+    """pw_heevd_batch_dev: the eigenvalues of every stacked Hermitian matrix equal numpy's,
+    the eigenvectors are orthonormal and satisfy A v = l v (only the upper triangle is read:
+    the strict lower triangle holds garbage)."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import Context, DeviceMatrix, PWError, ptr
+    c = Context(0)
+    rng = np.random.default_rng(n * 100 + nbatch)
+    A = rng.standard_normal((nbatch, n, n)) + 1j * rng.standard_normal((nbatch, n, n))
+    A = 0.5 * (A + np.conj(np.swapaxes(A, 1, 2)))
+    dev = np.array(A)
+    low = np.tril_indices(n, -1)
+    for k in range(nbatch):
+        dev[k][low] = 7.0 + 3.0j             # must be ignored
+    buf = np.ascontiguousarray(np.transpose(dev, (0, 2, 1)))   # column-major blocks
+    d = DeviceMatrix.from_host(c, buf.reshape(-1, 1))
+    w = np.empty((nbatch, n))
+    print(f"case heevd_batch n={n} nbatch={nbatch}", flush=True)
+    c.call("pw_heevd_batch_dev", n, nbatch, d.p, n, n * n, ptr(w))
+    V = np.transpose(d.download().reshape(nbatch, n, n), (0, 2, 1))
+    ref = np.linalg.eigvalsh(A)
+    assert np.abs(w - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    eye = np.eye(n)
+    for k in range(nbatch):
+        assert np.abs(V[k].conj().T @ V[k] - eye).max() < 1e-12
+        assert np.abs(A[k] @ V[k] - V[k] * w[k][None, :]).max() < 1e-11 * max(1.0, np.abs(ref).max())
+    with pytest.raises(PWError):
+        c.call("pw_heevd_batch_dev", n, nbatch, d.p, n, n * n - 1, ptr(w))
+    d.free()
+    c.close()
+
+
 def test_lobpcg_batch_vs_per_k(al):
     """diag_LOBPCG! over all 182 k-points in lockstep gives the per-k solver's eigenvalues."""
     import pwdft_hip as ph
