@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 A_SI64 = 20.5262
+A_SI = 10.2631
 NBANDS = 160
 PEAK_HBM_GBS = 8000.0
 
@@ -190,6 +191,8 @@ def time_batched_k(steps, warmup):
         print(f"# Al 182k {tag}: batched op_H {tb * 1e3:.3f} ms, per-k loop {tl * 1e3:.3f} ms",
               file=sys.stderr, flush=True)
     H.mixed = False
+    ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=1)     # warm-up: solver handle, workspaces
+    H.ctx.call("pw_synchronize")
     t0 = time.perf_counter()
     _, st = ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=4)
     H.ctx.call("pw_synchronize")
@@ -197,6 +200,66 @@ def time_batched_k(steps, warmup):
     res["lobpcg_batch"] = {"s_per_iter": dt / 4, "band_apps_per_s": sum(x["ncols"] for x in st) / dt}
     print(f"# Al 182k batched LOBPCG: {dt / 4 * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
     return res
+
+
+SI8_SHIFT = np.array([[0.041, -0.027, 0.033, -0.012, 0.018, -0.045, 0.006, 0.029],
+                      [-0.036, 0.014, 0.022, -0.049, 0.031, 0.008, -0.019, 0.043],
+                      [0.025, 0.047, -0.038, 0.011, -0.023, 0.035, -0.044, -0.007]])
+
+
+def forces_leg():
+    """Si8 cubic (a = 10.2631 bohr, GTH LDA, 20 Ha, 2x2x2 k-points, 20 bands), every atom
+    moved by a fixed offset <= 0.05 bohr: KS_solve_SCF! to 1e-9 Ha in FP64 and in mixed
+    precision (FP32 until |dE| < 1e-6, then FP64), the total forces of both and their
+    largest difference, with the SCF seconds per iteration."""
+    import pwdft_hip as ph
+    L = ph.gen_lattice_sc(A_SI)
+    frac = np.array(ph.diamond_supercell(1.0, 1), dtype=float)
+    frac = np.linalg.solve(L, L @ frac.T + SI8_SHIFT).T
+    res = {"workload": "Si8 sc a=10.2631, GTH LDA, 20 Ha, 2x2x2 k-points (no symmetry), 20 bands, "
+                       "atoms displaced <= 0.05 bohr"}
+    F = {}
+    E = {}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        H = ph.Hamiltonian(ph.Atoms(["Si"] * 8, frac, L), ["Si-q4.lda.gth"], 20.0, meshk=(2, 2, 2), Nstates=20,
+                           use_symmetry=False, mixed=mixed)
+        t0 = time.perf_counter()
+        out = ph.KS_solve_SCF_(H, betamix=0.3, etot_conv_thr=1e-9, ethr_evals_last=1e-10, NiterMax=80,
+                               mixed_until=1e-6 if mixed else None)
+        dt = time.perf_counter() - t0
+        rho = ph.calc_rhoe(H, out["psiks"])
+        F[tag] = ph.calc_forces(H, out["psiks"], rho)
+        E[tag] = out["energies"]["Total"]
+        res[tag] = {"s_per_iter": dt / out["iterations"], "iterations": out["iterations"], "Etot": E[tag],
+                    "max_abs_force": float(np.abs(F[tag]).max())}
+        H.ctx.close()
+    res["dE_per_atom_mixed_vs_fp64"] = abs(E["mixed"] - E["fp64"]) / 8
+    res["max_dF_mixed_vs_fp64"] = float(np.abs(F["mixed"] - F["fp64"]).max())
+    print(f"# Si8 forces: dE/atom {res['dE_per_atom_mixed_vs_fp64']:.2e}, max|dF| {res['max_dF_mixed_vs_fp64']:.2e}, "
+          f"SCF {res['fp64']['s_per_iter'] * 1e3:.1f} / {res['mixed']['s_per_iter'] * 1e3:.1f} ms/iteration",
+          file=sys.stderr, flush=True)
+    return res
+
+
+def stages(prof, H, mixed, steps):
+    """Achieved rate of every op_H stage: algorithmic GB/s (HBM peak 8 TB/s) for the FFT
+    kernels, TFLOP/s (MFMA peak FP32 157.3 / FP64 78.6) for the projector products."""
+    out = {}
+    nb, ngw, nbeta = NBANDS, H.pw.Ngw[0], H.pspotNL.NbetaNL
+    for k, (ms, n, b) in prof.items():
+        if not n:
+            continue
+        us = ms / n * 1e3
+        if k.startswith("gemm"):
+            fl = 8.0 * ngw * nbeta * nb
+            a = fl / (us * 1e-6) / 1e12
+            pk = 157.3 if mixed else 78.6
+            out[k] = {"us": us, "achieved": a, "unit": "TFLOP/s", "peak": pk, "frac": a / pk}
+        else:
+            a = b / n / (us * 1e-6) / 1e9
+            out[k] = {"us": us, "achieved": a, "unit": "GB/s", "peak": 8000.0, "frac": a / 8000.0}
+    return out
 
 
 def roofline(prof, H, mixed):
@@ -245,6 +308,8 @@ def main():
     ap.add_argument("--scf-iters", type=int, default=3, help="SCF iterations timed per precision (0 = skip)")
     ap.add_argument("--no-batched-k", dest="batched_k", action="store_false",
                     help="skip the 182-k-point batched op_H leg")
+    ap.add_argument("--no-forces", dest="forces", action="store_false",
+                    help="skip the Si8 mixed-vs-FP64 force leg")
     ap.add_argument("--no-overlap", dest="overlap", action="store_false",
                     help="run op_H's nonlocal products on the main stream (kernels in sequence), "
                          "e.g. under a kernel-trace profiler")
@@ -306,6 +371,7 @@ def main():
             scf[tag] = time_scf(H, args.scf_iters, mixed)
             print(f"# SCF {tag}: {scf[tag]['s_per_iter']:.3f} s/iteration", file=sys.stderr, flush=True)
     batched = time_batched_k(args.steps, args.warmup) if args.batched_k else None
+    forces = forces_leg() if args.forces else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "Hpsi band-applications/s (mixed precision; FP64 beside it)",
@@ -334,6 +400,8 @@ def main():
         "scf_s_per_iter": scf or None,
         "kernels_us_per_step": {t: {k: v[0] * 1e3 / args.steps for k, v in res[t]["prof"].items() if v[1]}
                                 for t in ("fp64", "mixed")},
+        "stages": {t: stages(res[t]["prof"], H, t == "mixed", args.steps) for t in ("fp64", "mixed")},
+        "forces_si8": forces,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

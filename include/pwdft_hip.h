@@ -69,7 +69,9 @@ enum {
   PW_TUNE_GEMM_NN_WG = 3,     /* A B split-K target workgroup count (0 = default 512) */
   PW_TUNE_BAND_BATCH = 4,     /* bands per fused FFT launch (same as pw_set_band_batch) */
   PW_TUNE_PLANE_COMPRESSED = 5, /* 1: always use the compressed z-plane kernels (they are used
-                                   anyway when the padded plane does not fit in LDS) */
+                                   anyway when the padded plane does not fit in LDS); 2: always
+                                   the global-plane pass (compressed planes in HBM, used anyway
+                                   when even the compressed plane exceeds LDS) */
   PW_TUNE_OVERLAP = 6           /* 1 (default): op_H runs its nonlocal products on a second
                                    stream beside the FFT passes; 0: one stream, in sequence */
 };

@@ -298,6 +298,7 @@ int pw_ctx_destroy(pw_ctx* c) {
   for (auto p : c->d_v64) hipFree(p);
   for (auto p : c->d_v32) hipFree(p);
   hipFree(c->d_work);
+  hipFree(c->d_yglob);
   hipFree(c->d_gpart);
   hipFree(c->d_bnl);
   hipFree(c->d_rpart);
@@ -348,7 +349,8 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
     case PW_TUNE_BAND_BATCH:
       return pw_set_band_batch(c, value);
     case PW_TUNE_PLANE_COMPRESSED:
-      c->plane_compressed = value != 0;
+      if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: plane mode must be 0..2");
+      c->plane_compressed = value;
       return PW_OK;
     case PW_TUNE_OVERLAP:
       c->overlap = value != 0;

@@ -637,6 +637,76 @@ k_plane_g2r_c(const cplx<double>* __restrict__ W, double* __restrict__ out, long
     plane_rows_c<double, 1>(Yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, o, scale);
 }
 
+// 2g. Planes whose compressed form does not fit in LDS either (FP64 on the rho sphere at
+// 108^2: 107 active x lines of 108): the compressed plane of each (z, band) lives in global
+// memory (Yg: nxact lines of stride ys), and the plane pass is split in three launches.
+// k_yfft_gp transforms groups of `glines` active x lines along y in LDS: with S = +1 the
+// lines are scattered from W first (start of G->R), with S = -1 they are loaded from Yg and
+// their sphere entries gathered back into W after the transform (end of R->G).  k_xrows_gp
+// runs plane_rows_c on the rows of Yg (row buffers in LDS): x-inverse, * V/N, x-forward
+// written back to Yg (RM < 0), or x-inverse and scale Re f(r) into out (RM 1 write, 2 add).
+template <typename T, int S>
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_yfft_gp(cplx<T>* __restrict__ W, cplx<T>* __restrict__ Yg, const int* __restrict__ col_cy, int ncol, int nxact,
+          int glines, FftDim dy, const cplx<T>* __restrict__ twy, int n3) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n2 = dy.n, ys = odd_stride(n2);
+  const int z = blockIdx.x, b = blockIdx.y, a0 = blockIdx.z * glines;
+  const int nl = min(glines, nxact - a0);
+  if (nl <= 0) return;
+  cplx<T>* ty = lds + glines * ys;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  cplx<T>* w = W + ((long)b * n3 + z) * ncol;
+  cplx<T>* yg = Yg + ((long)b * n3 + z) * nxact * ys + (long)a0 * ys;
+  const int s0 = a0 * ys, s1 = (a0 + nl) * ys;
+  stage(ty, twy, n2);
+  if (S > 0) {
+    for (int i = tid; i < nl * ys; i += nthr) lds[i] = mk((T)0, (T)0);
+    __syncthreads();
+    for (int c = tid; c < ncol; c += nthr) {
+      const int s = col_cy[c];
+      if (s >= s0 && s < s1) lds[s - s0] = w[c];
+    }
+  } else {
+    for (int i = tid; i < nl * ys; i += nthr) lds[i] = yg[i];
+  }
+  __syncthreads();
+  wave_lines<T, S, FFT_ME>(lds, nl, nullptr, ys, 1, dy, ty);
+  __syncthreads();
+  if (S > 0) {
+    for (int i = tid; i < nl * ys; i += nthr) yg[i] = lds[i];
+  } else {
+    for (int c = tid; c < ncol; c += nthr) {
+      const int s = col_cy[c];
+      if (s >= s0 && s < s1) w[c] = lds[s - s0];
+    }
+  }
+}
+
+template <typename T, int RM>
+__global__ void __launch_bounds__(PlaneNT<T>::value)
+k_xrows_gp(cplx<T>* __restrict__ Yg, const T* __restrict__ V, double* __restrict__ out, long N, double scale,
+           const int* __restrict__ xact, int nxact, FftDim dx, int n2, const cplx<T>* __restrict__ twx, int n3,
+           int rl) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n1 = dx.n, ys = odd_stride(n2);
+  cplx<T>* tx = lds;
+  cplx<T>* rows = tx + n1;
+  int* xl = reinterpret_cast<int*>(rows + (blockDim.x >> 6) * rl * n1);
+  const int z = blockIdx.x, b = blockIdx.y;
+  stage(tx, twx, n1);
+  stage(xl, xact, nxact);
+  __syncthreads();
+  cplx<T>* yc = Yg + ((long)b * n3 + z) * nxact * ys;
+  if constexpr (RM < 0)
+    plane_rows_c<T>(yc, ys, rows, xl, nxact, dx, n2, tx, V + (long)z * n1 * n2, rl, nullptr, 0.0);
+  else
+    plane_rows_c<T, RM>(yc, ys, rows, xl, nxact, dx, n2, tx, nullptr, rl, out + (long)b * N + (long)z * n1 * n2,
+                        scale);
+}
+
 // 3. forward z-FFT of the columns, gather the sphere, add kinetic, write Hpsi (one band:
 // w its block of W, p / h its psi / Hpsi columns).
 template <typename T>
@@ -843,6 +913,26 @@ static int plane_c_rows(const pw_ctx* c, size_t esz, int nxact, int nthreads, si
   return rl;
 }
 
+// Global-plane pass (k_yfft_gp / k_xrows_gp) for planes whose compressed form exceeds LDS:
+// y lines per group (at most 64 KiB of lines) and the x pass's rows in flight per wavefront,
+// with the LDS bytes of each launch.
+struct PlaneGP {
+  int glines = 0, rl = 0;
+  size_t ybytes = 0, xbytes = 0;
+};
+static bool plane_gp(const pw_ctx* c, size_t esz, int nxact, int nthreads, PlaneGP* g) {
+  const size_t line = (size_t)odd_stride(c->n2) * esz;
+  g->glines = std::max(1, std::min(nxact, (int)((64 * 1024) / line)));
+  g->ybytes = (size_t)g->glines * line + (size_t)c->n2 * esz;
+  const size_t fixed = (size_t)c->n1 * esz + (size_t)nxact * sizeof(int);
+  const size_t per = (size_t)(nthreads / 64) * c->n1 * esz;
+  const size_t lim = 160 * 1024;
+  int rl = fixed < lim ? (int)((lim - fixed) / per) : 0;
+  g->rl = std::min(rl, c->dim[0].gpw);
+  g->xbytes = fixed + (size_t)std::max(g->rl, 0) * per;
+  return g->rl >= 1 && g->ybytes <= lim;
+}
+
 template <typename T>
 static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>* psi, long ldpsi,
                     cplx<double>* hpsi, long ldh, int kin, int acc, hipEvent_t join = nullptr) {
@@ -850,10 +940,16 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
   const int C = cols_per_block(c, esz);
   size_t plane_bytes = plane_lds(c, esz, k.nxact);
   int rl = 0;
-  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed != 0;
+  PlaneGP gp;
+  bool global_plane = false;
   if (compressed) {
     rl = plane_c_rows(c, esz, k.nxact, PlaneNT<T>::value, &plane_bytes);
-    if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
+    if (rl < 1 || c->plane_compressed == 2) {
+      if (!plane_gp(c, esz, k.nxact, PlaneNT<T>::value, &gp))
+        return set_error(c, PW_ERR_ARG, "z-plane does not fit in LDS for this precision");
+      global_plane = true;
+    }
   }
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
@@ -862,6 +958,12 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
   int rc = ensure(c, &c->d_work, &c->work_bytes, wbytes);
   if (rc) return rc;
   cplx<T>* W = reinterpret_cast<cplx<T>*>(c->d_work);
+  cplx<T>* Yg = nullptr;
+  if (global_plane) {
+    rc = ensure(c, &c->d_yglob, &c->yglob_bytes, (size_t)bb * c->n3 * k.nxact * odd_stride(c->n2) * esz);
+    if (rc) return rc;
+    Yg = reinterpret_cast<cplx<T>*>(c->d_yglob);
+  }
   const cplx<T>* tw[3];
   for (int d = 0; d < 3; ++d)
     tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
@@ -879,7 +981,15 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
     {
       dim3 g2(c->n3, nbb);
       ProfScope ps(c, PW_PROF_PLANE_VLOC, 2.0 * wb + (double)c->N * sizeof(T));
-      if (compressed)
+      if (global_plane) {
+        const dim3 gy(c->n3, nbb, (k.nxact + gp.glines - 1) / gp.glines);
+        hipLaunchKernelGGL((k_yfft_gp<T, +1>), gy, dim3(PlaneNT<T>::value), gp.ybytes, c->stream, W, Yg, k.d_col_cy,
+                           k.ncol, k.nxact, gp.glines, c->dim[1], tw[1], c->n3);
+        hipLaunchKernelGGL((k_xrows_gp<T, -1>), g2, dim3(PlaneNT<T>::value), gp.xbytes, c->stream, Yg, V, nullptr,
+                           0L, 0.0, k.d_xact, k.nxact, c->dim[0], c->n2, tw[0], c->n3, gp.rl);
+        hipLaunchKernelGGL((k_yfft_gp<T, -1>), gy, dim3(PlaneNT<T>::value), gp.ybytes, c->stream, W, Yg, k.d_col_cy,
+                           k.ncol, k.nxact, gp.glines, c->dim[1], tw[1], c->n3);
+      } else if (compressed)
         hipLaunchKernelGGL(k_plane_vloc_c<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
                            k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
       else
@@ -1008,7 +1118,7 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const int np = c->n1 * c->n2;
   size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
   int rl = 0;
-  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed != 0;
   if (compressed) {
     rl = plane_c_rows(c, esz, k.nxact, PlaneNT<T>::value, &plane_bytes);
     if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
@@ -1054,10 +1164,16 @@ int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, dou
   const int C = cols_per_block(c, esz);
   size_t plane_bytes = plane_lds(c, esz, k.nxact);
   int rl = 0;
-  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed;
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed != 0;
+  PlaneGP gp;
+  bool global_plane = false;
   if (compressed) {
     rl = plane_c_rows(c, esz, k.nxact, PlaneNT<double>::value, &plane_bytes);
-    if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane too large for the G->R kernel");
+    if (rl < 1 || c->plane_compressed == 2) {
+      if (!plane_gp(c, esz, k.nxact, PlaneNT<double>::value, &gp))
+        return set_error(c, PW_ERR_ARG, "z-plane too large for the G->R kernel");
+      global_plane = true;
+    }
   }
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
@@ -1065,6 +1181,12 @@ int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, dou
   int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)bb * c->n3 * k.ncol * esz);
   if (rc) return rc;
   cplx<double>* W = reinterpret_cast<cplx<double>*>(c->d_work);
+  cplx<double>* Yg = nullptr;
+  if (global_plane) {
+    rc = ensure(c, &c->d_yglob, &c->yglob_bytes, (size_t)bb * c->n3 * k.nxact * odd_stride(c->n2) * esz);
+    if (rc) return rc;
+    Yg = reinterpret_cast<cplx<double>*>(c->d_yglob);
+  }
   const cplx<double>* tw[3];
   for (int d = 0; d < 3; ++d) tw[d] = reinterpret_cast<const cplx<double>*>(c->d_tw64[d]);
   for (int b0 = 0; b0 < nf; b0 += bb) {
@@ -1073,7 +1195,19 @@ int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, dou
                        c->stream, fG, ldf, b0, W, k.d_ent_igw, k.d_ent_pos, k.d_col_ptr, k.ncol, C, nbb,
                        c->dim[2], tw[2]);
     double* o = out + (long)b0 * c->N;
-    if (compressed)
+    if (global_plane) {
+      const dim3 gy(c->n3, nbb, (k.nxact + gp.glines - 1) / gp.glines);
+      hipLaunchKernelGGL((k_yfft_gp<double, +1>), gy, dim3(PlaneNT<double>::value), gp.ybytes, c->stream, W, Yg,
+                         k.d_col_cy, k.ncol, k.nxact, gp.glines, c->dim[1], tw[1], c->n3);
+      if (acc)
+        hipLaunchKernelGGL((k_xrows_gp<double, 2>), dim3(c->n3, nbb), dim3(PlaneNT<double>::value), gp.xbytes,
+                           c->stream, Yg, nullptr, o, c->N, scale, k.d_xact, k.nxact, c->dim[0], c->n2, tw[0],
+                           c->n3, gp.rl);
+      else
+        hipLaunchKernelGGL((k_xrows_gp<double, 1>), dim3(c->n3, nbb), dim3(PlaneNT<double>::value), gp.xbytes,
+                           c->stream, Yg, nullptr, o, c->N, scale, k.d_xact, k.nxact, c->dim[0], c->n2, tw[0],
+                           c->n3, gp.rl);
+    } else if (compressed)
       hipLaunchKernelGGL(k_plane_g2r_c, dim3(c->n3, nbb), dim3(PlaneNT<double>::value), plane_bytes, c->stream, W,
                          o, c->N, scale, acc, k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0],
                          tw[1], c->n3, rl);

@@ -166,6 +166,7 @@ struct pw_ctx {
   std::vector<double> h_D;     // nbeta x nbeta column-major
   // workspaces (grow on demand)
   void* d_work = nullptr;  size_t work_bytes = 0;     // FFT column/plane intermediates
+  void* d_yglob = nullptr; size_t yglob_bytes = 0;    // compressed planes beyond LDS (global-plane pass)
   void* d_gpart = nullptr; size_t gpart_bytes = 0;    // GEMM split-K partials
   void* d_bnl = nullptr;   size_t bnl_bytes = 0;      // beta^H psi
   void* d_rpart = nullptr; size_t rpart_bytes = 0;    // rho partial planes
@@ -173,7 +174,8 @@ struct pw_ctx {
   void* d_tmp = nullptr;   size_t tmp_bytes = 0;      // staging for host-pointer entry points
   void* d_wts = nullptr;   size_t wts_bytes = 0;      // band weights of calc_rhoe
   int band_batch = 64;
-  bool plane_compressed = false;  // force the compressed plane kernels (tuning / tests)
+  // 1: force the compressed plane kernels, 2: the global-plane pass (tuning / tests)
+  int plane_compressed = 0;
   // tuning (pw_tune_set): [0] = A^H B products, [1] = A B products
   // -1: by N (pw_gemm.hip: zgemm_t); 0: 2x2 wavefronts (64x64 tile), 1: 4x2, 2: 2x4, 3: 4x4,
   // 4: 4x1 (128 x 32), 5: 2x1 (64 x 32)

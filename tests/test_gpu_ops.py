@@ -173,11 +173,13 @@ def test_betaNL_psi(si_pair):
     assert rel(got, O.calc_betaNL_psi(oH.pspotNL.betaNL[3], psi)) < TOL64
 
 
-@pytest.mark.parametrize("case", ["al_fcc_20", "si8_45", "si8_72", "si8_72_compressed", "si8_108"])
+@pytest.mark.parametrize("case", ["al_fcc_20", "si8_45", "si8_72", "si8_72_compressed", "si8_72_global",
+                                  "si8_108"])
 def test_other_grids(case):
     """Grids with other radix mixes: 20 = 5*4, 45 = 9*5, 72 = 9*8, 108 = 9*4*3 (Ns_ override).
     At 108 the padded FP64 z-plane (188 KB) exceeds LDS and the compressed plane kernels
-    run; "si8_72_compressed" forces them where the padded plane would fit.  calc_rhoe is
+    run; "si8_72_compressed" forces them where the padded plane would fit, and
+    "si8_72_global" forces the global-plane pass (compressed planes in HBM).  calc_rhoe is
     checked on the same grids."""
     import pwdft_hip as ph
     from oracle import ops as O
@@ -202,6 +204,8 @@ def test_other_grids(case):
     H = product_ham(sym, frac, Lp, psp, ecut, **kwp)
     if case == "si8_72_compressed":
         H.ctx.call("pw_tune_set", 5, 1)
+    if case == "si8_72_global":
+        H.ctx.call("pw_tune_set", 5, 2)
     rng = np.random.default_rng(4)
     V = oH.V_Ps_loc + 0.2 * rng.standard_normal(oH.pw.Npoints)
     oH.V_loc[:, 0] = V

@@ -58,9 +58,12 @@ def test_update_potential_vs_oracle(name, make, xc):
         assert abs(got - want) < 1e-11 * max(abs(want), 1.0), (E, ref)
 
 
-def test_update_potential_compressed_planes():
-    """The same update with the compressed plane kernels forced on a grid whose planes fit
-    in LDS gives the uncompressed result to rounding."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_update_potential_compressed_planes(mode):
+    """The same update (PBE: R->G and G->R of several fields) and V_Ps_loc build with the
+    compressed plane kernels (mode 1) or the global-plane pass (mode 2, compressed planes
+    in HBM, three launches per plane pass) forced on a grid whose planes fit in LDS give the
+    result of the in-LDS kernels to rounding."""
     import pwdft_hip as ph
     from pwdft_hip._lib import PW_TUNE_PLANE_COMPRESSED
     sym, frac = si_fcc_structure()
@@ -68,13 +71,46 @@ def test_update_potential_compressed_planes():
     rho = _test_density(H, 3)
     E0 = H.update(rho)
     V0 = H.potentials.Total[:, 0].copy()
-    H.ctx.call("pw_tune_set", PW_TUNE_PLANE_COMPRESSED, 1)
+    P0 = H.potentials.Ps_loc.copy()
+    print(f"case update plane mode {mode}", flush=True)
+    H.ctx.call("pw_tune_set", PW_TUNE_PLANE_COMPRESSED, mode)
     try:
+        H._build_geometry()
         E1 = H.update(rho)
     finally:
         H.ctx.call("pw_tune_set", PW_TUNE_PLANE_COMPRESSED, 0)
+    assert rel(H.potentials.Ps_loc, P0) < 1e-14
     assert rel(H.potentials.Total[:, 0], V0) < 1e-14
     assert np.allclose(E0, E1, rtol=1e-13, atol=0)
+
+
+def test_si216_pbe_setup_and_update_vs_oracle():
+    """The Si216 PBE configuration (simple cubic a = 30.789 bohr, atoms moved by up to 0.02
+    bohr, 15 Ha, 108^3 grid): its rho-sphere planes exceed LDS even compressed (107 active x
+    lines of 108 in FP64), so V_Ps_loc, R->G and G->R run the global-plane pass.  V_Ps_loc and
+    the potentials of update(guess density) against the oracle at the full size."""
+    import pwdft_hip as ph
+    from oracle.density import Poisson_solve, calc_epsxc_Vxc_PBE
+    from oracle.setup import Atoms, PsPot_GTH, calc_V_Ps_loc
+    import os
+    frac = np.array(ph.diamond_supercell(1.0, 3), dtype=float)
+    L = ph.gen_lattice_sc(30.789)
+    cart = L @ frac.T + np.random.default_rng(216).uniform(-0.02, 0.02, size=(3, len(frac)))
+    print("case Si216 PBE 108^3 set-up", flush=True)
+    H = ph.Hamiltonian(ph.Atoms(["Si"] * len(frac), np.linalg.solve(L, cart).T, L), ["Si-q4.pbe.gth"], 15.0,
+                       Ns_=(108, 108, 108), Nstates=480, xcfunc="PBE", use_symmetry=False, band_batch=96)
+    assert H.pw.Ns == (108, 108, 108)
+    opw = _oracle_pw(H)
+    oat = Atoms(["Si"] * len(frac), cart.T, L, frac=False)
+    Vps = calc_V_Ps_loc(oat, opw, [PsPot_GTH(os.path.join(PSP, "Si-q4.pbe.gth"))])
+    assert rel(H.potentials.Ps_loc, Vps) < 1e-12, rel(H.potentials.Ps_loc, Vps)
+    rho = H.guess_rhoe()
+    print("case Si216 PBE update", flush=True)
+    H.update(rho)
+    VH = Poisson_solve(opw, rho)
+    _, Vxc = calc_epsxc_Vxc_PBE(opw, rho)
+    assert rel(H.potentials.Hartree, VH) < 1e-12, rel(H.potentials.Hartree, VH)
+    assert rel(H.potentials.XC[:, 0], Vxc) < 1e-11, rel(H.potentials.XC[:, 0], Vxc)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
