@@ -242,6 +242,54 @@ def forces_leg():
     return res
 
 
+def si216_leg():
+    """Si216 (simple cubic a = 30.789 bohr, atoms moved by up to 0.02 bohr, GTH PBE, 15 Ha,
+    108^3 grid, Gamma, 480 bands): op_H over all bands, diag_LOBPCG! iterations and the first
+    two KS_solve_SCF! iterations, FP64 and mixed."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import PW_MIXED, PW_TERM_ALL
+    from pwdft_hip.solvers import diag_LOBPCG_
+    frac = np.array(ph.diamond_supercell(1.0, 3), dtype=float)
+    L = ph.gen_lattice_sc(30.789)
+    cart = L @ frac.T + np.random.default_rng(216).uniform(-0.02, 0.02, size=(3, len(frac)))
+    H = ph.Hamiltonian(ph.Atoms(["Si"] * len(frac), np.linalg.solve(L, cart).T, L), ["Si-q4.pbe.gth"], 15.0,
+                       Ns_=(108, 108, 108), Nstates=480, xcfunc="PBE", use_symmetry=False, band_batch=96)
+    H.update(H.guess_rhoe())
+    X = ph.rand_BlochWavefunc(H, seed=1)[0]
+    Y = ph.DeviceMatrix(H.ctx, X.rows, X.cols)
+    res = {"workload": "Si216 sc a=30.789, GTH PBE, 15 Ha, 108^3 grid, Gamma, 480 bands",
+           "Ngw": int(H.pw.Ngw[0]), "Nbeta": int(H.pspotNL.NbetaNL)}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        fl = PW_TERM_ALL | (PW_MIXED if mixed else 0)
+        H.ctx.call("pw_op_H_dev", 0, 0, 480, X.p, X.rows, Y.p, Y.rows, fl)
+        H.ctx.call("pw_synchronize")
+        t = time.perf_counter()
+        for _ in range(3):
+            H.ctx.call("pw_op_H_dev", 0, 0, 480, X.p, X.rows, Y.p, Y.rows, fl)
+        H.ctx.call("pw_synchronize")
+        dt = (time.perf_counter() - t) / 3
+        H.mixed = mixed
+        Xc = ph.DeviceMatrix(H.ctx, X.rows, X.cols)
+        H.ctx.call("pw_copy_cols_dev", X.rows, X.cols, X.p, X.rows, Xc.p, Xc.rows)
+        diag_LOBPCG_(H, Xc, tol=0.0, NiterMax=1)
+        H.ctx.call("pw_copy_cols_dev", X.rows, X.cols, X.p, X.rows, Xc.p, Xc.rows)
+        H.ctx.call("pw_synchronize")
+        t = time.perf_counter()
+        _, st = diag_LOBPCG_(H, Xc, tol=0.0, NiterMax=3)
+        dl = (time.perf_counter() - t) / 3
+        Xc.free()
+        t = time.perf_counter()
+        out = ph.KS_solve_SCF_(H, betamix=0.3, NiterMax=2, etot_conv_thr=1e-12)
+        ds = (time.perf_counter() - t) / out["iterations"]
+        res[tag] = {"op_H_ms": dt * 1e3, "band_apps_per_s": 480 / dt, "lobpcg_s_per_iter": dl,
+                    "scf_s_per_iter_first2": ds}
+        print(f"# Si216 {tag}: op_H {dt * 1e3:.1f} ms, LOBPCG {dl * 1e3:.0f} ms/iteration, "
+              f"SCF {ds:.2f} s/iteration", file=sys.stderr, flush=True)
+    H.mixed = False
+    return res
+
+
 def stages(prof, H, mixed, steps):
     """Achieved rate of every op_H stage: algorithmic GB/s (HBM peak 8 TB/s) for the FFT
     kernels, TFLOP/s (MFMA peak FP32 157.3 / FP64 78.6) for the projector products."""
@@ -308,6 +356,8 @@ def main():
     ap.add_argument("--scf-iters", type=int, default=3, help="SCF iterations timed per precision (0 = skip)")
     ap.add_argument("--no-batched-k", dest="batched_k", action="store_false",
                     help="skip the 182-k-point batched op_H leg")
+    ap.add_argument("--si216", action="store_true",
+                    help="also run the Si216 PBE leg (108^3, 480 bands; adds about a minute)")
     ap.add_argument("--no-forces", dest="forces", action="store_false",
                     help="skip the Si8 mixed-vs-FP64 force leg")
     ap.add_argument("--no-overlap", dest="overlap", action="store_false",
@@ -372,6 +422,7 @@ def main():
             print(f"# SCF {tag}: {scf[tag]['s_per_iter']:.3f} s/iteration", file=sys.stderr, flush=True)
     batched = time_batched_k(args.steps, args.warmup) if args.batched_k else None
     forces = forces_leg() if args.forces else None
+    si216 = si216_leg() if args.si216 else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "Hpsi band-applications/s (mixed precision; FP64 beside it)",
@@ -402,6 +453,7 @@ def main():
                                 for t in ("fp64", "mixed")},
         "stages": {t: stages(res[t]["prof"], H, t == "mixed", args.steps) for t in ("fp64", "mixed")},
         "forces_si8": forces,
+        "si216": si216,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
