@@ -72,8 +72,10 @@ enum {
                                    anyway when the padded plane does not fit in LDS); 2: always
                                    the global-plane pass (compressed planes in HBM, used anyway
                                    when even the compressed plane exceeds LDS) */
-  PW_TUNE_OVERLAP = 6           /* 1 (default): op_H runs its nonlocal products on a second
+  PW_TUNE_OVERLAP = 6,          /* 1 (default): op_H runs its nonlocal products on a second
                                    stream beside the FFT passes; 0: one stream, in sequence */
+  PW_TUNE_COL_CHUNK = 7         /* z-columns per workgroup of the column FFT kernels (0: the
+                                   default, as many as 48 KiB of LDS hold) */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

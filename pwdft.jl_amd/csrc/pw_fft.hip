@@ -886,7 +886,9 @@ __global__ void k_kprec(const cplx<double>* __restrict__ in, long ldin, cplx<dou
 static int cols_per_block(const pw_ctx* c, size_t esz) {
   // column tile <= 48 KiB of LDS, at least one full round of line groups
   const FftDim& d = c->dim[2];
-  int C = (int)((48 * 1024) / (esz * odd_stride(d.n)));
+  const int cap = (int)((48 * 1024) / (esz * odd_stride(d.n)));
+  if (c->tune_col_chunk > 0) return std::max(1, std::min(c->tune_col_chunk, cap));
+  int C = cap;
   const int round = d.gpw * (COL_NT / 64);
   if (C > round) C = (C / round) * round;
   if (C < 1) C = 1;
