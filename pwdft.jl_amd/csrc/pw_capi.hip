@@ -352,6 +352,9 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: plane mode must be 0..2");
       c->plane_compressed = value;
       return PW_OK;
+    case PW_TUNE_GEMM_SPLIT_BF16:
+      c->gemm_split_bf16 = value != 0;
+      return PW_OK;
     case PW_TUNE_COL_CHUNK:
       if (value < 0) return set_error(c, PW_ERR_ARG, "pw_tune_set: column chunk must be >= 0");
       c->tune_col_chunk = value;

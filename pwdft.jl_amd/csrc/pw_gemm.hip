@@ -220,6 +220,188 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
       }
 }
 
+// FP32-accurate complex GEMM on BF16 MFMA (the mixed path's alternative to the FP32 MFMA
+// kernel above).  Every real operand x is split into three BF16 parts x0 + x1 + x2 (x0 =
+// bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1): 24 significant bits, as FP32), and
+// a b = sum of the six products with a combined order above 2^-24 (a0 b0, a0 b1, a1 b0, a0 b2,
+// a1 b1, a2 b0), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The parts are formed
+// once per element when a stage is written to LDS, with the 2x2 real form of op(A) already
+// applied (no sign work in the MFMA loop).  Lane l feeds row (or column) l % 16 with the 8
+// consecutive real k of slot block l / 16; the accumulator layout equals that of the FP32
+// 16x16x4 MFMA, so the tiles, split-K and epilogue are those of k_zgemm.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr int HBK = 16;          // complex k per stage = 32 real k = one MFMA k step
+constexpr int HKS = 2 * HBK + 8; // LDS row stride in bf16 (16-byte aligned rows, padded)
+
+__device__ inline void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
+  h0 = (__bf16)x;
+  const float r1 = x - (float)h0;
+  h1 = (__bf16)r1;
+  h2 = (__bf16)(r1 - (float)h1);
+}
+
+template <typename TA, typename TB, int TRANSA, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
+k_zgemm_b3(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cplx<TB>* __restrict__ B,
+           long ldb, int kchunk, cplx<double>* __restrict__ C, long ldc, cplx<double> alpha, cplx<double> beta,
+           int direct, int use_beta, cplx<double>* __restrict__ part, int tm, int tn, int nz) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BMC = 32 * WM, BNC = 32 * WN;
+  constexpr int NA = (BMC * HBK) / NT, NB = (BNC * HBK) / NT;
+  static_assert(NA >= 1 && NB >= 1 && (BMC * HBK) % NT == 0 && (BNC * HBK) % NT == 0, "tile");
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][2 * BMC][HKS];  // [part][real row][real k]
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BNC][HKS];      // [part][column][real k]
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  int mt, nt, z;
+  tile_of(blockIdx.x, tm, tn, nz, mt, nt, z);
+  if (z >= nz || mt >= tm || nt >= tn) return;
+  const int m0 = mt * BMC, n0 = nt * BNC;
+  const int kbeg = z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  v4f acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0f;
+  const int li = lane & 15, lg = lane >> 4;
+
+  cplx<TA> ra[NA];
+  cplx<TB> rb[NB];
+  unsigned va = 0, vb = 0;
+  auto a_idx = [&](int s, int& mm, int& kk) {
+    if (TRANSA == 0) { mm = tid % BMC; kk = tid / BMC + (NT / BMC) * s; }
+    else { kk = tid % HBK; mm = tid / HBK + (NT / HBK) * s; }
+  };
+  auto b_idx = [&](int s, int& kb, int& nn) { kb = tid % HBK; nn = tid / HBK + (NT / HBK) * s; };
+  auto load_stage = [&](int k0) {
+    va = 0;
+    vb = 0;
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      int mm, kk;
+      a_idx(s, mm, kk);
+      const int gm = m0 + mm, gk = k0 + kk;
+      const bool ok = gm < M && gk < kend;
+      ra[s] = A[ok ? (TRANSA == 0 ? (long)gk * lda + gm : (long)gm * lda + gk) : 0];
+      va |= (unsigned)ok << s;
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      int kb, nn;
+      b_idx(s, kb, nn);
+      const int gkb = k0 + kb, gn = n0 + nn;
+      const bool ok = gkb < kend && gn < N;
+      rb[s] = B[ok ? (long)gn * ldb + gkb : 0];
+      vb |= (unsigned)ok << s;
+    }
+  };
+  // 2x2 real form of op(A)[m][k] = (x + i y): rows 2m, 2m+1, columns 2k, 2k+1 hold
+  // [[x, -y], [y, x]] (x, y of A[m][k] for op = identity; of conj(A[k][m]) for op = ^H)
+  auto store_stage = [&]() {
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      int mm, kk;
+      a_idx(s, mm, kk);
+      float x = 0.0f, y = 0.0f;
+      if ((va >> s) & 1u) {
+        x = (float)ra[s].x;
+        y = TRANSA == 0 ? (float)ra[s].y : -(float)ra[s].y;
+      }
+      // row 2m holds (x, -y), row 2m+1 holds (y, x) at columns 2k, 2k+1: one 4-byte write
+      // per row and part
+      __bf16 p0[2][2], p1[2][2], p2[2][2];
+      split3(x, p0[0][0], p1[0][0], p2[0][0]);
+      split3(-y, p0[0][1], p1[0][1], p2[0][1]);
+      split3(y, p0[1][0], p1[1][0], p2[1][0]);
+      p0[1][1] = p0[0][0];
+      p1[1][1] = p1[0][0];
+      p2[1][1] = p2[0][0];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+        bf16x2_t w0 = {p0[h][0], p0[h][1]}, w1 = {p1[h][0], p1[h][1]}, w2 = {p2[h][0], p2[h][1]};
+        *reinterpret_cast<bf16x2_t*>(&As[0][2 * mm + h][2 * kk]) = w0;
+        *reinterpret_cast<bf16x2_t*>(&As[1][2 * mm + h][2 * kk]) = w1;
+        *reinterpret_cast<bf16x2_t*>(&As[2][2 * mm + h][2 * kk]) = w2;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      int kb, nn;
+      b_idx(s, kb, nn);
+      float x = 0.0f, y = 0.0f;
+      if ((vb >> s) & 1u) {
+        x = (float)rb[s].x;
+        y = (float)rb[s].y;
+      }
+      typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+      __bf16 x0, x1, x2, y0, y1, y2;
+      split3(x, x0, x1, x2);
+      split3(y, y0, y1, y2);
+      const bf16x2_t w0 = {x0, y0}, w1 = {x1, y1}, w2 = {x2, y2};
+      *reinterpret_cast<bf16x2_t*>(&Bs[0][nn][2 * kb]) = w0;
+      *reinterpret_cast<bf16x2_t*>(&Bs[1][nn][2 * kb]) = w1;
+      *reinterpret_cast<bf16x2_t*>(&Bs[2][nn][2 * kb]) = w2;
+    }
+  };
+
+  if (kbeg < kend) load_stage(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += HBK) {
+    store_stage();
+    __syncthreads();
+    if (k0 + HBK < kend) load_stage(k0 + HBK);  // in flight during the MFMA phase
+    // six products, smallest first; the 8 accumulators interleave so that dependent MFMAs
+    // on one accumulator are 8 instructions apart
+    constexpr int PA[6] = {2, 1, 0, 1, 0, 0}, PB[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+    for (int ks = 0; ks < HBK / 16; ++ks) {
+      bf16x8_t a[3][4], b[3][2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+          a[p][rt] = *reinterpret_cast<const bf16x8_t*>(&As[p][wm * 64 + rt * 16 + li][32 * ks + 8 * lg]);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          b[p][ct] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn * 32 + ct * 16 + li][32 * ks + 8 * lg]);
+      }
+#pragma unroll
+      for (int pr = 0; pr < 6; ++pr)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+            acc[rt][ct] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[PA[pr]][rt], b[PB[pr]][ct], acc[rt][ct], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int ip = wm * 64 + rt * 16 + 4 * lg + 2 * p;
+        const int m = m0 + (ip >> 1);
+        const int n = n0 + wn * 32 + ct * 16 + li;
+        if (m < M && n < N) {
+          cplx<double> v = mk((double)acc[rt][ct][2 * p], (double)acc[rt][ct][2 * p + 1]);
+          if (direct) {
+            cplx<double> r = cmul(alpha, v);
+            if (use_beta) r = r + cmul(beta, C[(long)n * ldc + m]);
+            C[(long)n * ldc + m] = r;
+          } else {
+            part[((long)z * N + n) * M + m] = v;
+          }
+        }
+      }
+}
+
 __global__ void k_zgemm_reduce(int M, int N, int nz, const cplx<double>* __restrict__ part,
                                cplx<double>* __restrict__ C, long ldc, cplx<double> alpha,
                                cplx<double> beta, int use_beta) {
@@ -268,7 +450,22 @@ static int zgemm_launch(pw_ctx* c, int transa, int M, int N, int K, cplx<double>
   const double gbytes = (double)M * K * sizeof(cplx<TA>) + (double)K * N * sizeof(cplx<TB>) +
                         (double)M * N * 16.0 * (use_beta ? 2 : 1);
   ProfScope ps(c, transa ? PW_PROF_GEMM_CN : PW_PROF_GEMM_NN, gbytes);
-  if (transa)
+  bool split_bf16 = false;
+  // the three-part operands of a 128 x 128 tile do not fit in LDS: that shape keeps FP32 MFMA
+  if constexpr (sizeof(TC) == 4 && WM * WN < 16) split_bf16 = c->gemm_split_bf16 != 0;
+  if constexpr (sizeof(TC) == 4 && WM * WN < 16) {
+    if (split_bf16) {
+      if (transa)
+        hipLaunchKernelGGL((k_zgemm_b3<TA, TB, 1, WM, WN>), dim3(nwg), dim3(64 * WM * WN), 0, c->stream, M, N, K, A,
+                           lda, B, ldb, kchunk, C, ldc, alpha, beta, direct, use_beta, part, tm, tn, nz);
+      else
+        hipLaunchKernelGGL((k_zgemm_b3<TA, TB, 0, WM, WN>), dim3(nwg), dim3(64 * WM * WN), 0, c->stream, M, N, K, A,
+                           lda, B, ldb, kchunk, C, ldc, alpha, beta, direct, use_beta, part, tm, tn, nz);
+    }
+  }
+  if (split_bf16) {
+    // launched above
+  } else if (transa)
     hipLaunchKernelGGL((k_zgemm<TC, TA, TB, 1, WM, WN>), dim3(nwg), dim3(64 * WM * WN), 0, c->stream, M, N, K, A,
                        lda, B, ldb, kchunk, C, ldc, alpha, beta, direct, use_beta, part, tm, tn, nz);
   else

@@ -119,16 +119,20 @@ def test_zgemm(si_pair, transa, shape, mixed):
     assert err < (1e-5 if mixed else 1e-14), err
 
 
+@pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("tile", [-1, 0, 1, 2, 3, 4, 5])
-def test_zgemm_every_tile(si_pair, tile):
+def test_zgemm_every_tile(si_pair, tile, split):
     """Each workgroup tile shape (pw_tune_set PW_TUNE_GEMM_*_SHAPE) on ragged sizes, with
-    split-K (K = 3001) and without, both precisions and both op(A)."""
+    split-K (K = 3001) and without, both precisions and both op(A); split = 1 runs the
+    mixed products on BF16 MFMA with three-part operands (PW_TUNE_GEMM_SPLIT_BF16), held to
+    the FP32 path's tolerance."""
     from pwdft_hip._lib import DeviceMatrix, PW_MIXED, ptr
     H, _ = si_pair
     c = H.ctx
     try:
         c.call("pw_tune_set", 0, tile)
         c.call("pw_tune_set", 1, tile)
+        c.call("pw_tune_set", 8, split)
         for transa in (0, 1):
             for M, N, K in ((37, 19, 100), (100, 130, 3001), (300, 160, 70), (1, 1, 1)):
                 rng = np.random.default_rng(M + 5 * N + 11 * K + transa)
@@ -150,6 +154,42 @@ def test_zgemm_every_tile(si_pair, tile):
     finally:
         c.call("pw_tune_set", 0, -1)
         c.call("pw_tune_set", 1, -1)
+        c.call("pw_tune_set", 8, 0)
+
+
+def test_split_bf16_op_H_and_error_level(si_pair):
+    """The mixed op_H with its projector products on three-part BF16 MFMA against the
+    oracle (2e-5, the mixed tolerance), and the GEMM error of the split form at the level
+    of the FP32 MFMA path (within 4x of it) on a long inner product."""
+    import pwdft_hip as ph
+    from oracle import ops as O
+    from pwdft_hip._lib import DeviceMatrix, PW_MIXED, ptr
+    H, oH = si_pair
+    c = H.ctx
+    rng = np.random.default_rng(23)
+    M, N, K = 96, 40, 20000
+    A = rand_psi(rng, K, M)
+    B = rand_psi(rng, K, N)
+    ref = A.conj().T @ B
+    one, zero = np.array([1.0, 0.0]), np.array([0.0, 0.0])
+    errs = {}
+    for split in (0, 1):
+        c.call("pw_tune_set", 8, split)
+        dA, dB, dC = DeviceMatrix.from_host(c, A), DeviceMatrix.from_host(c, B), DeviceMatrix(c, M, N)
+        c.call("pw_zgemm_dev", 1, M, N, K, ptr(one), dA.p, K, dB.p, K, ptr(zero), dC.p, M, PW_MIXED)
+        errs[split] = np.abs(dC.download() - ref).max() / np.abs(ref).max()
+    print(f"case split bf16 GEMM error {errs[1]:.2e} vs FP32 MFMA {errs[0]:.2e}", flush=True)
+    assert errs[1] < 4 * errs[0] + 1e-7, errs
+    try:
+        c.call("pw_tune_set", 8, 1)
+        H.mixed = True
+        for ik in range(H.pw.Nkpt):
+            H.ik = oH.ik = ik
+            psi = rand_psi(rng, H.pw.Ngw[ik], 8)
+            assert rel(ph.op_H(H, psi), O.op_H(oH, psi)) < TOLMIX
+    finally:
+        H.mixed = False
+        c.call("pw_tune_set", 8, 0)
 
 
 def test_kprec(si_pair):
