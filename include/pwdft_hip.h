@@ -76,8 +76,10 @@ enum {
                                    stream beside the FFT passes; 0: one stream, in sequence */
   PW_TUNE_COL_CHUNK = 7,        /* z-columns per workgroup of the column FFT kernels (0: the
                                    default, as many as 48 KiB of LDS hold) */
-  PW_TUNE_GEMM_SPLIT_BF16 = 8   /* 1: the mixed path's complex GEMMs run on BF16 MFMA with each
-                                   operand split in three BF16 parts (FP32-level accuracy) */
+  PW_TUNE_GEMM_SPLIT_BF16 = 8   /* the mixed path's complex GEMMs on BF16 MFMA with each operand
+                                   split in three BF16 parts (FP32-level accuracy): 1 always,
+                                   0 never, -1 (default) for products of >= 2^33 complex
+                                   multiply-adds */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

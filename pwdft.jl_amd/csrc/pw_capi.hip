@@ -353,7 +353,8 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       c->plane_compressed = value;
       return PW_OK;
     case PW_TUNE_GEMM_SPLIT_BF16:
-      c->gemm_split_bf16 = value != 0;
+      if (value < -1 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: split mode must be -1..1");
+      c->gemm_split_bf16 = value;
       return PW_OK;
     case PW_TUNE_COL_CHUNK:
       if (value < 0) return set_error(c, PW_ERR_ARG, "pw_tune_set: column chunk must be >= 0");

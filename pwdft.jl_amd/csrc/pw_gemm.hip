@@ -452,7 +452,11 @@ static int zgemm_launch(pw_ctx* c, int transa, int M, int N, int K, cplx<double>
   ProfScope ps(c, transa ? PW_PROF_GEMM_CN : PW_PROF_GEMM_NN, gbytes);
   bool split_bf16 = false;
   // the three-part operands of a 128 x 128 tile do not fit in LDS: that shape keeps FP32 MFMA
-  if constexpr (sizeof(TC) == 4 && WM * WN < 16) split_bf16 = c->gemm_split_bf16 != 0;
+  // default (-1): by size -- the split form pays off on large products (Si216's projector and
+  // Gram products: 4e10 complex multiply-adds, 3.49 -> 3.00 ms), not beside the FFT passes
+  // at Si64 (1.2e9)
+  if constexpr (sizeof(TC) == 4 && WM * WN < 16)
+    split_bf16 = c->gemm_split_bf16 > 0 || (c->gemm_split_bf16 < 0 && (double)M * N * K >= 8589934592.0);
   if constexpr (sizeof(TC) == 4 && WM * WN < 16) {
     if (split_bf16) {
       if (transa)

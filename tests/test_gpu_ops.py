@@ -154,7 +154,7 @@ def test_zgemm_every_tile(si_pair, tile, split):
     finally:
         c.call("pw_tune_set", 0, -1)
         c.call("pw_tune_set", 1, -1)
-        c.call("pw_tune_set", 8, 0)
+        c.call("pw_tune_set", 8, -1)
 
 
 def test_split_bf16_op_H_and_error_level(si_pair):
@@ -189,7 +189,7 @@ def test_split_bf16_op_H_and_error_level(si_pair):
             assert rel(ph.op_H(H, psi), O.op_H(oH, psi)) < TOLMIX
     finally:
         H.mixed = False
-        c.call("pw_tune_set", 8, 0)
+        c.call("pw_tune_set", 8, -1)
 
 
 def test_kprec(si_pair):
