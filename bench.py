@@ -119,6 +119,39 @@ def time_op_H(H, X, Y, steps, warmup, mixed, profile=False):
     return ms.value / 1e3, wall, prof
 
 
+def time_rhoe(H, X, steps, warmup, mixed):
+    """calc_rhoe! over the 160 bands (pw_rhoe_zero + pw_rhoe_accum_dev, weights 2/Nbands):
+    seconds per step and the per-kernel profile of the density stage."""
+    from pwdft_hip._lib import PW_MIXED, ptr
+    c = H.ctx
+    w = np.full(NBANDS, 2.0 / NBANDS)
+    fl = PW_MIXED if mixed else 0
+    for _ in range(warmup):
+        c.call("pw_rhoe_zero")
+        c.call("pw_rhoe_accum_dev", 0, NBANDS, X.p, X.rows, ptr(w), fl)
+    c.call("pw_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.call("pw_rhoe_zero")
+        c.call("pw_rhoe_accum_dev", 0, NBANDS, X.p, X.rows, ptr(w), fl)
+    c.call("pw_synchronize")
+    sec = (time.perf_counter() - t0) / steps
+    c.profile(True)
+    for _ in range(steps):
+        c.call("pw_rhoe_zero")
+        c.call("pw_rhoe_accum_dev", 0, NBANDS, X.p, X.rows, ptr(w), fl)
+    prof = c.profile_read()
+    c.profile(False)
+    out = {"ms_per_step": sec * 1e3, "band_apps_per_s": NBANDS / sec}
+    for k in ("col_ifft", "plane_rhoe"):
+        ms, n, b = prof[k]
+        if n:
+            us = ms / n * 1e3
+            out[k] = {"us": us, "achieved": b / n / (us * 1e-6) / 1e9, "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                      "frac": b / n / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}
+    return out
+
+
 def time_lobpcg(H, X0, iters, mixed):
     """diag_LOBPCG! on the Si64 bands for a fixed number of iterations (tol = 0, so no band
     converges or locks) from the same orthonormal start; returns band-applications/s
@@ -401,6 +434,11 @@ def main():
             if n:
                 print(f"#   {tag} {k:11s} {ms / n * 1e3:9.1f} us/launch x{n // args.steps} per step, "
                       f"{b / n / (ms / n / 1e3) / 1e9:8.1f} GB/s algorithmic", file=sys.stderr, flush=True)
+    rhoe = {}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        rhoe[tag] = time_rhoe(H, X, args.steps, args.warmup, mixed)
+        print(f"# calc_rhoe {tag}: {rhoe[tag]['ms_per_step']:.3f} ms per 160 bands", file=sys.stderr, flush=True)
     Natoms = H.atoms.Natoms
     dE = abs(np.sum(res["mixed"]["eband"]) - np.sum(res["fp64"]["eband"])) * 2.0 / Natoms
     t_mix = res["mixed"]["sec"] / args.steps
@@ -452,6 +490,7 @@ def main():
         "kernels_us_per_step": {t: {k: v[0] * 1e3 / args.steps for k, v in res[t]["prof"].items() if v[1]}
                                 for t in ("fp64", "mixed")},
         "stages": {t: stages(res[t]["prof"], H, t == "mixed", args.steps) for t in ("fp64", "mixed")},
+        "calc_rhoe": rhoe,
         "forces_si8": forces,
         "si216": si216,
         "cpu_baseline": cpu,

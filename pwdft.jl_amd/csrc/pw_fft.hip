@@ -809,7 +809,9 @@ k_col_fft_batch(const KBatch* __restrict__ kb, int nk, int nb, const cplx<T>* __
 }
 
 // rho planes: per (z-plane, band group): y/x inverse FFT of each band and
-// sum_b w_b |psi_b(r)|^2 accumulated in an LDS plane of doubles; one partial plane per group.
+// sum_b w_b |psi_b(r)|^2 accumulated in registers (plane point tid + r * blockDim.x in
+// register r, at most RHO_REGS per thread); one partial plane per group.
+constexpr int RHO_REGS = 16;
 template <typename T>
 __global__ void __launch_bounds__(PlaneNT<T>::value)
 k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int nb, int bpg,
@@ -821,14 +823,15 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
   const int n1 = dx.n, n2 = dy.n, np = n1 * n2, ps = odd_stride(n1);
   cplx<T>* tx = lds + ps * n2;
   cplx<T>* ty = tx + n1;
-  double* acc = reinterpret_cast<double*>(ty + n2);
-  int* xl = reinterpret_cast<int*>(acc + np);
+  int* xl = reinterpret_cast<int*>(ty + n2);
   const int z = blockIdx.x, g = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
   stage(tx, twx, n1);
   stage(ty, twy, n2);
   stage(xl, xact, nxact);
-  for (int i = tid; i < np; i += nthr) acc[i] = 0.0;
+  double acc[RHO_REGS];
+#pragma unroll
+  for (int r = 0; r < RHO_REGS; ++r) acc[r] = 0.0;
   const int bend = min(nb, (g + 1) * bpg);
   for (int b = g * bpg; b < bend; ++b) {
     for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
@@ -841,15 +844,23 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
     wave_lines<T, +1, FFT_ME>(lds, n2, nullptr, ps, 1, dx, tx);
     __syncthreads();
     const double wb = wts[b];
-    for (int i = tid; i < np; i += nthr) {
-      const int j = i / n1, x = i - j * n1;
-      const double re = (double)lds[x + ps * j].x, im = (double)lds[x + ps * j].y;
-      acc[i] += wb * (re * re + im * im);
+#pragma unroll
+    for (int r = 0; r < RHO_REGS; ++r) {
+      const int i = tid + r * nthr;
+      if (i < np) {
+        const int j = i / n1, x = i - j * n1;
+        const double re = (double)lds[x + ps * j].x, im = (double)lds[x + ps * j].y;
+        acc[r] += wb * (re * re + im * im);
+      }
     }
     __syncthreads();
   }
   double* out = rpart + (long)g * N + (long)z * np;
-  for (int i = tid; i < np; i += nthr) out[i] = acc[i];
+#pragma unroll
+  for (int r = 0; r < RHO_REGS; ++r) {
+    const int i = tid + r * nthr;
+    if (i < np) out[i] = acc[r];
+  }
 }
 
 __global__ void k_rho_reduce(double* __restrict__ rho, const double* __restrict__ rpart, int ng,
@@ -1118,9 +1129,12 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const size_t esz = sizeof(cplx<T>);
   const int C = cols_per_block(c, esz);
   const int np = c->n1 * c->n2;
-  size_t plane_bytes = plane_lds(c, esz, k.nxact) + (size_t)np * sizeof(double);
+  size_t plane_bytes = plane_lds(c, esz, k.nxact);
   int rl = 0;
-  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed != 0;
+  // the padded-plane kernel keeps each thread's share of the plane (<= RHO_REGS points) in
+  // registers; planes beyond that take the compressed kernel, which accumulates in HBM
+  const bool compressed = plane_bytes > 160 * 1024 || c->plane_compressed != 0 ||
+                          np > RHO_REGS * PlaneNT<T>::value;
   if (compressed) {
     rl = plane_c_rows(c, esz, k.nxact, PlaneNT<T>::value, &plane_bytes);
     if (rl < 1) return set_error(c, PW_ERR_ARG, "z-plane too large for calc_rhoe kernel");
@@ -1130,7 +1144,9 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
   const int bb = std::max(1, std::min(nb, c->band_batch));
   int rc = ensure(c, &c->d_work, &c->work_bytes, (size_t)bb * c->n3 * k.ncol * esz);
   if (rc) return rc;
-  int ngroup = std::max(1, std::min(bb, (512 + c->n3 - 1) / c->n3));
+  // band groups: about one round of resident plane workgroups (3 per CU on 256 CUs) over the
+  // n3 planes; each group's partial plane is reduced afterwards in a fixed order
+  int ngroup = std::max(1, std::min(bb, (768 + c->n3 - 1) / c->n3));
   rc = ensure(c, &c->d_rpart, &c->rpart_bytes, (size_t)ngroup * c->N * sizeof(double));
   if (rc) return rc;
   cplx<T>* W = reinterpret_cast<cplx<T>*>(c->d_work);
