@@ -312,6 +312,13 @@ int pw_hegvd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, void* B, int64_t ldb,
  * many k-points (diag_LOBPCG.jl:176-203, 217-222).  Eigenvectors into A_i (ascending),
  * eigenvalues into w (host, n x nbatch, column i for matrix i). */
 int pw_heevd_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA, double* w);
+/* pw_herm_fix_dev on each of nbatch matrices A_i = A + i*strideA (nbatch <= 65535). */
+int pw_herm_fix_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int mode);
+/* A_i <- inv(cholesky(Hermitian(A_i)).U) for nbatch matrices A_i = A + i*strideA, the strict
+ * lower triangle zeroed: the C = U^H U factorisation and the inv(U) of every k-point's Gram
+ * matrix in one call (diag_LOBPCG.jl:106-118, 193-203 over many k-points).
+ * PW_ERR_NOT_POSDEF names the first matrix that is not positive definite. */
+int pw_chol_inv_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA);
 
 /* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */
 

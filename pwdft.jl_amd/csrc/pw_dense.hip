@@ -55,7 +55,9 @@ static int read_info(pw_ctx* c, const int* d_info, int* info) {
 
 // mode 0: A <- Hermitian(A) (upper triangle mirrored, real diagonal); 1: strict lower
 // triangle zeroed (an upper-triangular factor); 2: A <- (A + A^H) / 2.
-__global__ void k_herm_fix(cplx<double>* __restrict__ A, int n, long lda, int mode) {
+// Matrix blockIdx.y of a strided stack (stride 0 for a single matrix).
+__global__ void k_herm_fix(cplx<double>* __restrict__ A, int n, long lda, long stride, int mode) {
+  A += (long)blockIdx.y * stride;
   const long tot = (long)n * n;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
     const int i = (int)(t % n), j = (int)(t / n);
@@ -72,7 +74,8 @@ __global__ void k_herm_fix(cplx<double>* __restrict__ A, int n, long lda, int mo
   }
 }
 
-__global__ void k_herm_avg(cplx<double>* __restrict__ A, int n, long lda) {
+__global__ void k_herm_avg(cplx<double>* __restrict__ A, int n, long lda, long stride) {
+  A += (long)blockIdx.y * stride;
   const long tot = (long)n * n;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
     const int i = (int)(t % n), j = (int)(t / n);
@@ -96,20 +99,75 @@ using namespace pw;
 
 extern "C" {
 
+static int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t stride, int mode) {
+  const long tot = (long)n * n;
+  const int blocks = (int)std::min<long>((tot + 255) / 256, nbatch > 1 ? 64 : 2048);
+  cplx<double>* a = reinterpret_cast<cplx<double>*>(A);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_herm_avg, dim3(blocks, nbatch), dim3(256), 0, c->stream, a, n, (long)lda, (long)stride);
+  else
+    hipLaunchKernelGGL(k_herm_fix, dim3(blocks, nbatch), dim3(256), 0, c->stream, a, n, (long)lda, (long)stride,
+                       mode);
+  return hip_check(c, hipGetLastError(), "k_herm_fix");
+}
+
 int pw_herm_fix_dev(pw_ctx* c, int n, void* A, int64_t lda, int mode) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   if (n < 0 || !A || lda < (n > 0 ? n : 1) || mode < 0 || mode > 2)
     return set_error(c, PW_ERR_ARG, "pw_herm_fix_dev: bad arguments");
   if (n == 0) return PW_OK;
-  const long tot = (long)n * n;
-  const int blocks = (int)std::min<long>((tot + 255) / 256, 2048);
-  cplx<double>* a = reinterpret_cast<cplx<double>*>(A);
-  if (mode == 2)
-    hipLaunchKernelGGL(k_herm_avg, dim3(blocks), dim3(256), 0, c->stream, a, n, (long)lda);
-  else
-    hipLaunchKernelGGL(k_herm_fix, dim3(blocks), dim3(256), 0, c->stream, a, n, (long)lda, mode);
-  return hip_check(c, hipGetLastError(), "k_herm_fix");
+  return herm_fix_launch(c, n, 1, A, lda, 0, mode);
+}
+
+int pw_herm_fix_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int mode) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || nbatch < 0 || nbatch > 65535 || (n > 0 && nbatch > 0 && !A) || lda < (n > 0 ? n : 1) ||
+      strideA < lda * (int64_t)n || mode < 0 || mode > 2)
+    return set_error(c, PW_ERR_ARG, "pw_herm_fix_batch_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  return herm_fix_launch(c, n, nbatch, A, lda, strideA, mode);
+}
+
+int pw_chol_inv_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || nbatch < 0 || nbatch > 65535 || (n > 0 && nbatch > 0 && !A) || lda < (n > 0 ? n : 1) ||
+      strideA < lda * (int64_t)n)
+    return set_error(c, PW_ERR_ARG, "pw_chol_inv_batch_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  // one potrf info and one trtri info per matrix
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, 2 * (size_t)nbatch * sizeof(int));
+  if (rc) return rc;
+  int* dinfo = reinterpret_cast<int*>(c->d_dwork);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  rocblas_double_complex* a = reinterpret_cast<rocblas_double_complex*>(A);
+  rc = st_check(c, rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, a, (int)lda, (rocblas_stride)strideA,
+                                                    dinfo, nbatch),
+                "zpotrf_strided_batched");
+  if (rc) return rc;
+  rc = st_check(c, rocsolver_ztrtri_strided_batched(h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, a, (int)lda,
+                                                    (rocblas_stride)strideA, dinfo + nbatch, nbatch),
+                "ztrtri_strided_batched");
+  if (rc) return rc;
+  rc = herm_fix_launch(c, n, nbatch, A, lda, strideA, 1);
+  if (rc) return rc;
+  std::vector<int> info(2 * (size_t)nbatch);
+  hipError_t e = hipMemcpyAsync(info.data(), dinfo, info.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "batched cholesky info");
+  for (int i = 0; i < nbatch; ++i) {
+    if (info[i] != 0)
+      return set_error(c, PW_ERR_NOT_POSDEF, "cholesky of matrix " + std::to_string(i) + ": leading minor " +
+                                                 std::to_string(info[i]) + " not positive definite");
+    if (info[nbatch + i] != 0)
+      return set_error(c, PW_ERR_NOT_POSDEF, "triangular inverse of matrix " + std::to_string(i) +
+                                                 ": singular diagonal " + std::to_string(info[nbatch + i]));
+  }
+  return PW_OK;
 }
 
 int pw_potrf_dev(pw_ctx* c, int n, void* A, int64_t lda) {

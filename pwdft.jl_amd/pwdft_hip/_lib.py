@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpwdft_hip.so")
 
 PW_OK = 0
+PW_ERR_NOT_POSDEF = -4
 PW_TERM_K = 1
 PW_TERM_VLOC = 2
 PW_TERM_NLOC = 4
@@ -59,6 +60,8 @@ _PROTOS = {
     "pw_trtri_dev": [_vp, _i, _vp, _i64],
     "pw_heevd_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_heevd_batch_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp],
+    "pw_herm_fix_batch_dev": [_vp, _i, _i, _vp, _i64, _i64, _i],
+    "pw_chol_inv_batch_dev": [_vp, _i, _i, _vp, _i64, _i64],
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -156,7 +159,7 @@ class Context:
     def check(self, rc):
         if rc != PW_OK:
             msg = self.lib.pw_last_error(self.h).decode()
-            raise PWError(rc, msg)
+            raise (PosDefException if rc == PW_ERR_NOT_POSDEF else PWError)(rc, msg)
 
     def call(self, name, *args):
         self.check(getattr(self.lib, name)(self.h, *args))

@@ -5,9 +5,10 @@ k-points of a few hundred plane waves and a handful of bands) every step of that
 tiny launch.  Here every step of one LOBPCG iteration is one batched launch for all k-points
 that have not converged yet (pw_op_H_batch_dev, pw_zgemm_batch_dev, pw_kprec_batch_dev,
 pw_copy_cols_batch_dev), and the small dense problems of all k-points (Nst..3Nst) are
-solved together: the Cholesky factors of the Gram matrices as stacked numpy arrays, the
-Hermitian eigenproblems of the Rayleigh-Ritz steps in one batched device call
-(pw_heevd_batch_dev).  A k-point leaves the batch when it meets the
+solved together on the device, in stacked buffers that never leave HBM: the Cholesky
+factors and their inverses in one call (pw_chol_inv_batch_dev), the Hermitian eigenproblems
+of the Rayleigh-Ritz steps in another (pw_heevd_batch_dev).  Only eigenvalues reach the
+host, for the convergence test.  A k-point leaves the batch when it meets the
 reference's convergence test; one that reaches the reference's lock branch finishes that
 iteration through the per-k code (solvers._lobpcg_lock_step).  The arithmetic per k-point is
 that of solvers.diag_LOBPCG_."""
@@ -74,14 +75,6 @@ class _Small:
             self.buf = DeviceMatrix(self.ctx, max(need, 1), 1)
         return [self.buf.p.value + 16 * r * c * i for i in range(nk)]
 
-    def upload(self, M):
-        """M: nk x r x c complex -> device pointers of each k's column-major block."""
-        nk, r, c = M.shape
-        p = self.ptrs(nk, r, c)
-        a = np.ascontiguousarray(np.transpose(M, (0, 2, 1)), dtype=np.complex128)   # column-major blocks
-        self.ctx.call("pw_memcpy_h2d", ctypes.c_void_p(p[0]), ptr(a), a.nbytes)
-        return p
-
     def download(self, nk, r, c):
         a = np.empty((nk, c, r), dtype=np.complex128)
         self.ctx.call("pw_memcpy_d2h", ptr(a), ctypes.c_void_p(self.buf.p.value), a.nbytes)
@@ -99,8 +92,6 @@ class _Batch:
         self.Ham = Ham
         self.c = Ham.ctx
         self.mixed = mixed
-        self.small = _Small(Ham.ctx)
-        self.small2 = _Small(Ham.ctx)
 
     def gemm(self, transa, m, n, k, A, lda, B, ldb, C, ldc, alpha=_ONE, beta=_ZERO):
         nb = len(A)
@@ -111,12 +102,30 @@ class _Batch:
         self.c.call("pw_zgemm_batch_dev", transa, nb, ptr(ms), ptr(ns), ptr(ks), ptr(alpha), _pa(A), ptr(la), _pa(B),
                     ptr(lb), ptr(beta), _pa(C), ptr(lc))
 
-    def gram(self, A, B, na, nbc, rows):
-        """A_k^H B_k for column blocks -> host nk x na x nbc."""
-        nk = len(A)
-        out = self.small.ptrs(nk, na, nbc)
+    def gram(self, small, A, B, na, nbc, rows):
+        """A_k^H B_k for column blocks into the stacked buffer `small` (na x nbc per k-point);
+        returns the device pointers of the blocks."""
+        out = small.ptrs(len(A), na, nbc)
         self.gemm(1, na, nbc, rows, A, rows, B, rows, out, na)
-        return self.small.download(nk, na, nbc)
+        return out
+
+    def herm(self, p, n, mode):
+        """pw_herm_fix_dev on every n x n block at p (mode 0: Hermitian(); 2: (A + A^H) / 2)."""
+        self.c.call("pw_herm_fix_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n, mode)
+
+    def chol_inv(self, p, n, iks):
+        """Every n x n block at p <- inv(cholesky(Hermitian(block)).U), strict lower zeroed."""
+        try:
+            self.c.call("pw_chol_inv_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n)
+        except PosDefException as e:
+            raise PosDefException(e.code, f"{e} (batch of k-points {list(iks)})")
+
+    def eigh(self, p, n):
+        """eigen(Hermitian(block)) of every n x n block at p: eigenvectors in place,
+        eigenvalues returned (len(p) x n, ascending)."""
+        w = np.empty((len(p), n))
+        self.c.call("pw_heevd_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n, ptr(w))
+        return w
 
     def copy(self, rows, ncols, src, dst):
         nb = len(src)
@@ -137,53 +146,20 @@ class _Batch:
         ld = np.ascontiguousarray(rows, dtype=np.int64)
         self.c.call("pw_kprec_batch_dev", len(iks), ptr(ik), ncols, _pa(src), ptr(ld), _pa(dst), ptr(ld))
 
-    def free(self):
-        self.small.free()
-        self.small2.free()
 
-
-def _herm_u(A):
-    """Hermitian(A) of each stacked matrix: upper triangle mirrored, real diagonal."""
-    U = np.triu(A, 1)
-    d = np.real(np.diagonal(A, axis1=-2, axis2=-1))
-    out = U + np.conj(np.swapaxes(U, -1, -2))
-    idx = np.arange(A.shape[-1])
-    out[..., idx, idx] = d
-    return out
-
-
-def _chol_upper_inv(C, iks):
-    """cholesky(Hermitian(C)).U and its inverse for every stacked matrix."""
-    try:
-        L = np.linalg.cholesky(_herm_u(C))
-    except np.linalg.LinAlgError as e:
-        raise PosDefException(-4, f"cholesky (k-points {list(iks)}): {e}")
-    U = np.conj(np.swapaxes(L, -1, -2))
-    return U, np.linalg.inv(U)
-
-
-def _eigh_dev(small, A):
-    """eigh of every stacked Hermitian matrix A (nk x n x n), upper triangle read, in one
-    batched device call (pw_heevd_batch_dev): (eigenvalues nk x n, eigenvectors nk x n x n)."""
-    nk, n, _ = A.shape
-    p = small.upload(A)
-    w = np.empty((nk, n))
-    small.ctx.call("pw_heevd_batch_dev", n, nk, ctypes.c_void_p(p[0]), n, n * n, ptr(w))
-    return w, small.download(nk, n, n)
-
-
-def _geneig_lowest(T, G, n, small):
-    """Lowest n eigenvectors of T x = l G x for every stacked pair (eigen(T, G)): G = L L^H
-    on the host, the standard problems L^-1 T L^-H of all k-points on the device."""
-    try:
-        L = np.linalg.cholesky(G)
-    except np.linalg.LinAlgError as e:
-        raise PosDefException(-4, f"eigen(T, G): {e}")
-    Li = np.linalg.inv(L)
-    A = Li @ T @ np.conj(np.swapaxes(Li, -1, -2))
-    A = 0.5 * (A + np.conj(np.swapaxes(A, -1, -2)))
-    _, Y = _eigh_dev(small, A)
-    return np.conj(np.swapaxes(Li, -1, -2)) @ Y[..., :n]
+def _ritz_coefs(B, sM, T, G, nq, Nst, iks):
+    """Lowest Nst eigenvectors of eigen(Hermitian(T), Hermitian(G)) for every stacked pair
+    (diag_LOBPCG.jl:176-190), on the device: G = U^H U, C = U^-H T U^-1, C y = l y,
+    x = U^-1 y (the reduction zhegv performs).  T and G (nq x nq blocks) are overwritten;
+    returns the pointers of the nq x Nst coefficient blocks (leading dimension nq) in sM."""
+    B.herm(T, nq, 0)
+    B.chol_inv(G, nq, iks)
+    M = sM.ptrs(len(T), nq, nq)
+    B.gemm(0, nq, nq, nq, T, nq, G, nq, M, nq)      # T Ui
+    B.gemm(1, nq, nq, nq, G, nq, M, nq, T, nq)      # Ui^H T Ui
+    B.eigh(T, nq)
+    B.gemm(0, nq, Nst, nq, G, nq, T, nq, M, nq)     # Ui y, lowest Nst
+    return M
 
 
 def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, iks=None):
@@ -204,7 +180,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     Q = _Slab(c, rows, 3 * Nst)
     HQ = _Slab(c, rows, 3 * Nst)
     Tmp = _Slab(c, rows, 4 * Nst)
-    eig_small = _Small(c)
+    sX, sC, sT, sG, sM = (_Small(c) for _ in range(5))   # stacked small matrices in HBM
     allk = list(range(nk))
     cols = lambda S, ks, j: [S.col(k, j) for k in ks]
     rws = lambda ks: [rows[k] for k in ks]
@@ -222,14 +198,14 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         ka = [iks[k] for k in a]
         iters[a] = it
         x, hx, w, hw = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, Nst), cols(HQ, a, Nst)
-        XHX = _herm_u(B.gram(x, hx, Nst, Nst, rws(a)))
-        evals = np.linalg.eigvalsh(XHX)
+        pX = B.gram(sX, x, hx, Nst, Nst, rws(a))
+        B.herm(pX, Nst, 0)
+        evals = np.linalg.eigvalsh(sX.download(len(a), Nst, Nst))   # eigvals(Hermitian(XHX))
         devals = np.abs(evals - evals_old[a])
         evals_old[a] = evals
         # R = HX - X*XHX (into the W slot)
         B.copy(rws(a), Nst, hx, w)
-        dX = B.small2.upload(XHX)
-        B.gemm(0, rws(a), Nst, Nst, x, rws(a), dX, Nst, w, rws(a), alpha=_MONE, beta=_ONE)
+        B.gemm(0, rws(a), Nst, Nst, x, rws(a), pX, Nst, w, rws(a), alpha=_MONE, beta=_ONE)
         nconv = np.count_nonzero(devals < tol, axis=1)
         done = nconv >= Nstates_conv
         lock = (~done) & np.any(devals <= SMALL, axis=1)
@@ -255,33 +231,32 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         B.kprec(ka, w, w, Nst, ra)
         B.opH(ka, w, hw, Nst, ra)
         ncols[a] += Nst
-        U, Ui = _chol_upper_inv(B.gram(w, w, Nst, Nst, ra), ka)
-        dUi = B.small2.upload(Ui)
+        pC = B.gram(sC, w, w, Nst, Nst, ra)
+        B.chol_inv(pC, Nst, ka)                          # inv(cholesky(Hermitian(W'W)).U)
         t0, t1 = cols(Tmp, a, 0), cols(Tmp, a, Nst)
-        B.gemm(0, ra, Nst, Nst, w, ra, dUi, Nst, t0, ra)
-        B.gemm(0, ra, Nst, Nst, hw, ra, dUi, Nst, t1, ra)
+        B.gemm(0, ra, Nst, Nst, w, ra, pC, Nst, t0, ra)
+        B.gemm(0, ra, Nst, Nst, hw, ra, pC, Nst, t1, ra)
         B.copy(ra, Nst, t0, w)
         B.copy(ra, Nst, t1, hw)
         nq = 3 * Nst if it > 1 else 2 * Nst
         qa, hqa = cols(Q, a, 0), cols(HQ, a, 0)
-        T = _herm_u(B.gram(qa, hqa, nq, nq, ra))
-        G = _herm_u(B.gram(qa, qa, nq, nq, ra))
-        Um = _geneig_lowest(T, G, Nst, eig_small)
-        dU = B.small2.upload(Um)
-        B.gemm(0, ra, Nst, nq, qa, ra, dU, nq, t0, ra)
-        B.gemm(0, ra, Nst, nq, hqa, ra, dU, nq, t1, ra)
+        pT = B.gram(sT, qa, hqa, nq, nq, ra)
+        pG = B.gram(sG, qa, qa, nq, nq, ra)
+        pU = _ritz_coefs(B, sM, pT, pG, nq, Nst, ka)
+        B.gemm(0, ra, Nst, nq, qa, ra, pU, nq, t0, ra)
+        B.gemm(0, ra, Nst, nq, hqa, ra, pU, nq, t1, ra)
         x, hx, p, hp = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, 2 * Nst), cols(HQ, a, 2 * Nst)
         if it > 1:
-            dU23 = B.small2.upload(np.ascontiguousarray(Um[:, Nst:3 * Nst, :]))
+            pU23 = [u + 16 * Nst for u in pU]           # rows Nst:3Nst of the coefficients
             t2, t3 = cols(Tmp, a, 2 * Nst), cols(Tmp, a, 3 * Nst)
-            B.gemm(0, ra, Nst, 2 * Nst, w, ra, dU23, 2 * Nst, t2, ra)
-            B.gemm(0, ra, Nst, 2 * Nst, hw, ra, dU23, 2 * Nst, t3, ra)
+            B.gemm(0, ra, Nst, 2 * Nst, w, ra, pU23, nq, t2, ra)
+            B.gemm(0, ra, Nst, 2 * Nst, hw, ra, pU23, nq, t3, ra)
             B.copy(ra, Nst, t0, x)
             B.copy(ra, Nst, t1, hx)
-            U, Ui = _chol_upper_inv(B.gram(t2, t2, Nst, Nst, ra), ka)
-            dUi = B.small2.upload(Ui)
-            B.gemm(0, ra, Nst, Nst, t2, ra, dUi, Nst, p, ra)
-            B.gemm(0, ra, Nst, Nst, t3, ra, dUi, Nst, hp, ra)
+            pC = B.gram(sC, t2, t2, Nst, Nst, ra)
+            B.chol_inv(pC, Nst, ka)
+            B.gemm(0, ra, Nst, Nst, t2, ra, pC, Nst, p, ra)
+            B.gemm(0, ra, Nst, Nst, t3, ra, pC, Nst, hp, ra)
         else:
             B.copy(ra, Nst, w, p)
             B.copy(ra, Nst, hw, hp)
@@ -289,14 +264,12 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
             B.copy(ra, Nst, t1, hx)
     # final Rayleigh-Ritz of every k-point
     x, hx = cols(Q, allk, 0), cols(HQ, allk, 0)
-    S = B.gram(x, hx, Nst, Nst, rws(allk))
-    S = _herm_u(0.5 * (S + np.conj(np.swapaxes(S, -1, -2))))
-    ev, Qm = _eigh_dev(eig_small, S)
-    dQ = B.small2.upload(Qm)
-    B.gemm(0, rws(allk), Nst, Nst, x, rws(allk), dQ, Nst, [psiks[k].p.value for k in allk], rws(allk))
+    pS = B.gram(sX, x, hx, Nst, Nst, rws(allk))
+    B.herm(pS, Nst, 2)
+    ev = B.eigh(pS, Nst)
+    B.gemm(0, rws(allk), Nst, Nst, x, rws(allk), pS, Nst, [psiks[k].p.value for k in allk], rws(allk))
     c.call("pw_synchronize")
-    for s in (Q, HQ, Tmp, eig_small):
+    for s in (Q, HQ, Tmp, sX, sC, sT, sG, sM):
         s.free()
-    B.free()
     stats = [{"iter": int(iters[k]), "ncols": int(ncols[k])} for k in allk]
     return ev.T.copy(), stats

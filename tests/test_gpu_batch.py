@@ -152,6 +152,52 @@ def test_heevd_batch_vs_numpy(n, nbatch):
     c.close()
 
 
+@pytest.mark.parametrize("n,nbatch", [(1, 3), (8, 182), (24, 182), (40, 5)])
+def test_chol_inv_batch_vs_numpy(n, nbatch):
+    """pw_chol_inv_batch_dev: every block becomes inv(cholesky(Hermitian(A)).U) of numpy's
+    factor (upper triangle read, strict lower triangle zeroed); a batch holding an indefinite
+    matrix raises PosDefException naming it.  pw_herm_fix_batch_dev modes 0 and 2 against
+    numpy's Hermitian() and (A + A^H)/2."""
+    from pwdft_hip._lib import Context, DeviceMatrix, PosDefException
+    c = Context(0)
+    rng = np.random.default_rng(n * 7 + nbatch)
+    Z = rng.standard_normal((nbatch, n, 2 * n)) + 1j * rng.standard_normal((nbatch, n, 2 * n))
+    A = Z @ np.conj(np.swapaxes(Z, 1, 2)) / (2 * n) + 0.1 * np.eye(n)
+    dev = np.array(A)
+    low = np.tril_indices(n, -1)
+    for k in range(nbatch):
+        dev[k][low] = 7.0 + 3.0j             # must be ignored
+    col = lambda M: np.ascontiguousarray(np.transpose(M, (0, 2, 1))).reshape(-1, 1)
+    back = lambda d: np.transpose(d.download().reshape(nbatch, n, n), (0, 2, 1))
+    d = DeviceMatrix.from_host(c, col(dev))
+    print(f"case chol_inv_batch n={n} nbatch={nbatch}", flush=True)
+    c.call("pw_chol_inv_batch_dev", n, nbatch, d.p, n, n * n)
+    Ui = back(d)
+    U = np.conj(np.swapaxes(np.linalg.cholesky(A), 1, 2))
+    ref = np.linalg.inv(U)
+    scale = np.abs(ref).max()
+    assert np.abs(Ui - ref).max() < 1e-11 * scale
+    assert np.all(Ui[:, low[0], low[1]] == 0)
+    assert np.abs(U @ Ui - np.eye(n)).max() < 1e-11 * scale
+    if n > 1:
+        bad = np.array(dev)
+        bad[min(2, nbatch - 1)][0, 0] = -1.0
+        d.upload(col(bad).ravel())
+        with pytest.raises(PosDefException, match=f"matrix {min(2, nbatch - 1)}"):
+            c.call("pw_chol_inv_batch_dev", n, nbatch, d.p, n, n * n)
+    G = rng.standard_normal((nbatch, n, n)) + 1j * rng.standard_normal((nbatch, n, n))
+    up = np.triu(G, 1)
+    herm = up + np.conj(np.swapaxes(up, 1, 2))
+    idx = np.arange(n)
+    herm[:, idx, idx] = G[:, idx, idx].real
+    for mode, want in ((0, herm), (2, 0.5 * (G + np.conj(np.swapaxes(G, 1, 2))))):
+        d.upload(col(G).ravel())
+        c.call("pw_herm_fix_batch_dev", n, nbatch, d.p, n, n * n, mode)
+        assert np.abs(back(d) - want).max() < 1e-15 * max(1.0, np.abs(G).max()), mode
+    d.free()
+    c.close()
+
+
 def test_lobpcg_batch_vs_per_k(al):
     """diag_LOBPCG! over all 182 k-points in lockstep gives the per-k solver's eigenvalues."""
     import pwdft_hip as ph
