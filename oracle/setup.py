@@ -206,15 +206,26 @@ class PWGrid:
         return (self.G[0, idx] + kk[0]) ** 2 + (self.G[1, idx] + kk[1]) ** 2 + (self.G[2, idx] + kk[2]) ** 2
 
 
+# Threads of the 3-D FFTs (scipy.fft workers) when above 1; the default runs numpy's FFT on
+# one thread.  The all-core CPU timing sets it to the number of cores.
+FFT_WORKERS = 1
+
+
 def G_to_R(pw, fG):
     """wrappers_fft.jl:42-51: in-place backward FFT, normalised by 1/N, exp(+i)."""
     f = np.asarray(fG).reshape(pw.Ns, order="F")
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifftn(f, workers=FFT_WORKERS).reshape(-1, order="F")
     return np.fft.ifftn(f).reshape(-1, order="F")
 
 
 def R_to_G(pw, fR):
     """wrappers_fft.jl:53-62: forward FFT, unnormalised, exp(-i)."""
     f = np.asarray(fR, dtype=complex).reshape(pw.Ns, order="F")
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fftn(f, workers=FFT_WORKERS).reshape(-1, order="F")
     return np.fft.fftn(f).reshape(-1, order="F")
 
 
