@@ -178,7 +178,7 @@ struct pw_ctx {
   int plane_compressed = 0;
   int tune_col_chunk = 0;
   // FP32-path complex GEMMs on BF16 MFMA with 3-part operands: 1 always, 0 never, -1 by size
-  int gemm_split_bf16 = -1;  // columns per column-kernel workgroup (0: by LDS, cols_per_block)
+  int gemm_split_bf16 = -1;
   // tuning (pw_tune_set): [0] = A^H B products, [1] = A B products
   // -1: by N (pw_gemm.hip: zgemm_t); 0: 2x2 wavefronts (64x64 tile), 1: 4x2, 2: 2x4, 3: 4x4,
   // 4: 4x1 (128 x 32), 5: 2x1 (64 x 32)
