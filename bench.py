@@ -163,6 +163,8 @@ def time_rhoe(H, X, steps, warmup, mixed):
             us = ms / n * 1e3
             out[k] = {"us": us, "achieved": b / n / (us * 1e-6) / 1e9, "unit": "GB/s", "peak": PEAK_HBM_GBS,
                       "frac": b / n / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}
+    if "plane_rhoe" in out:
+        out["plane_rhoe"]["traffic"] = _traffic("plane_rhoe", mixed)
     return out
 
 

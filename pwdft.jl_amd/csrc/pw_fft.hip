@@ -810,9 +810,10 @@ k_col_fft_batch(const KBatch* __restrict__ kb, int nk, int nb, const cplx<T>* __
 
 // rho planes: per (z-plane, band group): y/x inverse FFT of each band and
 // sum_b w_b |psi_b(r)|^2 accumulated in registers (plane point tid + r * blockDim.x in
-// register r, at most RHO_REGS per thread); one partial plane per group.
+// register r, NR <= RHO_REGS per thread: the launch picks the smallest NR that covers the
+// plane, so no accumulator registers go unused); one partial plane per group.
 constexpr int RHO_REGS = 16;
-template <typename T>
+template <typename T, int NR>
 __global__ void __launch_bounds__(PlaneNT<T>::value)
 k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int nb, int bpg,
              double* __restrict__ rpart, long N, const int* __restrict__ col_ij, int ncol,
@@ -829,9 +830,9 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
   stage(tx, twx, n1);
   stage(ty, twy, n2);
   stage(xl, xact, nxact);
-  double acc[RHO_REGS];
+  double acc[NR];
 #pragma unroll
-  for (int r = 0; r < RHO_REGS; ++r) acc[r] = 0.0;
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
   const int bend = min(nb, (g + 1) * bpg);
   for (int b = g * bpg; b < bend; ++b) {
     for (int i = tid; i < ps * n2; i += nthr) lds[i] = mk((T)0, (T)0);
@@ -845,7 +846,7 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
     __syncthreads();
     const double wb = wts[b];
 #pragma unroll
-    for (int r = 0; r < RHO_REGS; ++r) {
+    for (int r = 0; r < NR; ++r) {
       const int i = tid + r * nthr;
       if (i < np) {
         const int j = i / n1, x = i - j * n1;
@@ -857,7 +858,7 @@ k_plane_rhoe(const cplx<T>* __restrict__ W, const double* __restrict__ wts, int 
   }
   double* out = rpart + (long)g * N + (long)z * np;
 #pragma unroll
-  for (int r = 0; r < RHO_REGS; ++r) {
+  for (int r = 0; r < NR; ++r) {
     const int i = tid + r * nthr;
     if (i < np) out[i] = acc[r];
   }
@@ -1166,10 +1167,14 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
       hipLaunchKernelGGL(k_plane_rhoe_c<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
                          d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_cy, k.ncol,
                          k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
-    else
-      hipLaunchKernelGGL(k_plane_rhoe<T>, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W,
-                         d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol,
-                         k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+    else {
+      const int nr = (np + PlaneNT<T>::value - 1) / PlaneNT<T>::value;  // <= RHO_REGS here
+      auto kern = nr <= 4 ? k_plane_rhoe<T, 4> : nr <= 6 ? k_plane_rhoe<T, 6> : nr <= 8 ? k_plane_rhoe<T, 8>
+                : nr <= 12 ? k_plane_rhoe<T, 12> : k_plane_rhoe<T, RHO_REGS>;
+      hipLaunchKernelGGL(kern, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, d_w + b0, nbb,
+                         bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol, k.d_xact, k.nxact,
+                         c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
+    }
     hipLaunchKernelGGL(k_rho_reduce, dim3(1024), dim3(256), 0, c->stream, c->d_rho,
                        reinterpret_cast<const double*>(c->d_rpart), ngu, c->N, 1.0 / c->omega);
   }
