@@ -38,9 +38,10 @@ def _rdiv_U(W, U):
     return sla.solve_triangular(U, W.T, trans="T", lower=False).T
 
 
-def diag_LOBPCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
+def diag_LOBPCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0, stats=None):
     """diag_LOBPCG.jl:51-225 for one k-point.  X (Nbasis x Nstates, orthonormal) is
-    overwritten in place; returns (evals, iterations, number of opH columns applied)."""
+    overwritten in place; returns (evals, iterations, number of opH columns applied).
+    stats (a dict, optional) receives "locks": the iterations that took the lock branch."""
     Nbasis, Nstates = X.shape
     if Nstates_conv == 0:
         Nstates_conv = Nstates
@@ -66,6 +67,8 @@ def diag_LOBPCG(opH, prec, X, tol=1e-5, NiterMax=100, Nstates_conv=0):
             break
         W = prec(R)
         if Nlock > 0:
+            if stats is not None:
+                stats["locks"] = stats.get("locks", 0) + 1
             # lock branch, diag_LOBPCG.jl:125-164 (the `ww = ww - ...` update at :130-133
             # rebinds a local in the reference and changes nothing; it is omitted)
             Xlock = X[:, ilock].copy()

@@ -118,6 +118,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     D.copy(Nb, Nst, X.p, x)
     D.opH(x, hx, Nst, Nb)
     ncols = Nst
+    nlock = 0
     evals_old = np.zeros(Nst)
     SMALL = 10.0 * EPS
     it_done = 0
@@ -147,6 +148,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
             break
         D.kprec(w, w, Nst, Nb)
         if len(ilock) > 0:
+            nlock += 1
             ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact)
             continue
         D.opH(w, hw, Nst, Nb)
@@ -195,7 +197,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     c.call("pw_synchronize")
     for m in (Q, HQ, Tmp, Sm, Gm, Cm, S2, S3):
         m.free()
-    return evals, {"iter": it_done, "ncols": ncols}
+    return evals, {"iter": it_done, "ncols": ncols, "locks": nlock}
 
 
 DENSE = "device"   # where diag_LOBPCG_ solves its small dense problems: "device" or "host"
@@ -208,7 +210,8 @@ MIXED_TOL_FLOOR = 1e-6
 def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, dense=None):
     """diag_LOBPCG! for the current Ham.ik (diag_LOBPCG.jl:51-225).  X is a DeviceMatrix
     (Ngw x Nst, orthonormal) overwritten with the Ritz vectors.  Returns (evals, stats)
-    where stats = {"iter": iterations, "ncols": op_H columns applied}.  dense = "device"
+    where stats = {"iter": iterations, "ncols": op_H columns applied, "locks": iterations
+    that took the lock branch}.  dense = "device"
     (default) runs the Cholesky and eigen problems through rocSOLVER on the context stream;
     "host" runs them with LAPACK on the host, as the reference does."""
     mixed = Ham.mixed if mixed is None else mixed
@@ -232,6 +235,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     D.copy(Nb, Nst, X.p, x)
     D.opH(x, hx, Nst, Nb)
     ncols = Nst
+    nlock = 0
     evals_old = np.zeros(Nst)
     SMALL = 10.0 * EPS
     it_done = 0
@@ -252,6 +256,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
             break
         D.kprec(w, w, Nst, Nb)
         if len(ilock) > 0:
+            nlock += 1
             ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact)
             continue
         D.opH(w, hw, Nst, Nb)
@@ -296,7 +301,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     c.call("pw_synchronize")
     for m in (Q, HQ, Tmp):
         m.free()
-    return evals, {"iter": it_done, "ncols": ncols}
+    return evals, {"iter": it_done, "ncols": ncols, "locks": nlock}
 
 
 def _gather(D, src_cols, cols, dst, Nb):

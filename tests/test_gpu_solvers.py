@@ -150,3 +150,46 @@ def test_scf_si_fcc_pbe_matches_reference_log():
     print("case scf si fcc pbe", flush=True)
     out = ph.KS_solve_SCF_(H, betamix=0.5, etot_conv_thr=1e-7, verbose=True)
     assert abs(out["energies"]["Total"] - (-7.8539433194)) < 1e-6, out["energies"]
+
+
+@pytest.mark.parametrize("dense", ["device", "host"])
+def test_lobpcg_lock_branch_vs_oracle(dense):
+    """The lock branch (diag_LOBPCG.jl:125-164) on the device: with the lock threshold
+    raised from 10 eps to 1e-8, bands pass through it on the way to tol = 1e-11.  The
+    device solver and the oracle both take it from the same start, and both end on the
+    dense eigenvalues of the explicit H to the accuracy a locked band keeps (1e-6)."""
+    import pwdft_hip as ph
+    import pwdft_hip.solvers as PS
+    import oracle.solvers as OS
+    from oracle import ops as O
+    from oracle.setup import gen_lattice_fcc
+    sym, frac = si_fcc_structure()
+    oH = oracle_ham(sym, frac, gen_lattice_fcc(A_SI), "Si-q4.lda.gth", 15.0, meshk=(2, 2, 2), extra_states=4)
+    H = _si(meshk=(2, 2, 2))
+    rho = H.guess_rhoe()
+    H.update(rho)
+    oH.update(rho)
+    H.ik = oH.ik = 1
+    n = H.pw.Ngw[1]
+    Hd = O.op_H(oH, np.eye(n, dtype=complex))
+    exact = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))[:8]
+    a = rand_psi(np.random.default_rng(21), n, 8)
+    X = ph.DeviceMatrix.from_host(H.ctx, a)
+    ph.ortho_sqrt_(H, X)
+    X0 = X.download()
+    saved = (PS.EPS, OS.EPS)
+    PS.EPS = OS.EPS = 1e-9
+    print(f"case LOBPCG lock branch dense={dense}", flush=True)
+    try:
+        ev, st = PS.diag_LOBPCG_(H, X, tol=1e-11, NiterMax=300, dense=dense)
+        ost = {}
+        Xo = X0.copy()
+        oev, _, _ = OS.diag_LOBPCG(lambda Y: O.op_H(oH, Y), lambda R: O.Kprec(oH, R), Xo, tol=1e-11, NiterMax=300,
+                                   stats=ost)
+    finally:
+        PS.EPS, OS.EPS = saved
+    assert st["locks"] >= 1 and ost.get("locks", 0) >= 1, (st, ost)
+    assert np.abs(ev - exact).max() < 1e-6, (ev, exact)
+    assert np.abs(oev - exact).max() < 1e-6, (oev, exact)
+    Xd = X.download()
+    assert np.abs(Xd.conj().T @ Xd - np.eye(8)).max() < 1e-10

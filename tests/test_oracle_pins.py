@@ -360,3 +360,75 @@ def test_pbe_potential_is_functional_derivative():
     fd = (Exc(rho + h * phi) - Exc(rho - h * phi)) / (2 * h)
     an = float(np.sum(V * phi) * dV)
     assert abs(fd - an) < 1e-7 * max(1.0, abs(an)), (fd, an)
+
+
+def test_entropy_is_grand_potential_identity():
+    """Fermi-Dirac smearing: sum 2 w f (e - E_F) + (-TS) equals the grand potential
+    -kT sum 2 w ln(1 + exp((E_F - e)/kT)) term by term (f ln f + (1-f) ln(1-f) =
+    f x - ln(1 + e^x), x = (E_F - e)/kT).  A dropped factor 2, weight, kT or sign in the
+    oracle's calc_entropy (occupations.jl:222-245) breaks it."""
+    from oracle.energy import calc_entropy, calc_Focc
+    rng = np.random.default_rng(11)
+    nst, nk = 12, 7
+    evals = np.sort(rng.uniform(-0.4, 0.6, (nst, nk)), axis=0)
+    wk = rng.uniform(0.5, 1.5, nk)
+    wk /= wk.sum()
+    for kT in (0.003, 0.01, 0.05):
+        Focc, Ef = calc_Focc(9.0, wk, kT, evals)
+        mTS = calc_entropy(wk, kT, evals, Ef)
+        x = (Ef - evals) / kT
+        omega = -kT * np.sum(2 * wk[None, :] * np.logaddexp(0.0, x))
+        band = np.sum(wk[None, :] * Focc * (evals - Ef))
+        assert abs(band + mTS - omega) < 1e-12 * max(1.0, abs(omega)), (kT, band + mTS, omega)
+        assert mTS <= 0.0
+
+
+def test_fermi_level_against_brentq_and_symmetry():
+    """find_E_fermi (occupations.jl:126-183): the same root as scipy's brentq on
+    sum_k 2 w_k sum_i f(E - e_ik) = N_e; E_F = c exactly for a spectrum symmetric about c
+    at half filling; out-of-range electron counts raise."""
+    from scipy.optimize import brentq
+    from oracle.energy import find_E_fermi, sumkg
+    rng = np.random.default_rng(12)
+    nst, nk = 10, 5
+    evals = np.sort(rng.uniform(-0.5, 0.5, (nst, nk)), axis=0)
+    wk = np.full(nk, 1.0 / nk)
+    for kT, Ne in ((0.01, 7.0), (0.02, 11.5), (0.005, 4.0)):
+        Ef = find_E_fermi(Ne, wk, kT, evals)
+        ref = brentq(lambda e: sumkg(wk, kT, evals, e) - Ne, -2.0, 2.0, xtol=1e-15, rtol=4 * np.finfo(float).eps)
+        assert abs(Ef - ref) < 1e-8, (kT, Ne, Ef, ref)
+        assert abs(sumkg(wk, kT, evals, Ef) - Ne) < 1e-9
+    c = 0.3
+    d = np.sort(rng.uniform(0.01, 0.4, 4))
+    sym = np.concatenate([c - d[::-1], c + d])[:, None]          # 8 levels, symmetric about c
+    Ef = find_E_fermi(8.0, np.ones(1), 0.02, sym)                # half of 2 x 8 states filled
+    assert abs(Ef - c) < 1e-9, Ef
+    with pytest.raises(RuntimeError):
+        find_E_fermi(2.0 * nst + 1.0, wk, 0.01, evals)
+
+
+def test_lobpcg_lock_branch_converges_to_dense():
+    """The lock branch of diag_LOBPCG.jl:125-164 (taken for bands whose Ritz value moved
+    by <= 10 eps; the threshold is raised to 1e-8 here so that bands pass through it on the
+    way to tol = 1e-11): the iteration still ends on the lowest eigenpairs of a dense
+    Hermitian H, with orthonormal Ritz vectors."""
+    import oracle.solvers as OS
+    rng = np.random.default_rng(13)
+    n, nst = 80, 6
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = 0.05 * (A + A.conj().T) + np.diag(np.linspace(-2.0, 6.0, n))
+    exact = np.linalg.eigvalsh(H)[:nst]
+    X = rand_wavefunc(rng, n, nst)
+    d = np.real(np.diag(H)) + 3.0
+    saved = OS.EPS
+    OS.EPS = 1e-9
+    try:
+        st = {}
+        ev, it, _ = diag_LOBPCG(lambda Y: H @ Y, lambda R: R / d[:, None], X, tol=1e-11, NiterMax=300, stats=st)
+    finally:
+        OS.EPS = saved
+    assert st.get("locks", 0) >= 1, st
+    # a locked band stops moving once its Ritz value changes by <= 1e-8 per iteration
+    assert np.abs(ev - exact).max() < 1e-6, (ev, exact)
+    assert np.allclose(X.conj().T @ X, np.eye(nst), atol=1e-10)
+    assert np.abs(H @ X - X * ev[None, :]).max() < 1e-3
