@@ -17,9 +17,12 @@ Modules:
              V_Ps_loc, k-point grids
   ops        op_K, op_V_loc, op_V_Ps_nloc, op_H, Kprec, calc_betaNL_psi
   density    calc_rhoe, guess_rhoe, Poisson_solve, Slater+VWN XC
-  solvers    ortho_sqrt, diag_LOBPCG, diag_davidson
+  solvers    ortho_sqrt, diag_LOBPCG, diag_davidson, diag_davidson_qe, diag_CheFSI
+             (chebyfilt, Lanczos bounds), diag_Emin_PCG, calc_grad
   energy     energies, Ewald E_NN, occupations (Fermi-Dirac), KS_solve_SCF
   forces     calc_forces_Ps_loc, calc_forces_Ps_nloc, calc_forces_NN
   symmetry   symmetrize_rhoe
   mixing     mix_broyden, mix_pulay, mix_rpulay, mix_ppulay, mix_anderson, mix_adaptive
+  nloc_upf   the norm-conserving UPF nonlocal operator (dense Deeq blocks), its energy
+             and forces
 """
