@@ -17,15 +17,24 @@ import ctypes
 import numpy as np
 
 from ._lib import DeviceMatrix, PosDefException, PW_TERM_ALL, ptr
-from .solvers import EPS, MIXED_TOL_FLOOR, _Dense, _lobpcg_lock_step
+from .solvers import EPS, MIXED_TOL_FLOOR, _Dense, _LockScratch, _lobpcg_lock_step
 
 _ONE = np.array([1.0, 0.0])
 _ZERO = np.array([0.0, 0.0])
 _MONE = np.array([-1.0, 0.0])
 
 
-def _pa(vals):
-    return (ctypes.c_void_p * len(vals))(*vals)
+def _pp(vals):
+    """Device pointers as a contiguous uint64 array: the layout of the C ABI's void*[]
+    arguments.  The caller keeps the array alive until the call returns."""
+    return np.ascontiguousarray(vals, dtype=np.uint64)
+
+
+def _ints(v, n, dtype):
+    """Per-product sizes: a scalar repeated n times, or one value per product."""
+    if np.ndim(v) == 0:
+        return np.full(n, v, dtype=dtype)
+    return np.ascontiguousarray(v, dtype=dtype)
 
 
 class _Slab:
@@ -34,13 +43,19 @@ class _Slab:
 
     def __init__(self, ctx, rows, ncols):
         self.rows = [int(r) for r in rows]
+        self.rows_a = np.asarray(self.rows, dtype=np.uint64)
         self.ncols = ncols
         self.off = np.concatenate([[0], np.cumsum([16 * r * ncols for r in self.rows])]).astype(np.int64)
         self.mem = DeviceMatrix(ctx, int(self.off[-1] // 16) or 1, 1)
         self.base = self.mem.p.value
+        self.start = np.uint64(self.base) + self.off[:-1].astype(np.uint64)
 
     def col(self, k, j):
         return self.base + int(self.off[k]) + 16 * self.rows[k] * int(j)
+
+    def cols(self, ks, j):
+        """Column j of the blocks of k-points ks (index array), as device pointers."""
+        return self.start[ks] + np.uint64(16 * int(j)) * self.rows_a[ks]
 
     def view(self, k):
         return _View(self.col(k, 0), self.rows[k], self.ncols)
@@ -73,7 +88,7 @@ class _Small:
             if self.buf is not None:
                 self.buf.free()
             self.buf = DeviceMatrix(self.ctx, max(need, 1), 1)
-        return [self.buf.p.value + 16 * r * c * i for i in range(nk)]
+        return np.uint64(self.buf.p.value) + np.uint64(16 * r * c) * np.arange(nk, dtype=np.uint64)
 
     def download(self, nk, r, c):
         a = np.empty((nk, c, r), dtype=np.complex128)
@@ -95,12 +110,11 @@ class _Batch:
 
     def gemm(self, transa, m, n, k, A, lda, B, ldb, C, ldc, alpha=_ONE, beta=_ZERO):
         nb = len(A)
-        i32 = lambda v: np.ascontiguousarray(np.broadcast_to(v, nb), dtype=np.int32)
-        i64 = lambda v: np.ascontiguousarray(np.broadcast_to(v, nb), dtype=np.int64)
-        ms, ns, ks = i32(m), i32(n), i32(k)
-        la, lb, lc = i64(lda), i64(ldb), i64(ldc)
-        self.c.call("pw_zgemm_batch_dev", transa, nb, ptr(ms), ptr(ns), ptr(ks), ptr(alpha), _pa(A), ptr(la), _pa(B),
-                    ptr(lb), ptr(beta), _pa(C), ptr(lc))
+        ms, ns, ks = _ints(m, nb, np.int32), _ints(n, nb, np.int32), _ints(k, nb, np.int32)
+        la, lb, lc = _ints(lda, nb, np.int64), _ints(ldb, nb, np.int64), _ints(ldc, nb, np.int64)
+        pa, pb, pc = _pp(A), _pp(B), _pp(C)
+        self.c.call("pw_zgemm_batch_dev", transa, nb, ptr(ms), ptr(ns), ptr(ks), ptr(alpha), ptr(pa), ptr(la),
+                    ptr(pb), ptr(lb), ptr(beta), ptr(pc), ptr(lc))
 
     def gram(self, small, A, B, na, nbc, rows):
         """A_k^H B_k for column blocks into the stacked buffer `small` (na x nbc per k-point);
@@ -111,12 +125,12 @@ class _Batch:
 
     def herm(self, p, n, mode):
         """pw_herm_fix_dev on every n x n block at p (mode 0: Hermitian(); 2: (A + A^H) / 2)."""
-        self.c.call("pw_herm_fix_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n, mode)
+        self.c.call("pw_herm_fix_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, mode)
 
     def chol_inv(self, p, n, iks):
         """Every n x n block at p <- inv(cholesky(Hermitian(block)).U), strict lower zeroed."""
         try:
-            self.c.call("pw_chol_inv_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n)
+            self.c.call("pw_chol_inv_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n)
         except PosDefException as e:
             raise PosDefException(e.code, f"{e} (batch of k-points {list(iks)})")
 
@@ -124,7 +138,7 @@ class _Batch:
         """eigen(Hermitian(block)) of every n x n block at p: eigenvectors in place,
         eigenvalues returned (len(p) x n, ascending)."""
         w = np.empty((len(p), n))
-        self.c.call("pw_heevd_batch_dev", n, len(p), ctypes.c_void_p(p[0]), n, n * n, ptr(w))
+        self.c.call("pw_heevd_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, ptr(w))
         return w
 
     def copy(self, rows, ncols, src, dst):
@@ -132,19 +146,22 @@ class _Batch:
         r = np.ascontiguousarray(rows, dtype=np.int32)
         cl = np.full(nb, ncols, dtype=np.int32)
         ld = np.ascontiguousarray(rows, dtype=np.int64)
-        self.c.call("pw_copy_cols_batch_dev", nb, ptr(r), ptr(cl), _pa(src), ptr(ld), _pa(dst), ptr(ld))
+        ps, pd = _pp(src), _pp(dst)
+        self.c.call("pw_copy_cols_batch_dev", nb, ptr(r), ptr(cl), ptr(ps), ptr(ld), ptr(pd), ptr(ld))
 
     def opH(self, iks, src, dst, ncols, rows):
         from ._lib import PW_MIXED
         ik = np.ascontiguousarray(iks, dtype=np.int32)
         ld = np.ascontiguousarray(rows, dtype=np.int64)
-        self.c.call("pw_op_H_batch_dev", len(iks), ptr(ik), self.Ham.ispin, ncols, _pa(src), ptr(ld), _pa(dst), ptr(ld),
+        ps, pd = _pp(src), _pp(dst)
+        self.c.call("pw_op_H_batch_dev", len(iks), ptr(ik), self.Ham.ispin, ncols, ptr(ps), ptr(ld), ptr(pd), ptr(ld),
                     PW_TERM_ALL | (PW_MIXED if self.mixed else 0))
 
     def kprec(self, iks, src, dst, ncols, rows):
         ik = np.ascontiguousarray(iks, dtype=np.int32)
         ld = np.ascontiguousarray(rows, dtype=np.int64)
-        self.c.call("pw_kprec_batch_dev", len(iks), ptr(ik), ncols, _pa(src), ptr(ld), _pa(dst), ptr(ld))
+        ps, pd = _pp(src), _pp(dst)
+        self.c.call("pw_kprec_batch_dev", len(iks), ptr(ik), ncols, ptr(ps), ptr(ld), ptr(pd), ptr(ld))
 
 
 def _ritz_coefs(B, sM, T, G, nq, Nst, iks):
@@ -165,7 +182,7 @@ def _ritz_coefs(B, sM, T, G, nq, Nst, iks):
 def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, iks=None):
     """diag_LOBPCG!(Ham, psiks) with all k-points iks (default: all) iterated together.
     psiks: list of DeviceMatrix (Ngw[ik] x Nst), overwritten with the Ritz vectors.
-    Returns (evals Nst x nk, stats per k-point {"iter", "ncols"})."""
+    Returns (evals Nst x nk, stats per k-point {"iter", "ncols", "locks"})."""
     mixed = Ham.mixed if mixed is None else mixed
     if mixed:
         tol = max(tol, MIXED_TOL_FLOOR)
@@ -182,14 +199,17 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     Tmp = _Slab(c, rows, 4 * Nst)
     sX, sC, sT, sG, sM = (_Small(c) for _ in range(5))   # stacked small matrices in HBM
     allk = list(range(nk))
-    cols = lambda S, ks, j: [S.col(k, j) for k in ks]
-    rws = lambda ks: [rows[k] for k in ks]
+    rows_a = np.asarray(rows, dtype=np.int64)
+    cols = lambda S, ks, j: S.cols(np.asarray(ks, dtype=np.int64), j)
+    rws = lambda ks: rows_a[np.asarray(ks, dtype=np.int64)]
     B.copy(rws(allk), Nst, [psiks[k].p.value for k in allk], cols(Q, allk, 0))
     B.opH([iks[k] for k in allk], cols(Q, allk, 0), cols(HQ, allk, 0), Nst, rws(allk))
     ncols = np.full(nk, Nst)
     iters = np.zeros(nk, dtype=int)
+    nlocks = np.zeros(nk, dtype=int)
     evals_old = np.zeros((nk, Nst))
     SMALL = 10.0 * EPS
+    lock_ws = None
     active = list(allk)
     for it in range(1, NiterMax + 1):
         if not active:
@@ -219,7 +239,11 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
                 dv = devals[a.index(k)]
                 ilock = np.flatnonzero(dv <= SMALL)
                 iact = np.flatnonzero(dv > SMALL)
-                ncols[k] += _lobpcg_lock_step(D, Q.view(k), HQ.view(k), Tmp.view(k), rows[k], Nst, ilock, iact)
+                nlocks[k] += 1
+                if lock_ws is None:
+                    lock_ws = _LockScratch(c, max(rows), Nst)
+                ncols[k] += _lobpcg_lock_step(D, Q.view(k), HQ.view(k), Tmp.view(k), rows[k], Nst, ilock, iact,
+                                              scratch=lock_ws)
         active = keep + locked
         active.sort()
         a = keep
@@ -247,7 +271,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         B.gemm(0, ra, Nst, nq, hqa, ra, pU, nq, t1, ra)
         x, hx, p, hp = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, 2 * Nst), cols(HQ, a, 2 * Nst)
         if it > 1:
-            pU23 = [u + 16 * Nst for u in pU]           # rows Nst:3Nst of the coefficients
+            pU23 = pU + np.uint64(16 * Nst)             # rows Nst:3Nst of the coefficients
             t2, t3 = cols(Tmp, a, 2 * Nst), cols(Tmp, a, 3 * Nst)
             B.gemm(0, ra, Nst, 2 * Nst, w, ra, pU23, nq, t2, ra)
             B.gemm(0, ra, Nst, 2 * Nst, hw, ra, pU23, nq, t3, ra)
@@ -271,5 +295,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     c.call("pw_synchronize")
     for s in (Q, HQ, Tmp, sX, sC, sT, sG, sM):
         s.free()
-    stats = [{"iter": int(iters[k]), "ncols": int(ncols[k])} for k in allk]
+    if lock_ws is not None:
+        lock_ws.free()
+    stats = [{"iter": int(iters[k]), "ncols": int(ncols[k]), "locks": int(nlocks[k])} for k in allk]
     return ev.T.copy(), stats

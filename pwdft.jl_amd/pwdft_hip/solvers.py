@@ -119,6 +119,7 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     D.opH(x, hx, Nst, Nb)
     ncols = Nst
     nlock = 0
+    lock_ws = None
     evals_old = np.zeros(Nst)
     SMALL = 10.0 * EPS
     it_done = 0
@@ -149,7 +150,9 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
         D.kprec(w, w, Nst, Nb)
         if len(ilock) > 0:
             nlock += 1
-            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact)
+            if lock_ws is None:
+                lock_ws = _LockScratch(c, Nb, Nst)
+            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact, scratch=lock_ws)
             continue
         D.opH(w, hw, Nst, Nb)
         ncols += Nst
@@ -197,6 +200,8 @@ def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
     c.call("pw_synchronize")
     for m in (Q, HQ, Tmp, Sm, Gm, Cm, S2, S3):
         m.free()
+    if lock_ws is not None:
+        lock_ws.free()
     return evals, {"iter": it_done, "ncols": ncols, "locks": nlock}
 
 
@@ -236,6 +241,7 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     D.opH(x, hx, Nst, Nb)
     ncols = Nst
     nlock = 0
+    lock_ws = None
     evals_old = np.zeros(Nst)
     SMALL = 10.0 * EPS
     it_done = 0
@@ -257,7 +263,9 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
         D.kprec(w, w, Nst, Nb)
         if len(ilock) > 0:
             nlock += 1
-            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact)
+            if lock_ws is None:
+                lock_ws = _LockScratch(c, Nb, Nst)
+            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact, scratch=lock_ws)
             continue
         D.opH(w, hw, Nst, Nb)
         ncols += Nst
@@ -301,6 +309,8 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     c.call("pw_synchronize")
     for m in (Q, HQ, Tmp):
         m.free()
+    if lock_ws is not None:
+        lock_ws.free()
     return evals, {"iter": it_done, "ncols": ncols, "locks": nlock}
 
 
@@ -314,16 +324,64 @@ def _col(M, j, Nb):
     return c_void_p(M.value + 16 * Nb * int(j)) if isinstance(M, c_void_p) else M.col(j)
 
 
-def _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact):
+class _Block:
+    """Column block inside a larger device allocation (leading dimension rows), with the
+    DeviceMatrix members the dense helpers use."""
+
+    def __init__(self, p, rows, cols):
+        self.p = ctypes.c_void_p(int(p))
+        self.rows, self.cols = rows, cols
+
+    def col(self, j):
+        return ctypes.c_void_p(self.p.value + 16 * self.rows * int(j))
+
+
+class _LockScratch:
+    """Device memory of the lock step, allocated once per solver call instead of at every
+    lock: the Xlock, [Xact W], [HXact HW] and staging blocks (Nb x (nl + 5 na) <= Nb x 5 Nst
+    columns for nl + na = Nst) and one buffer for the small coefficient matrices.  Uploads
+    into that buffer are ordered on the context stream after the products that read its
+    previous contents."""
+
+    def __init__(self, ctx, rows_max, Nst):
+        self.ctx = ctx
+        self.rows_max = rows_max
+        self.mem = DeviceMatrix(ctx, rows_max, 5 * Nst)
+        self.small = DeviceMatrix(ctx, 4 * Nst * Nst, 1)
+
+    def blocks(self, Nb, nl, na):
+        assert Nb <= self.rows_max and nl + 5 * na <= self.mem.cols
+        b, s = self.mem.p.value, 16 * Nb
+        return (_Block(b, Nb, nl), _Block(b + s * nl, Nb, 2 * na), _Block(b + s * (nl + 2 * na), Nb, 2 * na),
+                _Block(b + s * (nl + 4 * na), Nb, na))
+
+    def upload(self, M):
+        a = np.asfortranarray(M, dtype=np.complex128)
+        assert a.size <= self.small.rows
+        self.ctx.call("pw_memcpy_h2d", self.small.p, ptr(a), a.nbytes)
+        return _Block(self.small.p.value, a.shape[0], a.shape[1] if a.ndim > 1 else 1)
+
+    def free(self):
+        self.mem.free()
+        self.small.free()
+
+
+def _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact, scratch=None):
     """Lock branch of diag_LOBPCG.jl:125-164 (the W projection with Ylock at :130-133 is
-    a rebinding of a local in the reference and has no effect; it is not performed)."""
+    a rebinding of a local in the reference and has no effect; it is not performed).
+    scratch: a _LockScratch to take the device blocks from (else they are allocated)."""
     c = D.c
     nl, na = len(ilock), len(iact)
     x, w, hx = Q.col(0), Q.col(Nst), HQ.col(0)
-    Xl = DeviceMatrix(c, Nb, nl)
-    A = DeviceMatrix(c, Nb, 2 * na)     # [Xact Wact]
-    HA = DeviceMatrix(c, Nb, 2 * na)    # [HXact HW]
-    T2 = DeviceMatrix(c, Nb, 2 * na)
+    if scratch is None:
+        own = [DeviceMatrix(c, Nb, nl), DeviceMatrix(c, Nb, 2 * na), DeviceMatrix(c, Nb, 2 * na),
+               DeviceMatrix(c, Nb, na)]
+        Xl, A, HA, T2 = own                 # Xlock, [Xact Wact], [HXact HW], staging
+        up = D.upload_small
+    else:
+        own = []
+        Xl, A, HA, T2 = scratch.blocks(Nb, nl, na)
+        up = scratch.upload
     _gather(D, lambda j: Q.col(j), ilock, Xl, Nb)
     _gather(D, lambda j: Q.col(j), iact, A, Nb)
     _gather(D, lambda j: Q.col(Nst + j), iact, T2, Nb)
@@ -332,14 +390,14 @@ def _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact):
     def project(block_ptr, ncols):
         # block -= Xlock (Xlock' block)
         M = D.gram(Xl.p, block_ptr, nl, ncols, Nb)
-        dM = D.upload_small(M)
+        dM = up(M)
         D.gemm(0, Nb, ncols, nl, Xl.p, Nb, dM.p, nl, block_ptr, Nb, alpha=_MONE, beta=_ONE)
 
     project(A.p, 2 * na)
     D.opH(A.p, HA.p, 2 * na, Nb)
     project(HA.p, 2 * na)
     U, Ui = _chol_upper_inv(D.gram(A.col(na), A.col(na), na, na, Nb))
-    dUi = D.upload_small(Ui)
+    dUi = up(Ui)
     D.gemm(0, Nb, na, na, A.col(na), Nb, dUi.p, na, T2.p, Nb)
     D.copy(Nb, na, T2.p, A.col(na))
     D.gemm(0, Nb, na, na, HA.col(na), Nb, dUi.p, na, T2.p, Nb)
@@ -347,18 +405,18 @@ def _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact):
     T = herm_u(D.gram(A.p, HA.p, 2 * na, 2 * na, Nb))
     G = herm_u(D.gram(A.p, A.p, 2 * na, 2 * na, Nb))
     _, S = sla.eigh(T, G)
-    dU = D.upload_small(np.asfortranarray(S[:, :na]))
+    dU = up(np.asfortranarray(S[:, :na]))
     # X = [Xlock Xact]
     D.copy(Nb, nl, Xl.p, x)
     D.gemm(0, Nb, na, 2 * na, A.p, Nb, dU.p, 2 * na, Q.col(nl), Nb)
     D.opH(x, Tmp.p, Nst, Nb)
     T = herm_u(D.gram(x, Tmp.p, Nst, Nst, Nb))
     _, ev = np.linalg.eigh(T)
-    dE = D.upload_small(ev)
+    dE = up(ev)
     D.gemm(0, Nb, Nst, Nst, x, Nb, dE.p, Nst, Tmp.p, Nb)
     D.copy(Nb, Nst, Tmp.p, x)
     D.opH(x, hx, Nst, Nb)
-    for m in (Xl, A, HA, T2):
+    for m in own:
         m.free()
     return 2 * na + 2 * Nst
 

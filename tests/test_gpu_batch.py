@@ -216,6 +216,37 @@ def test_lobpcg_batch_vs_per_k(al):
         assert np.abs(X.conj().T @ X - np.eye(8)).max() < 1e-10
 
 
+def test_lobpcg_batch_lock_branch(al):
+    """The batched solver's lock path (a k-point that reaches the lock branch finishes the
+    iteration through the per-k lock step, with device blocks reused across k-points): with
+    the lock threshold raised from 10 eps to 1e-8 the k-points lock on the way to
+    tol = 1e-11, and the eigenvalues still equal the per-k solver's (no lock) to the accuracy
+    a locked band keeps (1e-6); the Ritz vectors stay orthonormal."""
+    import pwdft_hip as ph
+    import pwdft_hip.batch as BT
+    import pwdft_hip.solvers as PS
+    H, _ = al
+    iks = list(range(0, 182, 9))
+    print(f"case Al LOBPCG batch lock branch, {len(iks)} k-points", flush=True)
+    ps = ph.rand_BlochWavefunc(H, seed=9)
+    sub = [ps[k] for k in iks]
+    saved = (PS.EPS, BT.EPS)
+    PS.EPS = BT.EPS = 1e-9
+    try:
+        ev, st = ph.diag_LOBPCG_batch(H, sub, tol=1e-11, NiterMax=200, iks=iks)
+    finally:
+        PS.EPS, BT.EPS = saved
+    assert sum(x["locks"] for x in st) >= 1, st
+    ps2 = ph.rand_BlochWavefunc(H, seed=9)
+    for j, ik in enumerate(iks[::5]):
+        H.ik = ik
+        ev2, _ = PS.diag_LOBPCG_(H, ps2[ik], tol=1e-11, NiterMax=200)
+        assert np.abs(ev[:, iks.index(ik)] - ev2).max() < 1e-6, (ik, ev[:, iks.index(ik)], ev2)
+    for m in sub:
+        X = m.download()
+        assert np.abs(X.conj().T @ X - np.eye(8)).max() < 1e-10
+
+
 def test_scf_batched_vs_per_k():
     """Al fcc 4x4x4 shifted (10 k-points, Fermi-Dirac smearing 0.01 Ha): the SCF with the
     batched eigensolver converges to the per-k SCF's free energy."""
