@@ -319,6 +319,15 @@ int pw_herm_fix_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, 
  * matrix in one call (diag_LOBPCG.jl:106-118, 193-203 over many k-points).
  * PW_ERR_NOT_POSDEF names the first matrix that is not positive definite. */
 int pw_chol_inv_batch_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA);
+/* The same without waiting: the potrf and trtri info of matrix i go to d_info[i] and
+ * d_info[nbatch + i] (device memory, 2 * nbatch ints; nonzero = not positive definite /
+ * singular), for the caller to read when it next synchronises.  The batched LOBPCG
+ * iteration reads them together with its eigenvalues, once per iteration. */
+int pw_chol_inv_batch_async_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int* d_info);
+/* pw_heevd_batch_dev without waiting: eigenvalues into d_w (device, n x nbatch), the
+ * convergence info of matrix i into d_info[i] (device; nonzero = no convergence). */
+int pw_heevd_batch_async_dev(pw_ctx* ctx, int n, int nbatch, void* A, int64_t lda, int64_t strideA, double* d_w,
+                             int* d_info);
 
 /* ---- device memory (for device-resident psi across LOBPCG, calc_rhoe and energies) */
 

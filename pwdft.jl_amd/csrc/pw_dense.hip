@@ -130,22 +130,12 @@ int pw_herm_fix_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, in
   return herm_fix_launch(c, n, nbatch, A, lda, strideA, mode);
 }
 
-int pw_chol_inv_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA) {
-  CTX_CHECK(c);
-  hipSetDevice(c->device);
-  if (n < 0 || nbatch < 0 || nbatch > 65535 || (n > 0 && nbatch > 0 && !A) || lda < (n > 0 ? n : 1) ||
-      strideA < lda * (int64_t)n)
-    return set_error(c, PW_ERR_ARG, "pw_chol_inv_batch_dev: bad arguments");
-  if (n == 0 || nbatch == 0) return PW_OK;
-  int rc = ensure_solver(c);
-  if (rc) return rc;
-  // one potrf info and one trtri info per matrix
-  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, 2 * (size_t)nbatch * sizeof(int));
-  if (rc) return rc;
-  int* dinfo = reinterpret_cast<int*>(c->d_dwork);
+// potrf + trtri + lower-triangle clear of a strided stack; one potrf info and one trtri
+// info per matrix into dinfo[0..nbatch) and dinfo[nbatch..2 nbatch).  No synchronisation.
+static int chol_inv_batch_enqueue(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int* dinfo) {
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   rocblas_double_complex* a = reinterpret_cast<rocblas_double_complex*>(A);
-  rc = st_check(c, rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, a, (int)lda, (rocblas_stride)strideA,
+  int rc = st_check(c, rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, a, (int)lda, (rocblas_stride)strideA,
                                                     dinfo, nbatch),
                 "zpotrf_strided_batched");
   if (rc) return rc;
@@ -153,7 +143,26 @@ int pw_chol_inv_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, in
                                                     (rocblas_stride)strideA, dinfo + nbatch, nbatch),
                 "ztrtri_strided_batched");
   if (rc) return rc;
-  rc = herm_fix_launch(c, n, nbatch, A, lda, strideA, 1);
+  return herm_fix_launch(c, n, nbatch, A, lda, strideA, 1);
+}
+
+static bool chol_inv_args_ok(int n, int nbatch, const void* A, int64_t lda, int64_t strideA) {
+  return !(n < 0 || nbatch < 0 || nbatch > 65535 || (n > 0 && nbatch > 0 && !A) || lda < (n > 0 ? n : 1) ||
+           strideA < lda * (int64_t)n);
+}
+
+int pw_chol_inv_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (!chol_inv_args_ok(n, nbatch, A, lda, strideA))
+    return set_error(c, PW_ERR_ARG, "pw_chol_inv_batch_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, 2 * (size_t)nbatch * sizeof(int));
+  if (rc) return rc;
+  int* dinfo = reinterpret_cast<int*>(c->d_dwork);
+  rc = chol_inv_batch_enqueue(c, n, nbatch, A, lda, strideA, dinfo);
   if (rc) return rc;
   std::vector<int> info(2 * (size_t)nbatch);
   hipError_t e = hipMemcpyAsync(info.data(), dinfo, info.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -276,6 +285,38 @@ int pw_heevd_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64
       return set_error(c, PW_ERR_ARG, "eigen: no convergence for matrix " + std::to_string(i) + " (" +
                                           std::to_string(info[i]) + ")");
   return PW_OK;
+}
+
+int pw_chol_inv_batch_async_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int* d_info) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (!chol_inv_args_ok(n, nbatch, A, lda, strideA) || (n > 0 && nbatch > 0 && !d_info))
+    return set_error(c, PW_ERR_ARG, "pw_chol_inv_batch_async_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  return chol_inv_batch_enqueue(c, n, nbatch, A, lda, strideA, d_info);
+}
+
+int pw_heevd_batch_async_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, double* d_w,
+                             int* d_info) {
+  CTX_CHECK(c);
+  hipSetDevice(c->device);
+  if (n < 0 || nbatch < 0 || (n > 0 && nbatch > 0 && (!A || !d_w || !d_info)) || lda < (n > 0 ? n : 1) ||
+      strideA < lda * (int64_t)n)
+    return set_error(c, PW_ERR_ARG, "pw_heevd_batch_async_dev: bad arguments");
+  if (n == 0 || nbatch == 0) return PW_OK;
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  // off-diagonals of the tridiagonal forms
+  rc = ensure(c, &c->d_dwork, &c->dwork_bytes, (size_t)n * nbatch * sizeof(double));
+  if (rc) return rc;
+  double* E = reinterpret_cast<double*>(c->d_dwork);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  return st_check(c, rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n,
+                                                       reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
+                                                       (rocblas_stride)strideA, d_w, n, E, n, d_info, nbatch),
+                  "zheevd_strided_batched");
 }
 
 int pw_hegvd_dev(pw_ctx* c, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w) {

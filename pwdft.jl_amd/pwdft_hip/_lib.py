@@ -62,6 +62,8 @@ _PROTOS = {
     "pw_heevd_batch_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp],
     "pw_herm_fix_batch_dev": [_vp, _i, _i, _vp, _i64, _i64, _i],
     "pw_chol_inv_batch_dev": [_vp, _i, _i, _vp, _i64, _i64],
+    "pw_chol_inv_batch_async_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp],
+    "pw_heevd_batch_async_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp, _vp],
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -16,7 +16,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DeviceMatrix, PosDefException, PW_TERM_ALL, ptr
+from ._lib import DeviceMatrix, PosDefException, PWError, PW_ERR_NOT_POSDEF, PW_TERM_ALL, ptr
 from .solvers import EPS, MIXED_TOL_FLOOR, _Dense, _LockScratch, _lobpcg_lock_step
 
 _ONE = np.array([1.0, 0.0])
@@ -100,6 +100,50 @@ class _Small:
             self.buf.free()
 
 
+class _Infos:
+    """Device ints that the asynchronous batched Cholesky and eigensolver calls of one
+    iteration write their per-matrix status to; read back and checked once per iteration,
+    so that those calls do not wait for the device."""
+
+    def __init__(self, ctx, nslots):
+        self.ctx = ctx
+        self.cap = nslots
+        self.mem = DeviceMatrix(ctx, nslots // 4 + 1, 1)
+        self.used = []
+        self.next = 0
+
+    def take(self, count, what, iks):
+        if self.next + count > self.cap:
+            self.check()
+        off = self.next
+        self.next += count
+        self.used.append((off, count, what, list(iks)))
+        return ctypes.c_void_p(self.mem.p.value + 4 * off)
+
+    def check(self):
+        """Read every status written since the last check; raise for the first failure
+        (PosDefException for a Cholesky factor, PWError for an eigensolver)."""
+        if not self.used:
+            return
+        h = np.empty(self.next, dtype=np.int32)
+        self.ctx.call("pw_memcpy_d2h", ptr(h), self.mem.p, h.nbytes)
+        used, self.used, self.next = self.used, [], 0
+        for off, cnt, what, iks in used:
+            bad = np.flatnonzero(h[off:off + cnt])
+            if bad.size == 0:
+                continue
+            i = int(bad[0])
+            if what == "cholesky":
+                nb = cnt // 2
+                kind = "leading minor" if i < nb else "singular diagonal of the factor"
+                raise PosDefException(PW_ERR_NOT_POSDEF, f"cholesky (k-point {iks[i % nb]}): {kind} "
+                                                         f"{int(h[off + i])}")
+            raise PWError(-1, f"eigen: no convergence for k-point {iks[i]} ({int(h[off + i])})")
+
+    def free(self):
+        self.mem.free()
+
+
 class _Batch:
     """Batched device operations on the active k-points."""
 
@@ -127,8 +171,13 @@ class _Batch:
         """pw_herm_fix_dev on every n x n block at p (mode 0: Hermitian(); 2: (A + A^H) / 2)."""
         self.c.call("pw_herm_fix_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, mode)
 
-    def chol_inv(self, p, n, iks):
-        """Every n x n block at p <- inv(cholesky(Hermitian(block)).U), strict lower zeroed."""
+    def chol_inv(self, p, n, iks, infos=None):
+        """Every n x n block at p <- inv(cholesky(Hermitian(block)).U), strict lower zeroed.
+        With infos the call does not wait; the status is checked at infos.check()."""
+        if infos is not None:
+            d = infos.take(2 * len(p), "cholesky", iks)
+            self.c.call("pw_chol_inv_batch_async_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, d)
+            return
         try:
             self.c.call("pw_chol_inv_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n)
         except PosDefException as e:
@@ -140,6 +189,12 @@ class _Batch:
         w = np.empty((len(p), n))
         self.c.call("pw_heevd_batch_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, ptr(w))
         return w
+
+    def eigh_async(self, p, n, w_dev, infos, iks):
+        """eigen(Hermitian(block)) of every n x n block at p, eigenvectors in place and
+        eigenvalues into the device array w_dev, without waiting (status at infos.check())."""
+        d = infos.take(len(p), "eigen", iks)
+        self.c.call("pw_heevd_batch_async_dev", n, len(p), ctypes.c_void_p(int(p[0])), n, n * n, w_dev, d)
 
     def copy(self, rows, ncols, src, dst):
         nb = len(src)
@@ -164,17 +219,17 @@ class _Batch:
         self.c.call("pw_kprec_batch_dev", len(iks), ptr(ik), ncols, ptr(ps), ptr(ld), ptr(pd), ptr(ld))
 
 
-def _ritz_coefs(B, sM, T, G, nq, Nst, iks):
+def _ritz_coefs(B, sM, T, G, nq, Nst, iks, infos, w_dev):
     """Lowest Nst eigenvectors of eigen(Hermitian(T), Hermitian(G)) for every stacked pair
     (diag_LOBPCG.jl:176-190), on the device: G = U^H U, C = U^-H T U^-1, C y = l y,
     x = U^-1 y (the reduction zhegv performs).  T and G (nq x nq blocks) are overwritten;
     returns the pointers of the nq x Nst coefficient blocks (leading dimension nq) in sM."""
     B.herm(T, nq, 0)
-    B.chol_inv(G, nq, iks)
+    B.chol_inv(G, nq, iks, infos)
     M = sM.ptrs(len(T), nq, nq)
     B.gemm(0, nq, nq, nq, T, nq, G, nq, M, nq)      # T Ui
     B.gemm(1, nq, nq, nq, G, nq, M, nq, T, nq)      # Ui^H T Ui
-    B.eigh(T, nq)
+    B.eigh_async(T, nq, w_dev, infos, iks)
     B.gemm(0, nq, Nst, nq, G, nq, T, nq, M, nq)     # Ui y, lowest Nst
     return M
 
@@ -198,6 +253,8 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     HQ = _Slab(c, rows, 3 * Nst)
     Tmp = _Slab(c, rows, 4 * Nst)
     sX, sC, sT, sG, sM = (_Small(c) for _ in range(5))   # stacked small matrices in HBM
+    infos = _Infos(c, 8 * nk)                             # statuses of one iteration's dense calls
+    w_dev = DeviceMatrix(c, (3 * Nst * nk + 1) // 2, 1)   # Rayleigh-Ritz eigenvalues (unused on the host)
     allk = list(range(nk))
     rows_a = np.asarray(rows, dtype=np.int64)
     cols = lambda S, ks, j: S.cols(np.asarray(ks, dtype=np.int64), j)
@@ -220,6 +277,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         x, hx, w, hw = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, Nst), cols(HQ, a, Nst)
         pX = B.gram(sX, x, hx, Nst, Nst, rws(a))
         B.herm(pX, Nst, 0)
+        infos.check()                                   # the previous iteration's factorisations
         evals = np.linalg.eigvalsh(sX.download(len(a), Nst, Nst))   # eigvals(Hermitian(XHX))
         devals = np.abs(evals - evals_old[a])
         evals_old[a] = evals
@@ -256,7 +314,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         B.opH(ka, w, hw, Nst, ra)
         ncols[a] += Nst
         pC = B.gram(sC, w, w, Nst, Nst, ra)
-        B.chol_inv(pC, Nst, ka)                          # inv(cholesky(Hermitian(W'W)).U)
+        B.chol_inv(pC, Nst, ka, infos)                   # inv(cholesky(Hermitian(W'W)).U)
         t0, t1 = cols(Tmp, a, 0), cols(Tmp, a, Nst)
         B.gemm(0, ra, Nst, Nst, w, ra, pC, Nst, t0, ra)
         B.gemm(0, ra, Nst, Nst, hw, ra, pC, Nst, t1, ra)
@@ -266,7 +324,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
         qa, hqa = cols(Q, a, 0), cols(HQ, a, 0)
         pT = B.gram(sT, qa, hqa, nq, nq, ra)
         pG = B.gram(sG, qa, qa, nq, nq, ra)
-        pU = _ritz_coefs(B, sM, pT, pG, nq, Nst, ka)
+        pU = _ritz_coefs(B, sM, pT, pG, nq, Nst, ka, infos, w_dev.p)
         B.gemm(0, ra, Nst, nq, qa, ra, pU, nq, t0, ra)
         B.gemm(0, ra, Nst, nq, hqa, ra, pU, nq, t1, ra)
         x, hx, p, hp = cols(Q, a, 0), cols(HQ, a, 0), cols(Q, a, 2 * Nst), cols(HQ, a, 2 * Nst)
@@ -278,7 +336,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
             B.copy(ra, Nst, t0, x)
             B.copy(ra, Nst, t1, hx)
             pC = B.gram(sC, t2, t2, Nst, Nst, ra)
-            B.chol_inv(pC, Nst, ka)
+            B.chol_inv(pC, Nst, ka, infos)
             B.gemm(0, ra, Nst, Nst, t2, ra, pC, Nst, p, ra)
             B.gemm(0, ra, Nst, Nst, t3, ra, pC, Nst, hp, ra)
         else:
@@ -286,6 +344,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
             B.copy(ra, Nst, hw, hp)
             B.copy(ra, Nst, t0, x)
             B.copy(ra, Nst, t1, hx)
+    infos.check()
     # final Rayleigh-Ritz of every k-point
     x, hx = cols(Q, allk, 0), cols(HQ, allk, 0)
     pS = B.gram(sX, x, hx, Nst, Nst, rws(allk))
@@ -293,7 +352,7 @@ def diag_LOBPCG_batch(Ham, psiks, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=
     ev = B.eigh(pS, Nst)
     B.gemm(0, rws(allk), Nst, Nst, x, rws(allk), pS, Nst, [psiks[k].p.value for k in allk], rws(allk))
     c.call("pw_synchronize")
-    for s in (Q, HQ, Tmp, sX, sC, sT, sG, sM):
+    for s in (Q, HQ, Tmp, sX, sC, sT, sG, sM, infos, w_dev):
         s.free()
     if lock_ws is not None:
         lock_ws.free()

@@ -198,6 +198,46 @@ def test_chol_inv_batch_vs_numpy(n, nbatch):
     c.close()
 
 
+@pytest.mark.parametrize("n,nbatch", [(8, 182), (24, 182)])
+def test_async_dense_batch_matches_sync(n, nbatch):
+    """pw_chol_inv_batch_async_dev and pw_heevd_batch_async_dev give the synchronous calls'
+    results bit for bit and leave their per-matrix status in device memory: zero for
+    positive definite input, nonzero for the matrix made indefinite."""
+    from pwdft_hip._lib import Context, DeviceMatrix, ptr
+    c = Context(0)
+    rng = np.random.default_rng(n + 3 * nbatch)
+    Z = rng.standard_normal((nbatch, n, 2 * n)) + 1j * rng.standard_normal((nbatch, n, 2 * n))
+    A = Z @ np.conj(np.swapaxes(Z, 1, 2)) / (2 * n) + 0.1 * np.eye(n)
+    col = lambda M: np.ascontiguousarray(np.transpose(M, (0, 2, 1))).reshape(-1, 1)
+    d1, d2 = DeviceMatrix.from_host(c, col(A)), DeviceMatrix.from_host(c, col(A))
+    info = DeviceMatrix(c, nbatch, 1)                    # 4 * nbatch ints fit in 16 * nbatch bytes
+    wdev = DeviceMatrix(c, (n * nbatch + 1) // 2, 1)
+    get_info = lambda m: (lambda h: (c.call("pw_memcpy_d2h", ptr(h), info.p, h.nbytes), h)[1])(np.empty(m, np.int32))
+    print(f"case async dense n={n} nbatch={nbatch}", flush=True)
+    c.call("pw_chol_inv_batch_dev", n, nbatch, d1.p, n, n * n)
+    c.call("pw_chol_inv_batch_async_dev", n, nbatch, d2.p, n, n * n, info.p)
+    assert np.array_equal(d1.download(), d2.download())
+    assert not get_info(2 * nbatch).any()
+    w = np.empty((nbatch, n))
+    d1.upload(col(A).ravel())
+    d2.upload(col(A).ravel())
+    c.call("pw_heevd_batch_dev", n, nbatch, d1.p, n, n * n, ptr(w))
+    c.call("pw_heevd_batch_async_dev", n, nbatch, d2.p, n, n * n, wdev.p, info.p)
+    w2 = np.empty((nbatch, n))
+    c.call("pw_memcpy_d2h", ptr(w2), wdev.p, w2.nbytes)
+    assert np.array_equal(w, w2) and np.array_equal(d1.download(), d2.download())
+    assert not get_info(nbatch).any()
+    bad = np.array(A)
+    bad[5][0, 0] = -1.0
+    d2.upload(col(bad).ravel())
+    c.call("pw_chol_inv_batch_async_dev", n, nbatch, d2.p, n, n * n, info.p)
+    st = get_info(2 * nbatch)
+    assert st[5] != 0 and np.count_nonzero(st[:nbatch]) == 1
+    for m in (d1, d2, info, wdev):
+        m.free()
+    c.close()
+
+
 def test_lobpcg_batch_vs_per_k(al):
     """diag_LOBPCG! over all 182 k-points in lockstep gives the per-k solver's eigenvalues."""
     import pwdft_hip as ph
