@@ -912,6 +912,18 @@ static bool lanes_ok(const FftDim& d) { return d.L >= 1 && d.L <= 64 && d.gpw >=
 static size_t plane_lds(const pw_ctx* c, size_t esz, int nxact) {
   return ((size_t)odd_stride(c->n1) * c->n2 + c->n1 + c->n2) * esz + (size_t)nxact * sizeof(int);
 }
+// Threads of a padded-plane workgroup (the kernels index by blockDim.x): wavefronts for one
+// pass over the plane's lines, max(nxact y lines, n2 x rows) / lines per wavefront, when that
+// is at most half of PlaneNT.  Small planes (20^2: one wavefront) then fit many workgroups
+// per CU instead of one or two mostly idle ones; large planes keep PlaneNT.
+template <typename T>
+static int plane_threads(const pw_ctx* c, int nxact) {
+  const FftDim& dx = c->dim[0];
+  const FftDim& dy = c->dim[1];
+  const int waves = std::max(1, std::max((nxact + dy.gpw - 1) / dy.gpw, (c->n2 + dx.gpw - 1) / dx.gpw));
+  return 2 * 64 * waves <= PlaneNT<T>::value ? 64 * waves : PlaneNT<T>::value;
+}
+
 static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
 // Compressed plane (k_plane_vloc_c / k_plane_rhoe_c): rows in flight per wavefront that fit
@@ -1007,7 +1019,7 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
         hipLaunchKernelGGL(k_plane_vloc_c<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
                            k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
       else
-        hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, V,
+        hipLaunchKernelGGL(k_plane_vloc<T>, g2, dim3(plane_threads<T>(c, k.nxact)), plane_bytes, c->stream, W, V,
                            k.d_col_ij, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     }
     if (join && b0 == 0) {
@@ -1088,7 +1100,7 @@ static int vloc_batch_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb,
     tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
   hipLaunchKernelGGL(k_col_ifft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb,
                      nk, nb, W, C, c->dim[2], tw[2]);
-  hipLaunchKernelGGL(k_plane_vloc_batch<T>, dim3(c->n3, nk * nb), dim3(PlaneNT<T>::value), plane_bytes, c->stream,
+  hipLaunchKernelGGL(k_plane_vloc_batch<T>, dim3(c->n3, nk * nb), dim3(plane_threads<T>(c, maxx)), plane_bytes, c->stream,
                      dkb, nb, W, V, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
   hipLaunchKernelGGL(k_col_fft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb, nk,
                      nb, W, C, c->dim[2], tw[2], kin, acc);
@@ -1168,10 +1180,13 @@ static int rhoe_run(pw_ctx* c, int ik, int nb, const cplx<double>* psi, long ldp
                          d_w + b0, nbb, bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_cy, k.ncol,
                          k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0], tw[1], c->n3, rl);
     else {
-      const int nr = (np + PlaneNT<T>::value - 1) / PlaneNT<T>::value;  // <= RHO_REGS here
+      // the plane's points over the threads, at most RHO_REGS each (np <= RHO_REGS * PlaneNT here)
+      int nt = plane_threads<T>(c, k.nxact);
+      while ((np + nt - 1) / nt > RHO_REGS) nt += 64;
+      const int nr = (np + nt - 1) / nt;
       auto kern = nr <= 4 ? k_plane_rhoe<T, 4> : nr <= 6 ? k_plane_rhoe<T, 6> : nr <= 8 ? k_plane_rhoe<T, 8>
                 : nr <= 12 ? k_plane_rhoe<T, 12> : k_plane_rhoe<T, RHO_REGS>;
-      hipLaunchKernelGGL(kern, dim3(c->n3, ngu), dim3(PlaneNT<T>::value), plane_bytes, c->stream, W, d_w + b0, nbb,
+      hipLaunchKernelGGL(kern, dim3(c->n3, ngu), dim3(nt), plane_bytes, c->stream, W, d_w + b0, nbb,
                          bpg, reinterpret_cast<double*>(c->d_rpart), c->N, k.d_col_ij, k.ncol, k.d_xact, k.nxact,
                          c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
     }
