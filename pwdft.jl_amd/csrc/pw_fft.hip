@@ -1250,7 +1250,7 @@ int g2r_apply(pw_ctx* c, KData& k, int nf, const cplx<double>* fG, long ldf, dou
                          o, c->N, scale, acc, k.d_col_cy, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0],
                          tw[1], c->n3, rl);
     else
-      hipLaunchKernelGGL(k_plane_g2r, dim3(c->n3, nbb), dim3(PlaneNT<double>::value), plane_bytes, c->stream, W, o,
+      hipLaunchKernelGGL(k_plane_g2r, dim3(c->n3, nbb), dim3(plane_threads<double>(c, k.nxact)), plane_bytes, c->stream, W, o,
                          c->N, scale, acc, k.d_col_ij, k.ncol, k.d_xact, k.nxact, c->dim[0], c->dim[1], tw[0],
                          tw[1], c->n3);
   }
