@@ -87,24 +87,31 @@ int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out) {
     }
   if (c->capturing)
     return set_error(c, PW_ERR_STATE, "a new descriptor array during graph capture; run the sequence once first");
-  hipError_t e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "descriptor sync");
+  constexpr int kSlots = 256;
   int slot;
-  if (n < 64 || c->graphs_live > 0) {  // never overwrite an array a captured graph may read
+  if (n < kSlots || c->graphs_live > 0) {  // never overwrite an array a captured graph may read
+    // a new slot: no queued kernel reads it, so nothing to wait for
     c->desc_host.emplace_back();
     c->desc_dev.push_back(nullptr);
     c->desc_cap.push_back(0);
     slot = n;
   } else {
+    // reusing a slot: kernels still queued may read its old contents (and its host copy
+    // may still be the source of a queued upload)
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "descriptor sync");
     slot = c->desc_next;
-    c->desc_next = (c->desc_next + 1) % 64;
+    c->desc_next = (c->desc_next + 1) % kSlots;
   }
   int rc = ensure(c, &c->desc_dev[slot], &c->desc_cap[slot], bytes);
   if (rc) return rc;
-  e = hipMemcpy(c->desc_dev[slot], data, bytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_check(c, e, "descriptor upload");
   const char* p = reinterpret_cast<const char*>(data);
   c->desc_host[slot].assign(p, p + bytes);
+  // ordered on the stream before the kernels that read it; the host copy it reads from is
+  // not changed before the slot is reused, which waits for the stream first
+  hipError_t e = hipMemcpyAsync(c->desc_dev[slot], c->desc_host[slot].data(), bytes, hipMemcpyHostToDevice,
+                                c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "descriptor upload");
   *out = c->desc_dev[slot];
   return PW_OK;
 }

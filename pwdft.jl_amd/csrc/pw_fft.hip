@@ -924,6 +924,13 @@ static int plane_threads(const pw_ctx* c, int nxact) {
   return 2 * 64 * waves <= PlaneNT<T>::value ? 64 * waves : PlaneNT<T>::value;
 }
 
+// Threads of a column workgroup holding C columns: wavefronts for one pass over its C lines
+// when that is at most half of COL_NT, else COL_NT (the column kernels index by blockDim.x).
+static int col_threads(const pw_ctx* c, int C) {
+  const int waves = std::max(1, (C + c->dim[2].gpw - 1) / c->dim[2].gpw);
+  return 2 * 64 * waves <= COL_NT ? 64 * waves : COL_NT;
+}
+
 static size_t col_lds(const pw_ctx* c, int C, size_t esz) { return ((size_t)C * odd_stride(c->n3) + c->n3) * esz; }
 
 // Compressed plane (k_plane_vloc_c / k_plane_rhoe_c): rows in flight per wavefront that fit
@@ -1054,9 +1061,17 @@ template <typename T>
 static int vloc_batch_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, const cplx<double>* const* psi,
                           const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, int kin, int acc) {
   const size_t esz = sizeof(cplx<T>);
-  const int C = cols_per_block(c, esz);
+  int C = cols_per_block(c, esz);
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
+  // no column chunk wider than the widest k-point's column set (small grids: one chunk per
+  // band), so the LDS tile and the workgroup are sized to the columns actually there
+  if (c->tune_col_chunk <= 0) {
+    int maxcol = 1;
+    for (int kk = 0; kk < nk; ++kk) maxcol = std::max(maxcol, c->kd[iks[kk]].ncol);
+    C = std::min(C, maxcol);
+  }
+  const int cnt = col_threads(c, C);
   const int nb8 = (nb + 7) / 8 * 8;
   std::vector<KBatch> kb(nk);
   long woff = 0;
@@ -1098,11 +1113,11 @@ static int vloc_batch_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb,
   const cplx<T>* tw[3];
   for (int d = 0; d < 3; ++d)
     tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
-  hipLaunchKernelGGL(k_col_ifft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb,
+  hipLaunchKernelGGL(k_col_ifft_batch<T>, dim3((unsigned)blk), dim3(cnt), col_lds(c, C, esz), c->stream, dkb,
                      nk, nb, W, C, c->dim[2], tw[2]);
   hipLaunchKernelGGL(k_plane_vloc_batch<T>, dim3(c->n3, nk * nb), dim3(plane_threads<T>(c, maxx)), plane_bytes, c->stream,
                      dkb, nb, W, V, c->dim[0], c->dim[1], tw[0], tw[1], c->n3);
-  hipLaunchKernelGGL(k_col_fft_batch<T>, dim3((unsigned)blk), dim3(COL_NT), col_lds(c, C, esz), c->stream, dkb, nk,
+  hipLaunchKernelGGL(k_col_fft_batch<T>, dim3((unsigned)blk), dim3(cnt), col_lds(c, C, esz), c->stream, dkb, nk,
                      nb, W, C, c->dim[2], tw[2], kin, acc);
   return hip_check(c, hipGetLastError(), "batched vloc kernels");
 }

@@ -295,9 +295,10 @@ int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const
 // Batched complex GEMM C_i = alpha op(A_i) B_i + beta C_i (pw_batch.hip).
 int zgemm_batch(pw_ctx* c, int transa, const std::vector<GBatch>& g, cplx<double> alpha, cplx<double> beta);
 // Device copy of a descriptor array (bytes at `data`).  Arrays are cached by content in up
-// to 64 slots, so a call sequence that repeats (the steps of an eigensolver iteration)
-// uploads nothing after its first pass.  A miss synchronises the stream before it
-// overwrites a slot, since kernels still queued may read it.
+// to 256 slots, so a call sequence that repeats (the steps of an eigensolver iteration)
+// uploads nothing after its first pass.  A miss fills a new slot with an upload ordered on
+// the stream (no wait); only once all slots are taken does a miss synchronise the stream
+// before it overwrites one, since kernels still queued may read it.
 int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out);
 void free_desc(pw_ctx* c);
 void free_dense(pw_ctx* c);
