@@ -42,7 +42,7 @@ class Atoms:
         self.Nspecies = len(species)
 
 
-ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0}
+ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0, "Ce": 58.0, "Pt": 78.0}
 
 
 # ----------------------------------------------------------------------------- FFT grid sizes

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 PSP_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pseudopotentials")
-ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0, "Ce": 58.0}
+ZATOMS = {"H": 1.0, "Al": 13.0, "Si": 14.0, "Ce": 58.0, "Pt": 78.0}
 
 
 def gen_lattice_fcc(a):

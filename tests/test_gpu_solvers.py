@@ -193,3 +193,19 @@ def test_lobpcg_lock_branch_vs_oracle(dense):
     assert np.abs(oev - exact).max() < 1e-6, (oev, exact)
     Xd = X.download()
     assert np.abs(Xd.conj().T @ Xd - np.eye(8)).max() < 1e-10
+
+
+@pytest.mark.parametrize("nst", [8, 160])
+def test_ortho_sqrt_vs_oracle(nst):
+    """ortho_sqrt! (ortho_sqrt.jl:6-9) on the device against the oracle's Loewdin
+    orthonormalisation of the same block (unique: the polar factor)."""
+    import pwdft_hip as ph
+    from oracle.solvers import ortho_sqrt
+    H = _si(meshk=(1, 1, 1))
+    n = H.pw.Ngw[0] if nst == 8 else 4 * nst
+    psi = rand_psi(np.random.default_rng(nst), n, nst)
+    X = ph.DeviceMatrix.from_host(H.ctx, psi)
+    print(f"case ortho_sqrt nst={nst}", flush=True)
+    ph.ortho_sqrt_(H, X)
+    ref = ortho_sqrt(psi)
+    assert np.abs(X.download() - ref).max() < 1e-12 * np.abs(ref).max()

@@ -432,3 +432,73 @@ def test_lobpcg_lock_branch_converges_to_dense():
     assert np.abs(ev - exact).max() < 1e-6, (ev, exact)
     assert np.allclose(X.conj().T @ X, np.eye(nst), atol=1e-10)
     assert np.abs(H @ X - X * ev[None, :]).max() < 1e-3
+
+
+def test_eval_proj_G_is_hankel_transform_of_gth_projector():
+    """eval_proj_G (PsPot_GTH.jl:338-391), every (l, i) the reference implements, equals the
+    Fourier-Bessel transform (4 pi / sqrt(Omega)) int r^2 j_l(G r) p_i^l(r) dr of the
+    real-space GTH projector p_i^l(r) = sqrt(2) r^(l + 2(i-1)) exp(-r^2 / (2 r_l^2)) /
+    (r_l^(l + (4i-1)/2) sqrt(Gamma(l + (4i-1)/2))) of Hartwigsen, Goedecker and Hutter,
+    PRB 58, 3641 (1998), Eq. 3, by adaptive quadrature.  A wrong prefactor or polynomial in
+    any channel (the d and f channels included) fails it."""
+    from scipy.integrate import quad
+    from scipy.special import gamma, spherical_jn
+
+    class _Psp:
+        rc = [0.52, 0.66, 0.45, 0.61]
+
+    omega = 101.9
+    for l, imax in ((0, 3), (1, 3), (2, 2), (3, 1)):
+        rl = _Psp.rc[l]
+        for i in range(1, imax + 1):
+            e = l + (4 * i - 1) / 2
+
+            def p(r):
+                return math.sqrt(2) * r ** (l + 2 * (i - 1)) * math.exp(-r * r / (2 * rl * rl)) / (
+                    rl ** e * math.sqrt(gamma(e)))
+
+            for G in (0.0, 0.7, 2.3, 5.1):
+                ref = quad(lambda r: r * r * spherical_jn(l, G * r) * p(r), 0, 20 * rl, limit=200, epsabs=1e-14,
+                           epsrel=1e-13)[0] * 4 * math.pi / math.sqrt(omega)
+                got = float(S.eval_proj_G(_Psp, l, i, np.array([G]), omega)[0])
+                assert abs(got - ref) < 1e-13, (l, i, G, got, ref)
+
+
+def test_ortho_sqrt_is_polar_factor():
+    """ortho_sqrt (ortho_sqrt.jl:1-9): psi (psi'psi)^(-1/2) is the unitary polar factor U V^H
+    of psi = U S V^H, the unique orthonormal matrix nearest to psi (Loewdin)."""
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal((300, 12)) + 1j * rng.standard_normal((300, 12))
+    U, _, Vh = np.linalg.svd(psi, full_matrices=False)
+    got = ortho_sqrt(psi)
+    assert np.abs(got - U @ Vh).max() < 1e-13
+    assert np.abs(got.conj().T @ got - np.eye(12)).max() < 1e-13
+
+
+@pytest.mark.slow
+def test_scf_pt_fcc_smearing_matches_reference_log():
+    """test/LOG_runtest_v.1.1.1:2630-2663 (set-up test/validations_old/Pt_fcc.jl:1-22): Pt fcc
+    a = 3.9231 angstrom, GTH-PADE-q10 (s, p and d channels, two projectors each, off-diagonal
+    h), 30 Ha, 3x3x3, Davidson, simple mixing beta 0.5, Fermi-Dirac smearing kT = 1e-3.
+    The log stopped at |dE| 2.6e-7: free energy -26.2945052148, -TS -0.0005158875,
+    E_F -0.0148842322; ABINIT's free energy is -26.2945054360.  This pins the d-channel
+    projectors, find_E_fermi and calc_entropy.  The log predates the folding of the G = 0
+    local-potential constant into V_Ps_loc (its separate PspCore term, 6.3258423493 =
+    N_e * mean(V_Ps_loc)), so its eigenvalues and Fermi level sit lower by mean(V_Ps_loc)."""
+    from oracle.energy import calc_Focc
+    a = 3.9231 * 1.8897261254578281
+    H = oracle_ham(["Pt"], [[0, 0, 0]], S.gen_lattice_fcc(a), "Pt-q10.lda.gth", 30.0, meshk=(3, 3, 3),
+                   extra_states=4)
+    assert H.pspotNL.NbetaNL == 18 and H.Nstates == 9
+    e, evals, psiks, rho, hist = KS_solve_SCF(H, betamix=0.5, etot_conv_thr=1e-8, ethr_evals_last=1e-10,
+                                              use_smearing=True, kT=1e-3, update_psi="davidson")
+    _, Ef = calc_Focc(H.Nelectrons, H.pw.wk, 1e-3, evals)
+    assert abs(e["Total"] - (-26.2945052148)) < 1e-6, e
+    assert abs(e["Total"] - (-26.2945054360)) < 1e-6, e
+    assert abs(e["mTS"] - (-0.0005158875)) < 1e-8, e
+    shift = float(np.mean(H.V_Ps_loc))
+    assert abs(shift * H.Nelectrons - 6.3258423493) < 1e-8, shift
+    assert abs(Ef - shift - (-0.0148842322)) < 1e-5, (Ef, shift)
+    for k, v in (("Kinetic", 19.2060140731), ("Ps_nloc", -14.7734350216), ("Hartree", 1.8623453874),
+                 ("XC", -4.4748717329), ("Ps_loc", -3.5178517592 + 6.3258423493), ("NN", -30.9220326233)):
+        assert abs(e[k] - v) < 1e-6, (k, e[k], v)
