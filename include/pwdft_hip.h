@@ -76,10 +76,19 @@ enum {
                                    stream beside the FFT passes; 0: one stream, in sequence */
   PW_TUNE_COL_CHUNK = 7,        /* z-columns per workgroup of the column FFT kernels (0: the
                                    default, as many as 48 KiB of LDS hold) */
-  PW_TUNE_GEMM_SPLIT_BF16 = 8   /* the mixed path's complex GEMMs on BF16 MFMA with each operand
+  PW_TUNE_GEMM_SPLIT_BF16 = 8,  /* the mixed path's complex GEMMs on BF16 MFMA with each operand
                                    split in three BF16 parts (FP32-level accuracy): 1 always,
                                    0 never, -1 (default) for products of >= 2^33 complex
                                    multiply-adds */
+  PW_TUNE_EIG_PATH = 9,         /* Hermitian eigensolves: 0 (default) the device tridiagonal
+                                   reduction (one persistent launch, matrix in LDS) wherever
+                                   the order fits; 1 rocSOLVER's ZHEEVD throughout */
+  PW_TUNE_LOBPCG_LOCK = 10,     /* lock threshold of pw_diag_lobpcg_dev: 0 (default) the
+                                   reference's 10 eps; k > 0: 10 * 10^-k (tests of the branch) */
+  PW_TUNE_FUSED_BATCH = 11      /* pw_op_H_batch_dev: 1 (default) one fused launch, a workgroup
+                                   per (k-point, band) holding the band's whole grid in LDS,
+                                   wherever the grid fits (20^3 in FP64); 0: the three-stage
+                                   column / plane / column launches and batched GEMMs */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 
@@ -290,6 +299,18 @@ int pw_zgemm_batch_dev(pw_ctx* ctx, int transa, int nbatch, const int* m, const 
                        const int64_t* ldb, const double* beta, void* const* C, const int64_t* ldc);
 
 /* ---- small dense FP64 problems of the eigensolvers, on the device ------------------- */
+
+/* diag_LOBPCG!(Ham, psi) for k-point ik and spin ispin (diag_LOBPCG.jl:51-225, called per
+ * k from diag_LOBPCG!(Ham, psiks), :1-28, which KS_solve_SCF! calls at KS_solve_SCF.jl:
+ * 189-191): X (device, Ngw x nst, leading dimension ldx, orthonormal columns) is overwritten
+ * with the Ritz vectors, evals (host, nst) with the Ritz values.  tol, niter_max and
+ * nstates_conv (0 = nst) as the reference's tol, NiterMax and Nstates_conv: iteration stops
+ * once nstates_conv Ritz values moved by less than tol.  flags: PW_MIXED for the mixed
+ * precision products.  stats (host, 3 ints, may be NULL): iterations, op_H columns applied,
+ * lock-branch iterations.  Requires 3 nst <= Ngw.  PW_ERR_NOT_POSDEF where the reference's
+ * cholesky() throws PosDefException.  Synchronises once per iteration. */
+int pw_diag_lobpcg_dev(pw_ctx* ctx, int ik, int ispin, int nst, void* X, int64_t ldx, double tol, int niter_max,
+                       int nstates_conv, unsigned flags, double* evals, int* stats);
 
 /* The host LAPACK calls of diag_LOBPCG! / diag_davidson! / ortho_sqrt (cholesky(Hermitian(C)),
  * inv(U), eigen(Hermitian(T), Hermitian(G)), eigen(Hermitian(S))) on device matrices

@@ -260,6 +260,13 @@ int pw_op_H_batch_dev(pw_ctx* c, int nk, const int* iks, int ispin, int nb, void
   bool have = (flags & PW_ACCUMULATE) != 0;
   const cplx<double>* const* P = reinterpret_cast<const cplx<double>* const*>(psi);
   cplx<double>* const* Hp = reinterpret_cast<cplx<double>* const*>(Hpsi);
+  if ((terms & PW_TERM_VLOC) && c->fused_batch && op_H_fused_fits(c, (flags & PW_MIXED) != 0)) {
+    // the whole grid of a band fits one workgroup's LDS: every term of every band of every
+    // k-point in one launch, nothing of the FFTs in HBM
+    if (ispin < 0 || ispin >= c->nspin || !c->d_v64[ispin])
+      return set_error(c, PW_ERR_STATE, "op_V_loc: potential of spin " + std::to_string(ispin) + " not set");
+    return op_H_fused_batch(c, nk, iks, ispin, nb, P, ldpsi, Hp, ldh, flags);
+  }
   if (terms & PW_TERM_VLOC) {
     if (ispin < 0 || ispin >= c->nspin || !c->d_v64[ispin])
       return set_error(c, PW_ERR_STATE, "op_V_loc: potential of spin " + std::to_string(ispin) + " not set");

@@ -80,6 +80,7 @@ void free_k(KData& k) {
   hipFree(k.d_xact);
   hipFree(k.d_gk2);
   hipFree(k.d_gk);
+  hipFree(k.d_gpos);
   hipFree(k.d_beta64);
   hipFree(k.d_betaD64);
   hipFree(k.d_beta32);
@@ -116,6 +117,13 @@ __global__ void k_copy_cols(const cplx<double>* __restrict__ s, long lds, cplx<d
 
 static int launch_blocks(long tot) { return (int)std::max<long>(1, std::min<long>((tot + 255) / 256, 4096)); }
 
+int copy_cols(pw_ctx* c, int nrows, int ncols, const cplx<double>* src, long lds, cplx<double>* dst, long ldd) {
+  if (nrows <= 0 || ncols <= 0) return PW_OK;
+  hipLaunchKernelGGL(k_copy_cols, dim3(launch_blocks((long)nrows * ncols)), dim3(256), 0, c->stream, src, lds, dst, ldd,
+                     nrows, ncols);
+  return hip_check(c, hipGetLastError(), "copy_cols");
+}
+
 static int check_k(pw_ctx* c, int ik) {
   if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
   if (ik < 0 || ik >= (int)c->kd.size() || c->kd[ik].ngw == 0)
@@ -123,8 +131,8 @@ static int check_k(pw_ctx* c, int ik) {
   return PW_OK;
 }
 
-static int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi,
-                       cplx<double>* hpsi, long ldh, unsigned flags) {
+int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,
+                long ldh, unsigned flags) {
   int rc = check_k(c, ik);
   if (rc) return rc;
   KData& k = c->kd[ik];
@@ -363,6 +371,17 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
     case PW_TUNE_OVERLAP:
       c->overlap = value != 0;
       return PW_OK;
+    case PW_TUNE_FUSED_BATCH:
+      c->fused_batch = value != 0;
+      return PW_OK;
+    case PW_TUNE_LOBPCG_LOCK:
+      if (value < 0 || value > 300) return set_error(c, PW_ERR_ARG, "pw_tune_set: lock exponent must be 0..300");
+      c->lobpcg_lock_exp = value;
+      return PW_OK;
+    case PW_TUNE_EIG_PATH:
+      if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver path must be 0..1");
+      c->eig_path = value;
+      return PW_OK;
     default:
       return set_error(c, PW_ERR_ARG, "pw_tune_set: unknown key");
   }
@@ -480,6 +499,14 @@ int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, cons
     const double x = gk[3 * g], y = gk[3 * g + 1], z = gk[3 * g + 2];
     gk2[g] = x * x + y * y + z * z;
   }
+  // slot of each psi row in a whole padded grid (x stride 1, y stride odd_stride(n1), z stride
+  // odd_stride(n1) n2): the layout of the fused single-workgroup op_H
+  std::vector<int> gpos(ngw);
+  for (int g = 0; g < ngw; ++g) {
+    const int64_t ip = idx[g] - base;
+    const int i = (int)(ip % n1), j = (int)((ip / n1) % n2), z = (int)(ip / ((int64_t)n1 * n2));
+    gpos[g] = i + ps * (j + n2 * z);
+  }
   k.ngw = ngw;
   k.ncol = ncol;
   k.nxact = (int)xs.size();
@@ -493,6 +520,7 @@ int build_kdata(pw_ctx* c, KData& k, int ngw, const int64_t* idx, int base, cons
   if ((rc = upload(c, &k.d_xact, xs.data(), xs.size()))) return rc;
   if ((rc = upload(c, &k.d_gk2, gk2.data(), ngw))) return rc;
   if ((rc = upload(c, &k.d_gk, gk, 3 * (size_t)ngw))) return rc;
+  if ((rc = upload(c, &k.d_gpos, gpos.data(), ngw))) return rc;
   return PW_OK;
 }
 
@@ -792,11 +820,8 @@ int pw_graph_free(pw_ctx* c, int id) {
 
 int pw_copy_cols_dev(pw_ctx* c, int nrows, int ncols, const void* src, int64_t lds, void* dst, int64_t ldd) {
   CTX_CHECK(c);
-  if (nrows <= 0 || ncols <= 0) return PW_OK;
-  hipLaunchKernelGGL(k_copy_cols, dim3(launch_blocks((long)nrows * ncols)), dim3(256), 0, c->stream,
-                     reinterpret_cast<const cplx<double>*>(src), lds, reinterpret_cast<cplx<double>*>(dst), ldd,
-                     nrows, ncols);
-  return hip_check(c, hipGetLastError(), "copy_cols");
+  return copy_cols(c, nrows, ncols, reinterpret_cast<const cplx<double>*>(src), lds, reinterpret_cast<cplx<double>*>(dst),
+                   ldd);
 }
 
 int pw_profile_enable(pw_ctx* c, int on) {

@@ -40,6 +40,15 @@ void free_dense(pw_ctx* c) {
   hipFree(c->d_dwork);
   c->d_dwork = nullptr;
   c->dwork_bytes = 0;
+  hipFree(c->d_eig);
+  c->d_eig = nullptr;
+  c->eig_bytes = 0;
+  hipFree(c->d_lob);
+  c->d_lob = nullptr;
+  c->lob_bytes = 0;
+  hipFree(c->d_lob2);
+  c->d_lob2 = nullptr;
+  c->lob2_bytes = 0;
 }
 
 static int st_check(pw_ctx* c, rocblas_status s, const char* what) {
@@ -97,9 +106,9 @@ using namespace pw;
 #define CTX_CHECK(c) \
   if (!(c)) return PW_ERR_ARG
 
-extern "C" {
+namespace pw {
 
-static int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t stride, int mode) {
+int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t stride, int mode) {
   const long tot = (long)n * n;
   const int blocks = (int)std::min<long>((tot + 255) / 256, nbatch > 1 ? 64 : 2048);
   cplx<double>* a = reinterpret_cast<cplx<double>*>(A);
@@ -110,6 +119,61 @@ static int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, i
                        mode);
   return hip_check(c, hipGetLastError(), "k_herm_fix");
 }
+
+}  // namespace pw
+
+namespace pw {
+
+int solver_handle(pw_ctx* c, void** rb) {
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  *rb = c->rb;
+  return PW_OK;
+}
+
+// U = cholesky(Hermitian(A)).U and its inverse in place (strict lower triangle zeroed);
+// potrf's info into d_info[0], trtri's into d_info[1].  No synchronisation.
+int chol_inv_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, int* d_info) {
+  if (c->eig_path == 0 && n <= small_dense_max_n()) return chol_inv_small_batch(c, n, 1, A, lda, 0, d_info);
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  rocblas_double_complex* a = reinterpret_cast<rocblas_double_complex*>(A);
+  rc = st_check(c, rocsolver_zpotrf(h, rocblas_fill_upper, n, a, (int)lda, d_info), "zpotrf");
+  if (rc) return rc;
+  rc = st_check(c, rocsolver_ztrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, a, (int)lda, d_info + 1),
+                "ztrtri");
+  if (rc) return rc;
+  return herm_fix_launch(c, n, 1, A, lda, 0, 1);
+}
+
+__global__ void k_heev1(cplx<double>* A, double* w, int vec, int* info) {
+  w[0] = A[0].x;
+  if (vec) A[0] = mk(1.0, 0.0);
+  *info = 0;
+}
+
+// Eigenvalues of Hermitian(A) (upper triangle) into d_w, eigenvectors of the nvec lowest
+// into A's first nvec columns (nvec = 0: values only); E holds n doubles of workspace and
+// d_info one status word.  No synchronisation.
+int heev_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, double* d_w, int nvec, double* E, int* d_info) {
+  if (n == 1) {
+    hipLaunchKernelGGL(k_heev1, dim3(1), dim3(1), 0, c->stream, A, d_w, nvec > 0 ? 1 : 0, d_info);
+    return hip_check(c, hipGetLastError(), "k_heev1");
+  }
+  if (c->eig_path == 0 && n <= small_dense_max_n()) return heev_small_batch(c, n, 1, A, lda, 0, d_w, nvec, d_info);
+  int rc = ensure_solver(c);
+  if (rc) return rc;
+  if (c->eig_path == 0 && n <= heevd_dist_max_n()) return heevd_dist(c, c->rb, n, A, lda, d_w, nvec, d_info);
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  return st_check(c, rocsolver_zheevd(h, nvec > 0 ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
+                                      n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_w, E, d_info),
+                  "zheevd");
+}
+
+}  // namespace pw
+
+extern "C" {
 
 int pw_herm_fix_dev(pw_ctx* c, int n, void* A, int64_t lda, int mode) {
   CTX_CHECK(c);
@@ -133,6 +197,8 @@ int pw_herm_fix_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, in
 // potrf + trtri + lower-triangle clear of a strided stack; one potrf info and one trtri
 // info per matrix into dinfo[0..nbatch) and dinfo[nbatch..2 nbatch).  No synchronisation.
 static int chol_inv_batch_enqueue(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int* dinfo) {
+  if (c->eig_path == 0 && n <= small_dense_max_n())
+    return chol_inv_small_batch(c, n, nbatch, reinterpret_cast<cplx<double>*>(A), (long)lda, (long)strideA, dinfo);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   rocblas_double_complex* a = reinterpret_cast<rocblas_double_complex*>(A);
   int rc = st_check(c, rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, a, (int)lda, (rocblas_stride)strideA,
@@ -239,6 +305,20 @@ static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool ve
   double* D = reinterpret_cast<double*>(c->d_dwork);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
+  if (c->eig_path == 0 && n <= heevd_dist_max_n()) {
+    rc = n <= small_dense_max_n()
+             ? heev_small_batch(c, n, 1, reinterpret_cast<cplx<double>*>(A), (long)lda, 0, D, vectors ? n : 0, dinfo)
+             : heevd_dist(c, c->rb, n, reinterpret_cast<cplx<double>*>(A), (long)lda, D, vectors ? n : 0, dinfo);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "eigenvalues download");
+    int info = 0;
+    rc = read_info(c, dinfo, &info);
+    if (rc) return rc;
+    if (info == -1) return set_error(c, PW_ERR_HIP, "eigen: the device tridiagonal reduction timed out");
+    if (info != 0) return set_error(c, PW_ERR_ARG, "eigen: no convergence (" + std::to_string(info) + ")");
+    return PW_OK;
+  }
   rc = st_check(c, rocsolver_zheevd(h, vectors ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
                                     n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda, D, D + n, dinfo),
                 "zheevd");
@@ -269,10 +349,13 @@ int pw_heevd_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64
   double* E = D + (size_t)n * nbatch;
   int* dinfo = reinterpret_cast<int*>(D + nd);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
-  rc = st_check(c, rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n,
-                                                    reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
-                                                    (rocblas_stride)strideA, D, n, E, n, dinfo, nbatch),
-                "zheevd_strided_batched");
+  if (c->eig_path == 0 && n <= small_dense_max_n())
+    rc = heev_small_batch(c, n, nbatch, reinterpret_cast<cplx<double>*>(A), (long)lda, (long)strideA, D, n, dinfo);
+  else
+    rc = st_check(c, rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n,
+                                                      reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
+                                                      (rocblas_stride)strideA, D, n, E, n, dinfo, nbatch),
+                  "zheevd_strided_batched");
   if (rc) return rc;
   std::vector<int> info(nbatch);
   hipError_t e = hipMemcpyAsync(w, D, (size_t)n * nbatch * sizeof(double), hipMemcpyDeviceToHost, c->stream);
@@ -313,6 +396,9 @@ int pw_heevd_batch_async_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda,
   if (rc) return rc;
   double* E = reinterpret_cast<double*>(c->d_dwork);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
+  if (c->eig_path == 0 && n <= small_dense_max_n())
+    return heev_small_batch(c, n, nbatch, reinterpret_cast<cplx<double>*>(A), (long)lda, (long)strideA, d_w, n,
+                            d_info);
   return st_check(c, rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n,
                                                        reinterpret_cast<rocblas_double_complex*>(A), (int)lda,
                                                        (rocblas_stride)strideA, d_w, n, E, n, d_info, nbatch),

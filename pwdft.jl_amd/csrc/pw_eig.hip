@@ -1,0 +1,389 @@
+// Hermitian eigensolver of the Rayleigh-Ritz step (the ZHEEV of diag_LOBPCG.jl:176-203 and
+// diag_davidson.jl; ortho_sqrt.jl:6-9), built for the sizes the block eigensolvers meet
+// (n = Nst .. 3 Nst, a few hundred to about a thousand):
+//
+//  1. k_hetrd_dist: Householder reduction to real symmetric tridiagonal form,
+//     T = Q^H A Q, Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^H (LAPACK ZHETD2,
+//     lower variant).  One persistent cooperative launch: the matrix lives in the LDS of P
+//     workgroups (column j in workgroup j mod P), so the n - 1 Householder steps never
+//     re-read it from HBM and never pay a kernel launch.  Each step has two cross-workgroup
+//     synchronisation points: the owner of column i publishes v_i, and after every
+//     workgroup has formed p = tau A22 v for its own columns (A22 is Hermitian, so
+//     (A22 v)_j = A22[:, j]^H v needs only local columns) the p segments are gathered.  The
+//     owner of column i + 1 updates that column first and publishes v_{i+1} before it
+//     updates the rest of its columns (look-ahead).
+//  2. The eigenpairs of T through rocSOLVER's divide and conquer (ZSTEDC).
+//  3. k_apply_q: eigenvectors of A = Q (eigenvectors of T), the reflectors applied to the
+//     wanted columns only, two columns per wavefront, staged in LDS.
+//
+// Only the upper triangle of A is read, as Julia's Hermitian(A) does.  Cross-workgroup
+// waits are bounded: a wait that exceeds its spin limit sets a status word, every other
+// wait then ends at once, and the host reports the failure.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <string>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+constexpr int HT_NT = 256;                 // threads per workgroup (4 wavefronts)
+constexpr long HT_SPIN = 1L << 22;         // spin limit of one cross-workgroup wait
+constexpr size_t HT_LDS = 160 * 1024 - 256;  // LDS bytes per workgroup, less room for the static shared state
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Thread 0: wait until *ctr >= target (acquire); false once the status word is set or the
+// spin limit is reached (then it sets the status word itself).
+__device__ inline bool wait_geq(unsigned* ctr, unsigned target, int* status) {
+  long spins = 0;
+  while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (++spins > HT_SPIN) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// Block-wide sum (every thread gets the result); red holds HT_NT / 64 doubles.
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < HT_NT / 64; ++q) s += red[q];
+  return s;
+}
+
+struct HetrdShared {
+  int ok;
+  double tr, ti;  // tau_i
+};
+
+// Householder vector of column i (owner only; column in LDS, rows i+1..n-1 below the
+// diagonal): ZLARFG on (alpha = A[i+1, i], x = A[i+2:n, i]).  Writes v_i (v[i+1] = 1) to
+// V[:, i], tau_i, e_i = beta, d_i = Re A[i, i], then publishes sync[0] = i + 1.
+__device__ void hetrd_reflector(const cplx<double>* col, int i, int n, cplx<double>* V, long ldv,
+                                cplx<double>* tau, double* dd, double* ee, unsigned* sync, double* red) {
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int r = i + 2 + tid; r < n; r += HT_NT) s += col[r].x * col[r].x + col[r].y * col[r].y;
+  const double xn2 = block_sum(s, red);
+  const cplx<double> alpha = col[i + 1];
+  double beta, tr, ti;
+  cplx<double> scale;
+  if (xn2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tr = ti = 0.0;
+    scale = mk(0.0, 0.0);
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    tr = (beta - alpha.x) / beta;
+    ti = -alpha.y / beta;
+    // scale = 1 / (alpha - beta)
+    const double ar = alpha.x - beta, ai = alpha.y;
+    const double den = ar * ar + ai * ai;
+    scale = mk(ar / den, -ai / den);
+  }
+  cplx<double>* v = V + (long)i * ldv;
+  for (int r = i + 1 + tid; r < n; r += HT_NT) v[r] = (r == i + 1) ? mk(1.0, 0.0) : cmul(col[r], scale);
+  if (tid == 0) {
+    tau[i] = mk(tr, ti);
+    ee[i] = beta;
+    dd[i] = col[i].x;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&sync[0], (unsigned)(i + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A22 -= v w^H + w v^H on local column lc (global column j), rows i+1..n-1.
+__device__ inline void hetrd_update_col(cplx<double>* col, int j, int i, int n, const cplx<double>* vb,
+                                        const cplx<double>* wb) {
+  const cplx<double> vj = vb[j], wj = wb[j];
+  for (int r = i + 1 + threadIdx.x; r < n; r += HT_NT) {
+    const cplx<double> a = col[r];
+    const cplx<double> t = cmulc(vb[r], wj) + cmulc(wb[r], vj);
+    col[r] = a - t;
+  }
+}
+
+// sync[0]: index + 1 of the last published reflector; sync[1]: arrivals of the p gather.
+// pg: 2 x n published p (by step parity), sp: 2 x P x 2 partial sums s = sum conj(p_j) v_j.
+__global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __restrict__ A, long lda, int n,
+                                                      int ncl_max, cplx<double>* __restrict__ V, long ldv,
+                                                      cplx<double>* __restrict__ tau, double* __restrict__ dd,
+                                                      double* __restrict__ ee, cplx<double>* pg, double* sp,
+                                                      unsigned* sync, int* status) {
+  extern __shared__ char smem_raw[];
+  const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  cplx<double>* Acol = reinterpret_cast<cplx<double>*>(smem_raw);
+  cplx<double>* vb = Acol + (size_t)ncl_max * n;
+  cplx<double>* wb = vb + n;
+  double* red = reinterpret_cast<double*>(wb + n);
+  __shared__ HetrdShared sh;
+  const int ncl = w < n ? (n - 1 - w) / P + 1 : 0;
+
+  // load the owned columns of Hermitian(A): rows <= j from the upper triangle, rows > j as
+  // the conjugate of row j
+  for (long t = tid; t < (long)ncl * n; t += HT_NT) {
+    const int lc = (int)(t / n), r = (int)(t % n);
+    const int j = w + lc * P;
+    cplx<double> a;
+    if (r < j) {
+      a = A[(long)j * lda + r];
+    } else if (r == j) {
+      a = mk(A[(long)j * lda + r].x, 0.0);
+    } else {
+      const cplx<double> u = A[(long)r * lda + j];
+      a = mk(u.x, -u.y);
+    }
+    Acol[(long)lc * n + r] = a;
+  }
+  __syncthreads();
+  if (w == 0) hetrd_reflector(Acol, 0, n, V, ldv, tau, dd, ee, sync, red);
+
+  for (int i = 0; i <= n - 2; ++i) {
+    const int par = i & 1;
+    // B: v_i
+    if (tid == 0) sh.ok = wait_geq(&sync[0], (unsigned)(i + 1), status) ? 1 : 0;
+    __syncthreads();
+    if (!sh.ok) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int r = i + 1 + tid; r < n; r += HT_NT) vb[r] = V[(long)i * ldv + r];
+    if (tid == 0) {
+      const cplx<double> t = tau[i];
+      sh.tr = t.x;
+      sh.ti = t.y;
+    }
+    __syncthreads();
+    const cplx<double> taui = mk(sh.tr, sh.ti);
+    // C: p_j = tau * A22[:, j]^H v for the owned columns j > i
+    const int lc0 = (i + 1 <= w) ? 0 : (i + 1 - w + P - 1) / P;  // first local column with j > i
+    double spr = 0.0, spi = 0.0;
+    for (int lc = lc0 + wave; lc < ncl; lc += HT_NT / 64) {
+      const int j = w + lc * P;
+      const cplx<double>* col = Acol + (long)lc * n;
+      double ar = 0.0, ai = 0.0;
+      for (int r = i + 1 + lane; r < n; r += 64) {
+        const cplx<double> a = col[r], v = vb[r];
+        // conj(a) * v
+        ar += a.x * v.x + a.y * v.y;
+        ai += a.x * v.y - a.y * v.x;
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      const cplx<double> p = cmul(taui, mk(ar, ai));
+      if (lane == 0) {
+        pg[(long)par * n + j] = p;
+        const cplx<double> v = vb[j];
+        // conj(p) * v
+        spr += p.x * v.x + p.y * v.y;
+        spi += p.x * v.y - p.y * v.x;
+      }
+    }
+    {
+      double* red2 = red + HT_NT / 64;
+      __syncthreads();
+      if (lane == 0) {
+        red[wave] = spr;
+        red2[wave] = spi;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < HT_NT / 64; ++q) {
+          a += red[q];
+          b += red2[q];
+        }
+        sp[((long)par * P + w) * 2 + 0] = a;
+        sp[((long)par * P + w) * 2 + 1] = b;
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // D: gather p, form w, update
+    if (tid == 0) sh.ok = wait_geq(&sync[1], (unsigned)P * (unsigned)(i + 1), status) ? 1 : 0;
+    __syncthreads();
+    if (!sh.ok) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double sr = 0.0, si = 0.0;
+    for (int q = tid; q < P; q += HT_NT) {
+      sr += sp[((long)par * P + q) * 2 + 0];
+      si += sp[((long)par * P + q) * 2 + 1];
+    }
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    // alpha2 = -1/2 tau s
+    const cplx<double> alpha2 = cmul(mk(-0.5 * taui.x, -0.5 * taui.y), mk(sr, si));
+    for (int r = i + 1 + tid; r < n; r += HT_NT) wb[r] = pg[(long)par * n + r] + cmul(alpha2, vb[r]);
+    __syncthreads();
+    int done = -1;
+    if (i + 1 <= n - 2 && (i + 1) % P == w) {
+      const int lc = (i + 1) / P;
+      hetrd_update_col(Acol + (long)lc * n, i + 1, i, n, vb, wb);
+      __syncthreads();
+      hetrd_reflector(Acol + (long)lc * n, i + 1, n, V, ldv, tau, dd, ee, sync, red);
+      done = lc;
+    }
+    for (int lc = lc0; lc < ncl; ++lc) {
+      if (lc == done) continue;
+      hetrd_update_col(Acol + (long)lc * n, w + lc * P, i, n, vb, wb);
+    }
+    __syncthreads();
+  }
+  if ((n - 1) % P == w && tid == 0) dd[n - 1] = Acol[(long)((n - 1) / P) * n + (n - 1)].x;
+}
+
+// C[:, c] <- Q C[:, c] = H(0) (H(1) (... H(n-2) C[:, c])) for the columns c < ncols; each
+// wavefront owns two columns, staged in LDS.
+__global__ void __launch_bounds__(HT_NT) k_apply_q(const cplx<double>* __restrict__ V, long ldv,
+                                                   const cplx<double>* __restrict__ tau, int n,
+                                                   cplx<double>* __restrict__ C, long ldc, int ncols) {
+  extern __shared__ char smem_raw[];
+  constexpr int CPW = 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  cplx<double>* S = reinterpret_cast<cplx<double>*>(smem_raw) + (long)wave * CPW * n;
+  const int c0 = (blockIdx.x * (HT_NT / 64) + wave) * CPW;
+  if (c0 >= ncols) return;
+  const int nc = min(CPW, ncols - c0);
+  for (int q = 0; q < nc; ++q)
+    for (int r = lane; r < n; r += 64) S[q * n + r] = C[(long)(c0 + q) * ldc + r];
+  for (int i = n - 2; i >= 0; --i) {
+    const cplx<double> t = tau[i];
+    if (t.x == 0.0 && t.y == 0.0) continue;
+    const cplx<double>* v = V + (long)i * ldv;
+    double sr[CPW] = {0.0, 0.0}, si[CPW] = {0.0, 0.0};
+    for (int r = i + 1 + lane; r < n; r += 64) {
+      const cplx<double> vr = v[r];
+      for (int q = 0; q < CPW; ++q) {
+        if (q < nc) {
+          const cplx<double> cr = S[q * n + r];
+          // conj(v) * c
+          sr[q] += vr.x * cr.x + vr.y * cr.y;
+          si[q] += vr.x * cr.y - vr.y * cr.x;
+        }
+      }
+    }
+    cplx<double> f[CPW];
+    for (int q = 0; q < CPW; ++q) f[q] = cmul(t, mk(wave_sum(sr[q]), wave_sum(si[q])));
+    for (int r = i + 1 + lane; r < n; r += 64) {
+      const cplx<double> vr = v[r];
+      for (int q = 0; q < CPW; ++q)
+        if (q < nc) S[q * n + r] = S[q * n + r] - cmul(vr, f[q]);
+    }
+  }
+  for (int q = 0; q < nc; ++q)
+    for (int r = lane; r < n; r += 64) C[(long)(c0 + q) * ldc + r] = S[q * n + r];
+}
+
+__global__ void k_merge_status(int* info, const int* status) {
+  if (*status != 0) *info = -1;
+}
+
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// Workgroups of the distributed reduction for order n: the fewest whose column share,
+// the two staging vectors and the reduction scratch fit in LDS; 0 when n is out of range.
+int hetrd_workgroups(int n, int* ncl_max, size_t* lds_bytes) {
+  if (n < 2) return 0;
+  const size_t extra = 2 * (size_t)n * sizeof(cplx<double>) + 2 * (HT_NT / 64) * sizeof(double);
+  if (extra >= HT_LDS) return 0;
+  const long per = (long)((HT_LDS - extra) / ((size_t)n * sizeof(cplx<double>)));
+  if (per < 1) return 0;
+  int P = (int)((n + per - 1) / per);
+  P = std::max(P, std::min(n, 8));   // at least a few workgroups share the matrix-vector work
+  if (P > 256) return 0;
+  const int ncl = (n + P - 1) / P;
+  *ncl_max = ncl;
+  *lds_bytes = (size_t)ncl * n * sizeof(cplx<double>) + extra;
+  return P;
+}
+
+// Largest order the device reduction takes (beyond it, rocSOLVER's ZHEEVD runs).
+int heevd_dist_max_n() {
+  int lo = 2, hi = 4096, best = 0;
+  while (lo <= hi) {
+    const int mid = (lo + hi) / 2;
+    int ncl;
+    size_t b;
+    if (hetrd_workgroups(mid, &ncl, &b) > 0) {
+      best = mid;
+      lo = mid + 1;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return best;
+}
+
+// Eigenvalues (ascending, into d_w on the device) and, when nvec > 0, the eigenvectors of
+// the nvec lowest into A's first nvec columns.  rocSOLVER's handle runs on the context stream.
+// Enqueues everything on the context stream; d_info gets 0, or the ZSTEDC failure code, or
+// -1 when a cross-workgroup wait of the reduction timed out.
+int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info) {
+  int ncl = 0;
+  size_t lds = 0;
+  const int P = hetrd_workgroups(n, &ncl, &lds);
+  if (P <= 0) return set_error(c, PW_ERR_ARG, "heevd_dist: order " + std::to_string(n) + " out of range");
+  // workspace: V (n x n), tau (n), pg (2n) complex; e (n), sp (4P) doubles; sync (2) + status
+  const size_t vbytes = (size_t)n * n * sizeof(cplx<double>);
+  const size_t need = vbytes + 3 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
+  int rc = ensure(c, &c->d_eig, &c->eig_bytes, need);
+  if (rc) return rc;
+  char* base = reinterpret_cast<char*>(c->d_eig);
+  cplx<double>* V = reinterpret_cast<cplx<double>*>(base);
+  cplx<double>* tau = V + (size_t)n * n;
+  cplx<double>* pg = tau + n;
+  double* E = reinterpret_cast<double*>(pg + 2 * (size_t)n);
+  double* sp = E + n;
+  unsigned* sync = reinterpret_cast<unsigned*>(sp + 4 * (size_t)P);
+  int* status = reinterpret_cast<int*>(sync + 2);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, sync, 3, 0u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_check(c, e, "k_fill_u32");
+  const cplx<double>* Ac = A;
+  long ldal = lda, ldv = n;
+  int nn = n, nclm = ncl;
+  double* dd = d_w;
+  void* args[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
+                  (void*)&dd, (void*)&E, (void*)&pg, (void*)&sp, (void*)&sync, (void*)&status};
+  e = hipLaunchCooperativeKernel((const void*)k_hetrd_dist, dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "k_hetrd_dist (cooperative launch)");
+  // the timeout status of the reduction is folded into d_info below
+  rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);
+  rocblas_status st = rocsolver_zstedc(h, nvec > 0 ? rocblas_evect_tridiagonal : rocblas_evect_none, n, d_w, E,
+                                       reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_info);
+  if (st != rocblas_status_success)
+    return set_error(c, PW_ERR_HIP, "zstedc: rocSOLVER status " + std::to_string((int)st));
+  if (nvec > 0) {
+    const int waves = (nvec + 1) / 2;
+    const int blocks = (waves + HT_NT / 64 - 1) / (HT_NT / 64);
+    const size_t sh = (size_t)(HT_NT / 64) * 2 * n * sizeof(cplx<double>);
+    hipLaunchKernelGGL(k_apply_q, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, tau, n, A, lda, nvec);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(c, e, "k_apply_q");
+  }
+  // d_info = -1 when the reduction timed out (one tiny kernel keeps it on the stream)
+  hipLaunchKernelGGL(k_merge_status, dim3(1), dim3(1), 0, c->stream, d_info, status);
+  return hip_check(c, hipGetLastError(), "k_merge_status");
+}
+
+}  // namespace pw

@@ -1122,6 +1122,190 @@ static int vloc_batch_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb,
   return hip_check(c, hipGetLastError(), "batched vloc kernels");
 }
 
+// ---------------------------------------------------------------- fused single-workgroup op_H
+// For grids whose whole padded box fits one workgroup's LDS (20^3 in FP64: 134 KB), one
+// workgroup applies op_H to one band of one k-point with no intermediate in HBM: scatter psi
+// into the box, inverse z-FFT of the sphere's columns, inverse y-FFT of the lines at its
+// active x, inverse x-FFT of every row, times V/N (read from L2) inside the forward x-FFT,
+// forward y- and z-FFTs on the same pruned lines, then gather + 1/2 |G+k|^2 psi + the
+// nonlocal term beta D (beta^H psi) (op_V_loc.jl:62-90, op_K.jl:44-75, op_V_Ps_nloc.jl:39-78).
+// All bands of all k-points (uneven Ngw) go in one launch, one workgroup per (k-point, band).
+struct FusedK {
+  const cplx<double>* psi;
+  cplx<double>* hpsi;
+  long ldpsi, ldh;
+  const int* gpos;       // psi row -> slot in the padded box
+  const double* gk2;
+  const int* col_ij;     // active columns (slot of z = 0)
+  const int* xact;
+  const void* beta;      // ngw x nbeta in the kernel's precision (or null)
+  const void* betaD;
+  int ngw, ncol, nxact, nbeta;
+};
+
+constexpr int FUSED_NT = 256;
+
+template <typename T>
+__device__ inline void fused_rows_vloc(cplx<T>* lds, int nrows, int ps, const FftDim& dx, const cplx<T>* tx,
+                                       const T* __restrict__ V) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int n1 = dx.n;
+  const int g = lane / dx.L, q = lane - g * dx.L;
+  for (int t0 = wave * dx.gpw; t0 < nrows; t0 += nwave * dx.gpw) {
+    const int row = t0 + g;
+    const bool active = g < dx.gpw && row < nrows;
+    const int base = active ? row * ps : 0;
+    wave_line_fft<T, +1, FFT_ME>(lds, base, 1, dx, tx, q, active, nullptr);
+    wave_line_fft<T, -1, FFT_ME>(lds, base, 1, dx, tx, q, active, active ? V + (long)row * n1 : nullptr);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(FUSED_NT)
+k_op_H_fused(const FusedK* __restrict__ kd, int nb, const T* __restrict__ V, FftDim dx, FftDim dy, FftDim dz,
+             const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz,
+             int do_v, int do_k, int do_nl, int acc) {
+  extern __shared__ char smem_raw[];
+  const int kk = blockIdx.x / nb, b = blockIdx.x - kk * nb;
+  const FusedK F = kd[kk];
+  const int n1 = dx.n, n2 = dy.n, n3 = dz.n, ps = odd_stride(n1);
+  const int box = ps * n2 * n3;
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tx = lds + box;
+  cplx<T>* ty = tx + n1;
+  cplx<T>* tz = ty + n2;
+  cplx<T>* Bn = tz + n3;                                   // beta^H psi (nbeta)
+  int* lb = reinterpret_cast<int*>(Bn + (F.nbeta > 0 ? F.nbeta : 1));   // line bases
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const cplx<double>* p = F.psi + (long)b * F.ldpsi;
+  cplx<double>* h = F.hpsi + (long)b * F.ldh;
+  const int nyl = F.nxact * n3;
+  if (do_v) {
+    for (int i = tid; i < box; i += nthr) lds[i] = mk((T)0, (T)0);
+    stage(tx, twx, n1);
+    stage(ty, twy, n2);
+    stage(tz, twz, n3);
+    stage(lb, F.col_ij, F.ncol);
+    __syncthreads();
+    for (int g = tid; g < F.ngw; g += nthr) lds[F.gpos[g]] = cvt<T>(p[g]);
+    __syncthreads();
+    wave_lines<T, +1, FFT_ME>(lds, F.ncol, lb, 0, ps * n2, dz, tz);       // z-lines of the sphere's columns
+    __syncthreads();
+    for (int t = tid; t < nyl; t += nthr) {
+      const int a = t % F.nxact, z = t / F.nxact;
+      lb[t] = F.xact[a] + ps * n2 * z;
+    }
+    __syncthreads();
+    wave_lines<T, +1, FFT_ME>(lds, nyl, lb, 0, ps, dy, ty);             // y-lines at the active x
+    __syncthreads();
+    fused_rows_vloc<T>(lds, n2 * n3, ps, dx, tx, V);                    // x-rows: inverse, * V/N, forward
+    __syncthreads();
+    wave_lines<T, -1, FFT_ME>(lds, nyl, lb, 0, ps, dy, ty);
+    __syncthreads();
+    stage(lb, F.col_ij, F.ncol);
+    __syncthreads();
+    wave_lines<T, -1, FFT_ME>(lds, F.ncol, lb, 0, ps * n2, dz, tz);
+  }
+  if (do_nl && F.nbeta > 0) {
+    // B_j = sum_g conj(beta_gj) psi_g, one wavefront per projector
+    const cplx<T>* be = reinterpret_cast<const cplx<T>*>(F.beta);
+    const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
+    for (int j = wave; j < F.nbeta; j += nwave) {
+      T sr = 0, si = 0;
+      for (int g = lane; g < F.ngw; g += 64) {
+        const cplx<T> bg = be[(long)j * F.ngw + g];
+        const cplx<T> x = cvt<T>(p[g]);
+        sr += bg.x * x.x + bg.y * x.y;
+        si += bg.x * x.y - bg.y * x.x;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o, 64);
+        si += __shfl_xor(si, o, 64);
+      }
+      if (lane == 0) Bn[j] = mk(sr, si);
+    }
+  }
+  __syncthreads();
+  const cplx<T>* bd = reinterpret_cast<const cplx<T>*>(F.betaD);
+  for (int g = tid; g < F.ngw; g += nthr) {
+    const cplx<double> x = p[g];
+    cplx<double> r = acc ? h[g] : mk(0.0, 0.0);
+    if (do_v) r = r + cvt<double>(lds[F.gpos[g]]);
+    if (do_k) r = r + scal(0.5 * F.gk2[g], x);
+    if (do_nl && F.nbeta > 0) {
+      cplx<T> s = mk((T)0, (T)0);
+      for (int j = 0; j < F.nbeta; ++j) s = s + cmul(bd[(long)j * F.ngw + g], Bn[j]);
+      r = r + cvt<double>(s);
+    }
+    h[g] = r;
+  }
+}
+
+static size_t fused_lds(const pw_ctx* c, size_t esz, int maxcol, int maxyl, int nbeta) {
+  const size_t box = (size_t)odd_stride(c->n1) * c->n2 * c->n3;
+  return (box + c->n1 + c->n2 + c->n3 + std::max(nbeta, 1)) * esz + (size_t)std::max(maxcol, maxyl) * sizeof(int);
+}
+
+bool op_H_fused_fits(const pw_ctx* c, bool mixed) {
+  const size_t esz = mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  // worst case line tables: every column and every x active
+  const size_t need = fused_lds(c, esz, c->n1 * c->n2, c->n1 * c->n3, c->nbeta);
+  return need <= 160 * 1024 && lanes_ok(c->dim[0]) && lanes_ok(c->dim[1]) && lanes_ok(c->dim[2]);
+}
+
+template <typename T>
+static int fused_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, const cplx<double>* const* psi,
+                     const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
+  const size_t esz = sizeof(cplx<T>);
+  const unsigned terms = flags & PW_TERM_ALL;
+  std::vector<FusedK> kb(nk);
+  int maxcol = 1, maxyl = 1, nbeta = 0;
+  for (int kk = 0; kk < nk; ++kk) {
+    const KData& k = c->kd[iks[kk]];
+    FusedK d{};
+    d.psi = psi[kk];
+    d.hpsi = hpsi[kk];
+    d.ldpsi = ldpsi[kk];
+    d.ldh = ldh[kk];
+    d.gpos = k.d_gpos;
+    d.gk2 = k.d_gk2;
+    d.col_ij = k.d_col_ij;
+    d.xact = k.d_xact;
+    d.nbeta = (terms & PW_TERM_NLOC) ? k.nbeta : 0;
+    d.beta = sizeof(T) == 8 ? k.d_beta64 : k.d_beta32;
+    d.betaD = sizeof(T) == 8 ? k.d_betaD64 : k.d_betaD32;
+    d.ngw = k.ngw;
+    d.ncol = k.ncol;
+    d.nxact = k.nxact;
+    if (d.nbeta > 0 && (!d.beta || !d.betaD)) return set_error(c, PW_ERR_STATE, "op_H fused: projectors not set");
+    kb[kk] = d;
+    maxcol = std::max(maxcol, k.ncol);
+    maxyl = std::max(maxyl, k.nxact * c->n3);
+    nbeta = std::max(nbeta, d.nbeta);
+  }
+  const size_t lds = fused_lds(c, esz, maxcol, maxyl, nbeta);
+  if (lds > 160 * 1024) return set_error(c, PW_ERR_ARG, "op_H fused: the grid does not fit in LDS");
+  if ((long)nk * nb > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "op_H fused: too many bands for one launch");
+  const void* dptr = nullptr;
+  int rc = desc_get(c, kb.data(), kb.size() * sizeof(FusedK), &dptr);
+  if (rc) return rc;
+  const cplx<T>* tw[3];
+  for (int d = 0; d < 3; ++d)
+    tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
+  hipLaunchKernelGGL(k_op_H_fused<T>, dim3((unsigned)(nk * nb)), dim3(FUSED_NT), lds, c->stream,
+                     reinterpret_cast<const FusedK*>(dptr), nb, V, c->dim[0], c->dim[1], c->dim[2], tw[0], tw[1],
+                     tw[2], (terms & PW_TERM_VLOC) ? 1 : 0, (terms & PW_TERM_K) ? 1 : 0, (terms & PW_TERM_NLOC) ? 1 : 0,
+                     (flags & PW_ACCUMULATE) ? 1 : 0);
+  return hip_check(c, hipGetLastError(), "k_op_H_fused");
+}
+
+int op_H_fused_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
+                     const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
+  if (!op_H_fused_fits(c, (flags & PW_MIXED) != 0)) return set_error(c, PW_ERR_ARG, "op_H fused: grid too large");
+  if (flags & PW_MIXED) return fused_run<float>(c, nk, iks, c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
+  return fused_run<double>(c, nk, iks, c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
+}
+
 int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
                      const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
   const int kin = (flags & PW_TERM_K) ? 1 : 0;

@@ -94,6 +94,7 @@ struct KData {
   int* d_xact = nullptr;      // [nxact] distinct x of the columns
   double* d_gk2 = nullptr;    // [ngw]   |G+k|^2
   double* d_gk = nullptr;     // [3*ngw] G+k (x,y,z per entry, column-major 3 x ngw)
+  int* d_gpos = nullptr;      // [ngw]   psi row -> x + ps (y + n2 z), ps = odd_stride(n1): slot in a whole padded grid
   void* d_beta64 = nullptr;   // [ngw * nbeta] complex double, column-major
   void* d_betaD64 = nullptr;  // beta * D
   void* d_beta32 = nullptr;   // complex float copies for the mixed path
@@ -228,6 +229,12 @@ struct pw_ctx {
   void* rb = nullptr;
   void* d_dinfo = nullptr; size_t dinfo_bytes = 0;
   void* d_dwork = nullptr; size_t dwork_bytes = 0;
+  void* d_eig = nullptr;   size_t eig_bytes = 0;     // reflectors and exchange buffers of heevd_dist
+  int eig_path = 0;                                   // 0: device reduction where it fits, 1: rocSOLVER only
+  void* d_lob = nullptr;   size_t lob_bytes = 0;     // LOBPCG blocks and small matrices (pw_lobpcg.hip)
+  void* d_lob2 = nullptr;  size_t lob2_bytes = 0;    // LOBPCG lock-branch blocks
+  int lobpcg_lock_exp = 0;                            // 0: lock at 10 eps; k: at 10 * 10^-k
+  bool fused_batch = true;                            // batched op_H: one fused launch where the box fits LDS
   // density side of an SCF iteration (pw_scf.hip)
   double* d_vps = nullptr;                            // V_Ps_loc(r) [N]
   void* d_sgrid = nullptr; size_t sgrid_bytes = 0;    // grid fields of the potential update
@@ -259,6 +266,11 @@ int set_error(pw_ctx* c, int code, const std::string& msg);
 int hip_check(pw_ctx* c, hipError_t e, const char* what);
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need);
 
+// Hpsi of the bands psi at k-point ik, spin ispin, into hpsi (terms and precision by flags)
+// on the context stream (pw_capi.hip).
+int op_H_device(pw_ctx* c, int ik, int ispin, int nb, const cplx<double>* psi, long ldpsi, cplx<double>* hpsi,
+                long ldh, unsigned flags);
+
 // FFT drivers (pw_fft.hip)
 // join: when set, the stream waits for this event before the first gather pass (the event
 // marks work on another stream that writes hpsi first).
@@ -289,6 +301,12 @@ int ensure_dcsr(pw_ctx* c);
 // out[0..n) = 0 with a kernel (safe inside stream capture, unlike a memset node).
 int zero_fill(pw_ctx* c, double* out, long n);
 
+// op_H (kinetic + local + nonlocal terms selected by flags) of every band of the k-points
+// iks in one launch, one workgroup per (k-point, band) holding the band's whole grid in
+// LDS (pw_fft.hip).  Returns PW_ERR_ARG without launching when the grid does not fit.
+int op_H_fused_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
+                     const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags);
+bool op_H_fused_fits(const pw_ctx* c, bool mixed);
 // Batched op_V_loc (+ kinetic) over the k-points iks (pw_fft.hip); psi / hpsi per k-point.
 int vloc_apply_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
                      const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags);
@@ -302,6 +320,29 @@ int zgemm_batch(pw_ctx* c, int transa, const std::vector<GBatch>& g, cplx<double
 int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out);
 void free_desc(pw_ctx* c);
 void free_dense(pw_ctx* c);
+// Hermitian eigensolver with the device tridiagonal reduction (pw_eig.hip): eigenvalues into
+// d_w (device), eigenvectors of the nvec lowest into A's first nvec columns; enqueued on the
+// context stream, status (0, rocSOLVER's code, or -1 for a timed-out reduction) into d_info.
+int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info);
+int heevd_dist_max_n();
+// dst[:, 0:ncols] = src[:, 0:ncols] (nrows rows) on the context stream.
+int copy_cols(pw_ctx* c, int nrows, int ncols, const cplx<double>* src, long lds, cplx<double>* dst, long ldd);
+// Hermitian fix-ups of a strided stack (pw_herm_fix_dev modes 0, 1, 2).
+int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t stride, int mode);
+// Cholesky factor and its inverse in place (infos into d_info[0..1]), no synchronisation.
+int chol_inv_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, int* d_info);
+// Eigenvalues into d_w and the nvec lowest eigenvectors into A (E: n doubles), no
+// synchronisation; the device reduction where the order fits, rocSOLVER's ZHEEVD beyond.
+int heev_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, double* d_w, int nvec, double* E, int* d_info);
+int solver_handle(pw_ctx* c, void** rb);
+// Batched small dense problems (order <= small_dense_max_n(), one workgroup per matrix,
+// pw_small.hip): Hermitian eigenpairs (eigenvalues into d_w + b n, the nvec lowest vectors
+// into matrix b, status into d_info[b]) and Cholesky factor inverses (statuses into
+// d_info[b] and d_info[nbatch + b]).  No synchronisation.
+int heev_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, double* d_w, int nvec,
+                     int* d_info);
+int chol_inv_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, int* d_info);
+int small_dense_max_n();
 
 // GEMM drivers (pw_gemm.hip)
 // C (M x N) = alpha * op(A) * B + beta * C ; op(A) = A (transa = 0, A is M x K) or

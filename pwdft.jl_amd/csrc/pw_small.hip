@@ -1,0 +1,253 @@
+// Small dense problems of the block eigensolvers (order n <= 64), one workgroup per matrix
+// with the matrix in LDS, for many matrices in one launch: the Rayleigh-Ritz and Gram
+// matrices of every k-point of a batched LOBPCG step (2 Nst, 3 Nst = 16, 24 at 8 bands), and
+// those of small single k-point problems.  They replace the LAPACK calls of
+// diag_LOBPCG.jl:106-118 (cholesky, inv) and :176-190 (eigen) for these orders, launch
+// nothing else and need no library workspace, so a step that uses them can be captured
+// into a graph whole.
+//
+//  - k_heev_jacobi: Hermitian eigenpairs by cyclic Jacobi rotations in tournament order
+//    (n/2 disjoint rotations per round, all applied at once: columns, then rows), a pair
+//    rotated while |A_pq| > max(1e-15 sqrt|A_pp A_qq|, 1e-17 ||A||_F), until a sweep rotates
+//    no pair; eigenvalues sorted ascending with their vectors.  Complex 2x2 step: the pair (p, q) with A_pq = |g| e^{i phi} is made
+//    real by diag(1, e^{-i phi}) and diagonalised by the classical rotation
+//    t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = (A_qq - A_pp) / (2 |g|).
+//  - k_chol_inv_small: U = chol(Hermitian(A)).U by right-looking elimination, then U^-1 by
+//    column-parallel back substitution; the strict lower triangle zeroed.  Status: 0, or
+//    the index (1-based) of the first non-positive pivot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+constexpr int SM_NT = 256;
+constexpr int SM_NMAX = 64;
+constexpr int SM_SWEEPS = 30;
+
+__device__ inline double sm_block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < SM_NT / 64; ++q) s += red[q];
+  return s;
+}
+
+// Matrix b of the stack at A + b * stride (leading dimension lda): eigenvalues (ascending)
+// into w + b * n, the nvec lowest eigenvectors into the matrix's first nvec columns.
+__global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict__ A, long lda, long stride, int n,
+                                                       double* __restrict__ w, int nvec, int* __restrict__ info) {
+  extern __shared__ char smem_raw[];
+  cplx<double>* a = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n
+  cplx<double>* v = a + n * n;                                     // n x n
+  __shared__ double rc[SM_NMAX / 2], rs[SM_NMAX / 2];
+  __shared__ cplx<double> rph[SM_NMAX / 2];       // e^{-i phi} of each pair
+  __shared__ int rp[SM_NMAX / 2], rq[SM_NMAX / 2];
+  __shared__ double red[SM_NT / 64];
+  __shared__ double dsort[SM_NMAX];
+  __shared__ int rank[SM_NMAX];
+  cplx<double>* M = A + (long)blockIdx.x * stride;
+  const int tid = threadIdx.x;
+  const int nn = n * n;
+  // Hermitian(M): upper triangle, real diagonal
+  for (int t = tid; t < nn; t += SM_NT) {
+    const int i = t % n, j = t / n;
+    cplx<double> x;
+    if (i < j) x = M[(long)j * lda + i];
+    else if (i == j) x = mk(M[(long)j * lda + i].x, 0.0);
+    else { const cplx<double> u = M[(long)i * lda + j]; x = mk(u.x, -u.y); }
+    a[i + n * j] = x;
+    v[i + n * j] = mk(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  double fro = 0.0;
+  for (int t = tid; t < nn; t += SM_NT) fro += a[t].x * a[t].x + a[t].y * a[t].y;
+  fro = sm_block_sum(fro, red);
+  const double gfloor = 1e-17 * sqrt(fro);
+  const int m = (n + 1) / 2 * 2;     // players of the tournament (one dummy when n is odd)
+  const int npair = m / 2;
+  int sweep = 0;
+  bool converged = false;
+  for (; sweep < SM_SWEEPS; ++sweep) {
+    double nrot = 0.0;
+    for (int r = 0; r < m - 1; ++r) {
+      if (tid < npair) {
+        int p, q;
+        if (tid == 0) {
+          p = m - 1;
+          q = r;
+        } else {
+          p = (r + tid) % (m - 1);
+          q = (r - tid + (m - 1)) % (m - 1);
+        }
+        if (p > q) { const int t = p; p = q; q = t; }
+        double c = 1.0, s = 0.0;
+        cplx<double> ph = mk(1.0, 0.0);
+        if (q < n) {
+          const cplx<double> g = a[p + n * q];
+          const double ag = sqrt(g.x * g.x + g.y * g.y);
+          const double app = a[p + n * p].x, aqq = a[q + n * q].x;
+          if (ag > 1e-15 * sqrt(fabs(app * aqq)) && ag > gfloor) {
+            nrot += 1.0;
+            const double tau = (aqq - app) / (2.0 * ag);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+            ph = mk(g.x / ag, -g.y / ag);   // e^{-i phi}
+          }
+        } else {
+          p = q = -1;
+        }
+        rp[tid] = p;
+        rq[tid] = q;
+        rc[tid] = c;
+        rs[tid] = s;
+        rph[tid] = ph;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J with J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] on (p, q)
+      for (int t = tid; t < npair * n; t += SM_NT) {
+        const int k = t / n, i = t % n;
+        const int p = rp[k], q = rq[k];
+        if (p < 0 || rs[k] == 0.0) continue;
+        const double c = rc[k], s = rs[k];
+        const cplx<double> ph = rph[k];
+        {
+          const cplx<double> xp = a[i + n * p], xq = cmul(a[i + n * q], ph);
+          a[i + n * p] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
+          a[i + n * q] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+        }
+        {
+          const cplx<double> xp = v[i + n * p], xq = cmul(v[i + n * q], ph);
+          v[i + n * p] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
+          v[i + n * q] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+        }
+      }
+      __syncthreads();
+      // rows: A <- J^H A, rows p, q: [c A_p - s e^{i phi} A_q, s A_p + c e^{i phi} A_q]
+      for (int t = tid; t < npair * n; t += SM_NT) {
+        const int k = t / n, j = t % n;
+        const int p = rp[k], q = rq[k];
+        if (p < 0 || rs[k] == 0.0) continue;
+        const double c = rc[k], s = rs[k];
+        const cplx<double> phc = mk(rph[k].x, -rph[k].y);   // e^{i phi}
+        const cplx<double> xp = a[p + n * j], xq = cmul(a[q + n * j], phc);
+        a[p + n * j] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
+        a[q + n * j] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+      }
+      __syncthreads();
+    }
+    if (sm_block_sum(nrot, red) == 0.0) {
+      converged = true;
+      break;
+    }
+  }
+  // eigenvalues: the diagonal; sort ascending (ties by index) with the vectors
+  if (tid < n) dsort[tid] = a[tid + n * tid].x;
+  __syncthreads();
+  if (tid < n) {
+    const double d = dsort[tid];
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (dsort[j] < d || (dsort[j] == d && j < tid)) ? 1 : 0;
+    rank[tid] = r;
+    w[(long)blockIdx.x * n + r] = d;
+  }
+  __syncthreads();
+  for (int t = tid; t < nn; t += SM_NT) {
+    const int i = t % n, j = t / n;
+    const int r = rank[j];
+    if (r < nvec) M[(long)r * lda + i] = v[i + n * j];
+  }
+  if (tid == 0 && info) info[blockIdx.x] = converged ? 0 : 1;
+}
+
+// Matrix b of the stack: inv(chol(Hermitian(A)).U) in place, strict lower triangle zeroed;
+// info[b] (potrf) and info[nbatch + b] (trtri) as rocSOLVER would set them.
+__global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restrict__ A, long lda, long stride, int n,
+                                                          int* __restrict__ info, int nbatch) {
+  extern __shared__ char smem_raw[];
+  cplx<double>* u = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n
+  cplx<double>* x = u + n * n;                                     // n x n
+  __shared__ int bad;
+  cplx<double>* M = A + (long)blockIdx.x * stride;
+  const int tid = threadIdx.x;
+  const int nn = n * n;
+  for (int t = tid; t < nn; t += SM_NT) {
+    const int i = t % n, j = t / n;
+    u[i + n * j] = i <= j ? (i == j ? mk(M[(long)j * lda + i].x, 0.0) : M[(long)j * lda + i]) : mk(0.0, 0.0);
+  }
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  // right-looking: row k of U, then the trailing upper triangle
+  for (int k = 0; k < n; ++k) {
+    const double dkk = u[k + n * k].x;
+    if (!(dkk > 0.0)) {
+      if (tid == 0 && bad == 0) bad = k + 1;
+      __syncthreads();
+      break;
+    }
+    const double d = sqrt(dkk);
+    __syncthreads();
+    for (int j = k + tid; j < n; j += SM_NT) u[k + n * j] = j == k ? mk(d, 0.0) : scal(1.0 / d, u[k + n * j]);
+    __syncthreads();
+    // A_ij -= conj(U_ki) U_kj for k < i <= j
+    const int m = n - k - 1;
+    for (int t = tid; t < m * m; t += SM_NT) {
+      const int i = k + 1 + t % m, j = k + 1 + t / m;
+      if (i <= j) u[i + n * j] = u[i + n * j] - cmulc(u[k + n * j], u[k + n * i]);
+    }
+    __syncthreads();
+  }
+  const int b = blockIdx.x;
+  if (bad) {
+    if (tid == 0) {
+      info[b] = bad;
+      info[nbatch + b] = 0;
+    }
+    return;
+  }
+  // X = U^-1, one column per thread: U X[:, j] = e_j by back substitution
+  for (int j = tid; j < n; j += SM_NT) {
+    for (int i = n - 1; i >= 0; --i) {
+      cplx<double> s = mk(i == j ? 1.0 : 0.0, 0.0);
+      if (i <= j)
+        for (int k = i + 1; k <= j; ++k) s = s - cmul(u[i + n * k], x[k + n * j]);
+      else
+        s = mk(0.0, 0.0);
+      x[i + n * j] = i <= j ? scal(1.0 / u[i + n * i].x, s) : mk(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < nn; t += SM_NT) {
+    const int i = t % n, j = t / n;
+    M[(long)j * lda + i] = x[i + n * j];
+  }
+  if (tid == 0) {
+    info[b] = 0;
+    info[nbatch + b] = 0;
+  }
+}
+
+int heev_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, double* d_w, int nvec,
+                     int* d_info) {
+  if (n < 1 || n > SM_NMAX || nbatch < 1) return set_error(c, PW_ERR_ARG, "heev_small_batch: order out of range");
+  const size_t lds = 2 * (size_t)n * n * sizeof(cplx<double>);
+  hipLaunchKernelGGL(k_heev_jacobi, dim3(nbatch), dim3(SM_NT), lds, c->stream, A, lda, stride, n, d_w, nvec, d_info);
+  return hip_check(c, hipGetLastError(), "k_heev_jacobi");
+}
+
+int chol_inv_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, int* d_info) {
+  if (n < 1 || n > SM_NMAX || nbatch < 1) return set_error(c, PW_ERR_ARG, "chol_inv_small_batch: order out of range");
+  const size_t lds = 2 * (size_t)n * n * sizeof(cplx<double>);
+  hipLaunchKernelGGL(k_chol_inv_small, dim3(nbatch), dim3(SM_NT), lds, c->stream, A, lda, stride, n, d_info, nbatch);
+  return hip_check(c, hipGetLastError(), "k_chol_inv_small");
+}
+
+int small_dense_max_n() { return SM_NMAX; }
+
+}  // namespace pw

@@ -9,8 +9,10 @@ route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! (with symmetrize_rhoe!)
 update! (Poisson, VWN or PBE, V_Total) through the C ABI.  The op_H family also runs on the
 device for norm-conserving `Hamiltonian{PsPot_UPF}` (single grid, collinear), whose
 nonlocal term couples each atom's projectors through its dense Deeq block.  The reference's own
-KS_solve_SCF!, diag_LOBPCG! and diag_davidson! then run unchanged on top of them (they call
-op_H, calc_rhoe! and update!).  A HIP context is attached to each Hamiltonian the first
+KS_solve_SCF! and diag_davidson! then run unchanged on top of them (they call op_H,
+calc_rhoe! and update!).  diag_LOBPCG! itself runs in the library (pw_diag_lobpcg_dev): each
+k-point's block is uploaded once into its device mirror, every iteration's op_H, Gram
+products and small dense problems stay on the device, and the Ritz vectors come back once.  A HIP context is attached to each Hamiltonian the first
 time it is used, keyed by object identity; the device update! installs V_Total as the op_H
 potential itself, and `sync_potential!(Ham)` re-uploads Ham.potentials.Total for code that
 writes it directly.
@@ -23,7 +25,7 @@ module PWDFTHip
 using PWDFT
 import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
               calc_rhoe!, update!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH,
-              PsPot_UPF, BlochWavefunc
+              PsPot_UPF, BlochWavefunc, diag_LOBPCG!
 
 # Hamiltonians whose op_H family runs on the device: GTH, and norm-conserving UPF (the
 # nonlocal term then couples each atom's projectors through its dense Deeq block)
@@ -323,8 +325,38 @@ function E_Ps_loc(Ham::Hamiltonian{PsPot_GTH}, Rhoe::Matrix{Float64})
     return E[]
 end
 
+"""E_Ps_nloc at fixed psiks and occupations (calc_E_Ps_nloc for GTH) as a function of the
+atomic positions through the projectors, with its ChainRulesCore rrule backed by the same
+adjoint kernels as calc_forces_Ps_nloc (dE/dtau = -F_nloc, calc_forces_Ps_nloc.jl:291-411)."""
+function E_Ps_nloc(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc)
+    E, _ = _nloc_energy_forces(Ham, psiks)
+    return E
+end
+
+function _nloc_energy_forces(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc)
+    ctx = context(Ham)
+    F = zeros(Float64, 3, Ham.atoms.Natoms)
+    E = Ref(0.0)
+    wk, Focc = Ham.pw.gvecw.kpoints.wk, Ham.electrons.Focc
+    for ik in 1:Ham.pw.gvecw.kpoints.Nkpt
+        psi = psiks[ik]
+        w = wk[ik] .* Focc[:, ik]
+        check(ctx, ccall((:pw_calc_forces_Ps_nloc, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Int64, Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+                         ctx.h, ik - 1, size(psi, 2), psi, size(psi, 1), w, F, E))
+    end
+    return E[], F
+end
+
 if isdefined(Main, :ChainRulesCore)
     import ChainRulesCore
+    function ChainRulesCore.rrule(::typeof(E_Ps_nloc), Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc)
+        E, F = _nloc_energy_forces(Ham, psiks)
+        pullback(Ebar) = (ChainRulesCore.NoTangent(),
+                          ChainRulesCore.Tangent{typeof(Ham)}(; atoms = (; positions = -Ebar .* F)),
+                          ChainRulesCore.NoTangent())
+        return E, pullback
+    end
     function ChainRulesCore.rrule(::typeof(E_Ps_loc), Ham::Hamiltonian{PsPot_GTH}, Rhoe::Matrix{Float64})
         ctx = context(Ham)
         pw = Ham.pw
@@ -345,6 +377,96 @@ if isdefined(Main, :ChainRulesCore)
                           ChainRulesCore.NoTangent())
         return E[], pullback
     end
+end
+
+# ---- diag_LOBPCG! on the device (diag_LOBPCG.jl:1-28 over k, :51-225 per k) ----------
+
+# Device mirror of a BlochWavefunc, keyed by the identity of the psiks object: one device
+# block per k-point (Ngw[ik] x Nstates), reused across SCF iterations.
+mutable struct DeviceWavefunc
+    ctx::HipCtx
+    blocks::Vector{Ptr{Cvoid}}
+    bytes::Vector{Int}
+end
+
+const MIRRORS = IdDict{Any,DeviceWavefunc}()
+
+function mirror(Ham::HamHip, psiks::BlochWavefunc)
+    ctx = context(Ham)
+    m = get(MIRRORS, psiks, nothing)
+    if m === nothing || m.ctx !== ctx
+        m = DeviceWavefunc(ctx, Ptr{Cvoid}[], Int[])
+        MIRRORS[psiks] = m
+    end
+    while length(m.blocks) < length(psiks)
+        push!(m.blocks, C_NULL)
+        push!(m.bytes, 0)
+    end
+    for (i, psi) in enumerate(psiks)
+        nb = 16 * length(psi)
+        if m.bytes[i] < nb
+            m.blocks[i] != C_NULL && check(ctx, ccall((:pw_free, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, m.blocks[i]))
+            r = Ref{Ptr{Cvoid}}(C_NULL)
+            check(ctx, ccall((:pw_malloc, libpw), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}), ctx.h, nb, r))
+            m.blocks[i] = r[]
+            m.bytes[i] = nb
+        end
+    end
+    return m
+end
+
+"""diag_LOBPCG!(Ham, psi) for the current Ham.ik / Ham.ispin, on the device: the block is
+uploaded to its device mirror, iterated there and copied back (psi is overwritten with the
+Ritz vectors, as the reference does); returns the Ritz values."""
+function diag_LOBPCG!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}; tol=1e-5, NiterMax=100,
+                      verbose=false, verbose_last=false, Nstates_conv=0, dpsi::Ptr{Cvoid}=C_NULL)
+    ctx = context(Ham)
+    Ngw, Nst = size(psi)
+    own = dpsi == C_NULL
+    if own
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ctx, ccall((:pw_malloc, libpw), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}), ctx.h, 16 * Ngw * Nst, r))
+        dpsi = r[]
+    end
+    evals = zeros(Float64, Nst)
+    stats = zeros(Cint, 3)
+    try
+        check(ctx, ccall((:pw_memcpy_h2d, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{ComplexF64}, Csize_t),
+                         ctx.h, dpsi, psi, 16 * Ngw * Nst))
+        check(ctx, ccall((:pw_diag_lobpcg_dev, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Int64, Cdouble, Cint, Cint, Cuint, Ptr{Float64},
+                          Ptr{Cint}),
+                         ctx.h, Ham.ik - 1, Ham.ispin - 1, Nst, dpsi, Ngw, tol, NiterMax, Nstates_conv,
+                         ctx.mixed ? PW_MIXED : UInt32(0), evals, stats))
+        check(ctx, ccall((:pw_memcpy_d2h, libpw), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Cvoid}, Csize_t),
+                         ctx.h, psi, dpsi, 16 * Ngw * Nst))
+    finally
+        own && ccall((:pw_free, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, dpsi)
+    end
+    if verbose || verbose_last
+        println("diag_LOBPCG (device): ", stats[1], " iterations, ", stats[2], " op_H columns")
+    end
+    return evals
+end
+
+"""diag_LOBPCG!(Ham, psiks) over the k-points and spins (diag_LOBPCG.jl:1-28), each block
+iterated in its persistent device mirror."""
+function diag_LOBPCG!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc; tol=1e-5, NiterMax=100,
+                      verbose=false, verbose_last=false, Nstates_conv=0)
+    pw = Ham.pw
+    Nkpt = pw.gvecw.kpoints.Nkpt
+    Nspin = Ham.electrons.Nspin_wf
+    evals = zeros(Float64, Ham.electrons.Nstates, Nkpt * Nspin)
+    m = mirror(Ham, psiks)
+    for ispin in 1:Nspin, ik in 1:Nkpt
+        Ham.ik = ik
+        Ham.ispin = ispin
+        ikspin = ik + (ispin - 1) * Nkpt
+        evals[:, ikspin] = diag_LOBPCG!(Ham, psiks[ikspin]; tol=tol, NiterMax=NiterMax, verbose=verbose,
+                                        verbose_last=verbose_last, Nstates_conv=Nstates_conv,
+                                        dpsi=m.blocks[ikspin])
+    end
+    return evals
 end
 
 """The reference's KS_solve_SCF! on the device path (op_H, calc_rhoe!, update! above),

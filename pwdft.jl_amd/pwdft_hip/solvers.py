@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 import scipy.linalg as sla
 
-from ._lib import PosDefException, DeviceMatrix, PW_MIXED, PW_TERM_ALL, PW_TERM_K, ptr
+from ._lib import PosDefException, DeviceMatrix, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TUNE_LOBPCG_LOCK, ptr
 from .hamiltonian import op_H
 
 EPS = np.finfo(float).eps
@@ -85,124 +85,19 @@ def _colptr(M, j):
 
 
 def _lobpcg_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
-    """diag_LOBPCG! with the small dense problems solved on the device (pw_potrf_dev,
-    pw_trtri_dev, pw_heevd_dev, pw_hegvd_dev): the Gram and overlap matrices and the Ritz
-    coefficients stay in HBM; only eigenvalues reach the host for the convergence test."""
-    D = _Dense(Ham, mixed)
-    D64 = _Dense(Ham, False)   # the Gram matrices that are Cholesky-factorised stay FP64
-    Nb, Nst = X.rows, X.cols
-    c = Ham.ctx
-    Q = DeviceMatrix(c, Nb, 3 * Nst)
-    HQ = DeviceMatrix(c, Nb, 3 * Nst)
-    Tmp = DeviceMatrix(c, Nb, 4 * Nst)
-    Sm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
-    Gm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
-    Cm = DeviceMatrix(c, 3 * Nst, 3 * Nst)
-    S2 = DeviceMatrix(c, Nst, Nst)
-    S3 = DeviceMatrix(c, Nst, Nst)
-    x, w, p = Q.col(0), Q.col(Nst), Q.col(2 * Nst)
-    hx, hw, hp = HQ.col(0), HQ.col(Nst), HQ.col(2 * Nst)
-    t0, t1, t2, t3 = Tmp.col(0), Tmp.col(Nst), Tmp.col(2 * Nst), Tmp.col(3 * Nst)
-    ev = np.zeros(3 * Nst)
-    evp = ev.ctypes.data_as(ctypes.c_void_p)
-
-    def chol_inv_apply(src, hsrc, dst, hdst):
-        # U = cholesky(Hermitian(src' src)).U ; dst = src inv(U), hdst = hsrc inv(U)
-        D64.gemm(1, Nst, Nst, Nb, src, Nb, src, Nb, S3.p, Nst)
-        c.call("pw_potrf_dev", Nst, S3.p, Nst)
-        c.call("pw_trtri_dev", Nst, S3.p, Nst)
-        c.call("pw_herm_fix_dev", Nst, S3.p, Nst, 1)
-        D.gemm(0, Nb, Nst, Nst, src, Nb, S3.p, Nst, dst, Nb)
-        D.gemm(0, Nb, Nst, Nst, hsrc, Nb, S3.p, Nst, hdst, Nb)
-
-    D.copy(Nb, Nst, X.p, x)
-    D.opH(x, hx, Nst, Nb)
-    ncols = Nst
-    nlock = 0
-    lock_ws = None
-    evals_old = np.zeros(Nst)
-    SMALL = 10.0 * EPS
-    it_done = 0
-    rr_evals = None
-    for it in range(1, NiterMax + 1):
-        it_done = it
-        D.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, S2.p, Nst)          # X'HX
-        c.call("pw_herm_fix_dev", Nst, S2.p, Nst, 0)
-        if rr_evals is None:
-            c.call("pw_memcpy_d2d", S3.p, S2.p, 16 * Nst * Nst)
-            c.call("pw_heev_values_dev", Nst, S3.p, Nst, evp)      # eigvals(Hermitian(XHX))
-            evals = ev[:Nst].copy()
-        else:
-            # X and HX are the previous Rayleigh-Ritz vectors, so eigvals(X'HX) are that
-            # step's lowest Ritz values (equal to rounding); they are reused instead of
-            # decomposing X'HX again
-            evals = rr_evals
-        rr_evals = None
-        devals = np.abs(evals - evals_old)
-        evals_old = evals.copy()
-        D.copy(Nb, Nst, hx, w)                                      # R = HX - X*XHX
-        D.gemm(0, Nb, Nst, Nst, x, Nb, S2.p, Nst, w, Nb, alpha=_MONE, beta=_ONE)
-        nconv = int(np.count_nonzero(devals < tol))
-        ilock = np.flatnonzero(devals <= SMALL)
-        iact = np.flatnonzero(devals > SMALL)
-        if nconv >= Nstates_conv:
-            break
-        D.kprec(w, w, Nst, Nb)
-        if len(ilock) > 0:
-            nlock += 1
-            if lock_ws is None:
-                lock_ws = _LockScratch(c, Nb, Nst)
-            ncols += _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact, scratch=lock_ws)
-            continue
-        D.opH(w, hw, Nst, Nb)
-        ncols += Nst
-        chol_inv_apply(w, hw, t0, t1)
-        D.copy(Nb, Nst, t0, w)
-        D.copy(Nb, Nst, t1, hw)
-        nq = 3 * Nst if it > 1 else 2 * Nst
-        # T = Q'HQ and G = Q'Q: only the upper block triangle, the part Hermitian() reads
-        # (block column j needs rows of blocks 0..j); G in FP64 since it is factorised
-        for j in range(nq // Nst):
-            m = (j + 1) * Nst
-            D.gemm(1, m, Nst, Nb, Q.p, Nb, HQ.col(j * Nst), Nb, ctypes.c_void_p(Sm.p.value + 16 * nq * j * Nst), nq)
-            D64.gemm(1, m, Nst, Nb, Q.p, Nb, Q.col(j * Nst), Nb, ctypes.c_void_p(Gm.p.value + 16 * nq * j * Nst), nq)
-        # eigen(T, G): G = U^H U, C = U^-H T U^-1, C y = l y, x = U^-1 y (the reduction
-        # zhegv performs, with the standard eigenproblem through rocSOLVER's zheevd)
-        c.call("pw_herm_fix_dev", nq, Sm.p, nq, 0)
-        c.call("pw_potrf_dev", nq, Gm.p, nq)
-        c.call("pw_trtri_dev", nq, Gm.p, nq)
-        c.call("pw_herm_fix_dev", nq, Gm.p, nq, 1)                   # Ui = U^-1, upper
-        D64.gemm(0, nq, nq, nq, Sm.p, nq, Gm.p, nq, Cm.p, nq)       # T Ui
-        D64.gemm(1, nq, nq, nq, Gm.p, nq, Cm.p, nq, Sm.p, nq)       # Ui^H T Ui
-        c.call("pw_heevd_dev", nq, Sm.p, nq, evp)
-        rr_evals = ev[:Nst].copy()
-        D64.gemm(0, nq, Nst, nq, Gm.p, nq, Sm.p, nq, Cm.p, nq)      # x = Ui y, lowest Nst
-        c.call("pw_memcpy_d2d", Sm.p, Cm.p, 16 * nq * Nst)            # Ritz vectors in Sm
-        D.gemm(0, Nb, Nst, nq, Q.p, Nb, Sm.p, nq, t0, Nb)
-        D.gemm(0, Nb, Nst, nq, HQ.p, Nb, Sm.p, nq, t1, Nb)
-        if it > 1:
-            u23 = ctypes.c_void_p(Sm.p.value + 16 * Nst)              # rows Nst:3Nst of U
-            D.gemm(0, Nb, Nst, 2 * Nst, w, Nb, u23, nq, t2, Nb)
-            D.gemm(0, Nb, Nst, 2 * Nst, hw, Nb, u23, nq, t3, Nb)
-            D.copy(Nb, Nst, t0, x)
-            D.copy(Nb, Nst, t1, hx)
-            chol_inv_apply(t2, t3, p, hp)
-        else:
-            D.copy(Nb, Nst, w, p)
-            D.copy(Nb, Nst, hw, hp)
-            D.copy(Nb, Nst, t0, x)
-            D.copy(Nb, Nst, t1, hx)
-    D.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, S2.p, Nst)
-    c.call("pw_herm_fix_dev", Nst, S2.p, Nst, 2)
-    c.call("pw_heevd_dev", Nst, S2.p, Nst, evp)
-    evals = ev[:Nst].copy()
-    D.gemm(0, Nb, Nst, Nst, x, Nb, S2.p, Nst, X.p, Nb)
-    c.call("pw_synchronize")
-    for m in (Q, HQ, Tmp, Sm, Gm, Cm, S2, S3):
-        m.free()
-    if lock_ws is not None:
-        lock_ws.free()
-    return evals, {"iter": it_done, "ncols": ncols, "locks": nlock}
+    """diag_LOBPCG! through the C ABI (pw_diag_lobpcg_dev): every block, Gram matrix, small
+    dense problem and Ritz coefficient stays in HBM; the host receives the Ritz values once
+    per iteration for the |dlambda| test."""
+    Nst = X.cols
+    ev = np.zeros(Nst)
+    st = np.zeros(3, dtype=np.int32)
+    # EPS (the 10 eps lock threshold's unit) may be raised to exercise the lock branch
+    lock = 0 if EPS == np.finfo(float).eps else int(round(-np.log10(EPS)))
+    Ham.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_LOCK, lock)
+    Ham.ctx.call("pw_diag_lobpcg_dev", Ham.ik, Ham.ispin, Nst, X.p, X.rows, float(tol), int(NiterMax),
+                 int(Nstates_conv), PW_MIXED if mixed else 0, ev.ctypes.data_as(ctypes.c_void_p),
+                 st.ctypes.data_as(ctypes.c_void_p))
+    return ev, {"iter": int(st[0]), "ncols": int(st[1]), "locks": int(st[2])}
 
 
 DENSE = "device"   # where diag_LOBPCG_ solves its small dense problems: "device" or "host"
@@ -216,9 +111,10 @@ def diag_LOBPCG_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, den
     """diag_LOBPCG! for the current Ham.ik (diag_LOBPCG.jl:51-225).  X is a DeviceMatrix
     (Ngw x Nst, orthonormal) overwritten with the Ritz vectors.  Returns (evals, stats)
     where stats = {"iter": iterations, "ncols": op_H columns applied, "locks": iterations
-    that took the lock branch}.  dense = "device"
-    (default) runs the Cholesky and eigen problems through rocSOLVER on the context stream;
-    "host" runs them with LAPACK on the host, as the reference does."""
+    that took the lock branch}.  dense = "device" (default) runs the whole solver in the
+    library (pw_diag_lobpcg_dev), its small dense problems on the device; "host" keeps the
+    Ngw-sized products on the device and solves the small dense problems with LAPACK on
+    the host, as the reference does."""
     mixed = Ham.mixed if mixed is None else mixed
     if Nstates_conv == 0:
         Nstates_conv = X.cols

@@ -200,3 +200,25 @@ def test_rhoe_symmetrizer_and_vectors():
     vs = B.symmetrize_vector(si, ops, vv)
     assert np.abs(B.symmetrize_vector(si, ops, vs) - vs).max() < 1e-14
     assert np.abs(vs).max() < 1e-14   # no symmetric force on the diamond sites
+
+
+def test_packed_fp32_guard_catches_packed_instructions(tmp_path):
+    """The build's check disassembles every gfx950 code object: a float2 multiply-add
+    compiled without -packed-fp32-ops yields v_pk_fma_f32 and is refused; the same source
+    with the library's flags passes."""
+    import subprocess
+    from pwdft_hip import _build as B
+    src = tmp_path / "pk.hip"
+    src.write_text("#include <hip/hip_runtime.h>\n"
+                   "extern \"C\" __global__ void k(float2* a, const float2* b, float s, int n) {\n"
+                   "  int i = blockIdx.x * blockDim.x + threadIdx.x;\n"
+                   "  if (i < n) { float2 x = a[i], y = b[i]; x.x = x.x * s + y.x; x.y = x.y * s + y.y; a[i] = x; }\n"
+                   "}\n")
+    base = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-fPIC", "-shared"]
+    bad, good = tmp_path / "bad.so", tmp_path / "good.so"
+    subprocess.run(base + ["-o", str(bad), str(src)], check=True, capture_output=True)
+    subprocess.run(["/opt/rocm/bin/hipcc"] + B.HIPCC_FLAGS + ["-o", str(good), str(src)], check=True,
+                   capture_output=True)
+    with pytest.raises(RuntimeError, match="packed FP32"):
+        B.check_no_packed_fp32(str(bad))
+    assert B.check_no_packed_fp32(str(good)) == 1
