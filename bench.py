@@ -240,16 +240,26 @@ def time_batched_k(steps, warmup):
         print(f"# Al 182k {tag}: batched op_H {tb * 1e3:.3f} ms, per-k loop {tl * 1e3:.3f} ms",
               file=sys.stderr, flush=True)
     H.mixed = False
-    # warm-up over two iterations: the second is the first to reach the 3 Nst-sized dense
-    # problems and the P block, whose first calls load kernels and size library workspaces
-    ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=2)
-    H.ctx.call("pw_synchronize")
-    t0 = time.perf_counter()
-    _, st = ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=4)
-    H.ctx.call("pw_synchronize")
-    dt = time.perf_counter() - t0
-    res["lobpcg_batch"] = {"s_per_iter": dt / 4, "band_apps_per_s": sum(x["ncols"] for x in st) / dt}
-    print(f"# Al 182k batched LOBPCG: {dt / 4 * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
+    # batched LOBPCG: every phase of an iteration replayed from its captured hipGraph (the
+    # warm-up calls run each phase eagerly once, then capture it), and the same iterations
+    # enqueued eagerly for comparison
+    from pwdft_hip._lib import PW_TUNE_LOBPCG_GRAPH
+    res["lobpcg_batch"] = {}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        for graph in (1, 0):
+            H.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_GRAPH, graph)
+            for _ in range(2):
+                ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=4, mixed=mixed)
+            H.ctx.call("pw_synchronize")
+            t0 = time.perf_counter()
+            _, st = ph.diag_LOBPCG_batch(H, ps, tol=0.0, NiterMax=4, mixed=mixed)
+            H.ctx.call("pw_synchronize")
+            dt = time.perf_counter() - t0
+            key = tag + ("_graph" if graph else "_eager")
+            res["lobpcg_batch"][key] = {"s_per_iter": dt / 4, "band_apps_per_s": sum(x["ncols"] for x in st) / dt}
+            print(f"# Al 182k batched LOBPCG {key}: {dt / 4 * 1e3:.2f} ms/iteration", file=sys.stderr, flush=True)
+    H.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_GRAPH, 1)
     return res
 
 

@@ -85,10 +85,13 @@ enum {
                                    the order fits; 1 rocSOLVER's ZHEEVD throughout */
   PW_TUNE_LOBPCG_LOCK = 10,     /* lock threshold of pw_diag_lobpcg_dev: 0 (default) the
                                    reference's 10 eps; k > 0: 10 * 10^-k (tests of the branch) */
-  PW_TUNE_FUSED_BATCH = 11      /* pw_op_H_batch_dev: 1 (default) one fused launch, a workgroup
+  PW_TUNE_FUSED_BATCH = 11,     /* pw_op_H_batch_dev: 1 (default) one fused launch, a workgroup
                                    per (k-point, band) holding the band's whole grid in LDS,
                                    wherever the grid fits (20^3 in FP64); 0: the three-stage
                                    column / plane / column launches and batched GEMMs */
+  PW_TUNE_LOBPCG_GRAPH = 12     /* pw_diag_lobpcg_batch_dev: 1 (default) replay each step from
+                                   a captured hipGraph once it has run eagerly (orders 3 Nst <=
+                                   64); 0: every step enqueued eagerly */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 
@@ -311,6 +314,22 @@ int pw_zgemm_batch_dev(pw_ctx* ctx, int transa, int nbatch, const int* m, const 
  * cholesky() throws PosDefException.  Synchronises once per iteration. */
 int pw_diag_lobpcg_dev(pw_ctx* ctx, int ik, int ispin, int nst, void* X, int64_t ldx, double tol, int niter_max,
                        int nstates_conv, unsigned flags, double* evals, int* stats);
+
+/* diag_LOBPCG!(Ham, psiks) over the nk k-points iks at once (diag_LOBPCG.jl:1-28 running
+ * :51-225 for every k-point, KS_solve_SCF.jl:189-191): X[i] (device, Ngw[iks[i]] x nst,
+ * leading dimension ldx[i], orthonormal columns) is overwritten with the Ritz vectors of
+ * k-point iks[i], evals (host, nst x nk, column i for k-point iks[i]) with its Ritz values.
+ * Every k-point iterates as diag_LOBPCG! would on its own -- the same |dlambda| test, the
+ * same lock branch -- and leaves the batch once it has converged; each step of the k-points
+ * still active is one batched launch per operation (op_H of all their bands in one launch),
+ * and with 3 nst <= 64 (small dense problems on one workgroup each) the steps are captured
+ * into hipGraphs once seen and replayed (PW_TUNE_LOBPCG_GRAPH).  Synchronises once per
+ * iteration.  tol, niter_max, nstates_conv, flags as pw_diag_lobpcg_dev.  stats (host,
+ * 3 x nk ints, may be NULL): per k-point iterations, op_H columns, lock-branch iterations.
+ * Requires 3 nst <= Ngw of every k-point. */
+int pw_diag_lobpcg_batch_dev(pw_ctx* ctx, int nk, const int* iks, int ispin, int nst, void* const* X,
+                             const int64_t* ldx, double tol, int niter_max, int nstates_conv, unsigned flags,
+                             double* evals, int* stats);
 
 /* The host LAPACK calls of diag_LOBPCG! / diag_davidson! / ortho_sqrt (cholesky(Hermitian(C)),
  * inv(U), eigen(Hermitian(T), Hermitian(G)), eigen(Hermitian(S))) on device matrices

@@ -103,8 +103,20 @@ int desc_get(pw_ctx* c, const void* data, size_t bytes, const void** out) {
     slot = c->desc_next;
     c->desc_next = (c->desc_next + 1) % kSlots;
   }
-  int rc = ensure(c, &c->desc_dev[slot], &c->desc_cap[slot], bytes);
-  if (rc) return rc;
+  // allocated directly, not through ensure(): a new slot leaves every captured graph valid
+  // (no slot a live graph reads is ever freed or rewritten)
+  if (c->desc_cap[slot] < bytes) {
+    hipFree(c->desc_dev[slot]);
+    c->desc_dev[slot] = nullptr;
+    c->desc_cap[slot] = 0;
+    const size_t sz = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&c->desc_dev[slot], sz);
+    if (e != hipSuccess) {
+      c->desc_dev[slot] = nullptr;
+      return set_error(c, PW_ERR_NOMEM, std::string("hipMalloc descriptor: ") + hipGetErrorString(e));
+    }
+    c->desc_cap[slot] = sz;
+  }
   const char* p = reinterpret_cast<const char*>(data);
   c->desc_host[slot].assign(p, p + bytes);
   // ordered on the stream before the kernels that read it; the host copy it reads from is
@@ -180,7 +192,7 @@ __global__ void __launch_bounds__(256) k_elem_batch(const EBatch* __restrict__ e
   }
 }
 
-static int elem_batch(pw_ctx* c, std::vector<EBatch>& e0) {
+int elem_batch(pw_ctx* c, std::vector<EBatch>& e0) {
   std::vector<EBatch> e;
   long blk = 0;
   for (EBatch x : e0) {

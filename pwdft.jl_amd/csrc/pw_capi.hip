@@ -37,6 +37,7 @@ int ensure(pw_ctx* c, void** buf, size_t* have, size_t need) {
     *have = 0;
   }
   size_t sz = std::max<size_t>(need, 256);
+  c->graph_gen++;
   hipError_t e = hipMalloc(buf, sz);
   if (e != hipSuccess) {
     *buf = nullptr;
@@ -90,6 +91,7 @@ void free_k(KData& k) {
 
 template <typename T>
 int upload(pw_ctx* c, T** dst, const T* src, size_t n) {
+  c->graph_gen++;
   if (*dst) {
     hipFree(*dst);
     *dst = nullptr;
@@ -298,6 +300,7 @@ int pw_ctx_destroy(pw_ctx* c) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  free_lob_graphs(c);
   for (auto& k : c->kd) free_k(k);
   for (int d = 0; d < 3; ++d) {
     hipFree(c->d_tw64[d]);
@@ -373,6 +376,10 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       return PW_OK;
     case PW_TUNE_FUSED_BATCH:
       c->fused_batch = value != 0;
+      return PW_OK;
+    case PW_TUNE_LOBPCG_GRAPH:
+      c->lob_graph = value != 0;
+      if (!c->lob_graph) free_lob_graphs(c);
       return PW_OK;
     case PW_TUNE_LOBPCG_LOCK:
       if (value < 0 || value > 300) return set_error(c, PW_ERR_ARG, "pw_tune_set: lock exponent must be 0..300");

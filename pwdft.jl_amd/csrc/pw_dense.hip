@@ -49,6 +49,12 @@ void free_dense(pw_ctx* c) {
   hipFree(c->d_lob2);
   c->d_lob2 = nullptr;
   c->lob2_bytes = 0;
+  hipFree(c->d_lobb);
+  c->d_lobb = nullptr;
+  c->lobb_bytes = 0;
+  hipFree(c->d_lobs);
+  c->d_lobs = nullptr;
+  c->lobs_bytes = 0;
 }
 
 static int st_check(pw_ctx* c, rocblas_status s, const char* what) {

@@ -142,6 +142,17 @@ struct EBatch {
   int pad;
 };
 
+// A captured step of the batched LOBPCG driver (pw_lobpcg.hip), keyed by the k-points and
+// shapes it was captured for; usable while the context's graph_gen is unchanged.
+struct LobGraph {
+  std::vector<int> key;
+  void* exec = nullptr;    // hipGraphExec_t
+  void* graph = nullptr;   // hipGraph_t
+  unsigned gen = 0;
+  bool failed = false;     // capture was refused: this step runs eagerly
+  long stamp = 0;          // last use (the least recently used entry is evicted)
+};
+
 }  // namespace pw
 
 struct pw_ctx {
@@ -234,6 +245,14 @@ struct pw_ctx {
   void* d_lob = nullptr;   size_t lob_bytes = 0;     // LOBPCG blocks and small matrices (pw_lobpcg.hip)
   void* d_lob2 = nullptr;  size_t lob2_bytes = 0;    // LOBPCG lock-branch blocks
   int lobpcg_lock_exp = 0;                            // 0: lock at 10 eps; k: at 10 * 10^-k
+  void* d_lobb = nullptr;  size_t lobb_bytes = 0;    // batched LOBPCG: Q, HQ, Tmp of every k-point
+  void* d_lobs = nullptr;  size_t lobs_bytes = 0;    // batched LOBPCG: stacked small matrices, statuses
+  bool lob_graph = true;                              // batched LOBPCG steps replayed from hipGraphs
+  std::vector<pw::LobGraph> lob_graphs;               // captured steps (pw_lobpcg.hip)
+  long lob_stamp = 0;
+  // bumped whenever device memory that a captured step may address is allocated or freed;
+  // a captured step whose generation differs is discarded, never replayed
+  unsigned graph_gen = 0;
   bool fused_batch = true;                            // batched op_H: one fused launch where the box fits LDS
   // density side of an SCF iteration (pw_scf.hip)
   double* d_vps = nullptr;                            // V_Ps_loc(r) [N]
@@ -263,6 +282,8 @@ struct ProfScope {
 };
 
 int set_error(pw_ctx* c, int code, const std::string& msg);
+// Destroys the batched LOBPCG driver's captured steps (pw_lobpcg.hip).
+void free_lob_graphs(pw_ctx* c);
 int hip_check(pw_ctx* c, hipError_t e, const char* what);
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need);
 
@@ -327,6 +348,8 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
 int heevd_dist_max_n();
 // dst[:, 0:ncols] = src[:, 0:ncols] (nrows rows) on the context stream.
 int copy_cols(pw_ctx* c, int nrows, int ncols, const cplx<double>* src, long lds, cplx<double>* dst, long ldd);
+// One launch over the column blocks e (copies, or Kprec where gk2 is set) (pw_batch.hip).
+int elem_batch(pw_ctx* c, std::vector<EBatch>& e);
 // Hermitian fix-ups of a strided stack (pw_herm_fix_dev modes 0, 1, 2).
 int herm_fix_launch(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t stride, int mode);
 // Cholesky factor and its inverse in place (infos into d_info[0..1]), no synchronisation.

@@ -319,3 +319,528 @@ extern "C" int pw_diag_lobpcg_dev(pw_ctx* c, int ik, int ispin, int nst, void* X
   }
   return PW_OK;
 }
+
+// ------------------------------------------------------------------ all k-points at once
+//
+// pw_diag_lobpcg_batch_dev runs the iteration above for many k-points in lockstep.  Every
+// operation of an iteration is one launch for all k-points still iterating: op_H of all
+// their bands (pw_op_H_batch_dev; for grids whose box fits LDS one fused workgroup per
+// (k-point, band)), the Gram products and column updates (zgemm_batch), copies and Kprec
+// (elem_batch), and the small dense problems as strided stacks, one workgroup per matrix
+// (pw_small.hip) or rocSOLVER's batched routines beyond order 64.  An iteration is two
+// phases with one synchronisation between them:
+//   check(S):  S_X = X^H H X (Hermitian), R = HX - X S_X into the W block, eigvals(S_X);
+//              the host reads the eigenvalues and statuses and decides, per k-point, as
+//              diag_LOBPCG.jl:88-97,125 does: converged (leaves the batch), lock branch
+//              (:125-164, done per k-point with the single-k code above), or a plain step;
+//   step(S):   :99-203 for the k-points S that take the plain step.
+// A phase that has run once eagerly for the same k-points and shapes is captured into a
+// hipGraph the next time and replayed from then on (orders 3 Nst <= 64, whose dense
+// problems need no library workspace); the graphs persist in the context across calls (an
+// SCF calls the solver once per iteration with the same k-points) and are discarded when
+// any device buffer they address is reallocated.
+
+namespace pw {
+
+void free_lob_graphs(pw_ctx* c) {
+  if (c->lob_graphs.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& g : c->lob_graphs) {
+    if (g.exec) {
+      hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(g.exec));
+      c->graphs_live--;
+    }
+    if (g.graph) hipGraphDestroy(reinterpret_cast<hipGraph_t>(g.graph));
+  }
+  c->lob_graphs.clear();
+  if (c->graphs_live == 0) {
+    for (void* p : c->retired) hipFree(p);
+    c->retired.clear();
+  }
+}
+
+namespace {
+
+constexpr int kMaxLobGraphs = 64;
+
+GBatch gb(const cplx<double>* A, long lda, const cplx<double>* B, long ldb, cplx<double>* C, long ldc, int m, int n,
+          int k) {
+  GBatch g{};
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.ldc = ldc;
+  g.m = m;
+  g.n = n;
+  g.k = k;
+  return g;
+}
+
+EBatch eb(const cplx<double>* in, long ldin, cplx<double>* out, long ldout, int m, int n, const double* gk2 = nullptr) {
+  EBatch e{};
+  e.in = in;
+  e.out = out;
+  e.gk2 = gk2;
+  e.ldin = ldin;
+  e.ldout = ldout;
+  e.m = m;
+  e.n = n;
+  return e;
+}
+
+struct BatchLobpcg {
+  pw_ctx* c;
+  int nk, Nst, ispin;
+  unsigned hflags;
+  bool use_graph;
+  std::vector<int> iks, rows;
+  std::vector<cplx<double>*> blk;   // k-point i: [Q (3 Nst) | HQ (3 Nst) | Tmp (4 Nst)], rows[i] rows
+  // stacked small matrices, by position in the set of the phase
+  cplx<double> *sX, *sX2, *sC, *sT, *sG, *sM;
+  double *dwx, *dw;   // eigvals of S_X (Nst per position), Rayleigh-Ritz values (3 Nst per position)
+  int* info;          // 8 nk status words (layout below)
+  // single-k workspace of the lock branch
+  cplx<double> *lSm, *lGm, *lCm, *lS2, *lS3;
+  double *ldw, *lE;
+  int* linfo;
+  std::vector<int> key0;
+
+  cplx<double>* q(int i, int j) { return blk[i] + (long)j * rows[i]; }
+  cplx<double>* hq(int i, int j) { return blk[i] + (long)(3 * Nst + j) * rows[i]; }
+  cplx<double>* tmp(int i, int j) { return blk[i] + (long)(6 * Nst + j) * rows[i]; }
+  cplx<double>* sx(int p) { return sX + (long)p * Nst * Nst; }
+  cplx<double>* sc(int p) { return sC + (long)p * Nst * Nst; }
+  int* inf_check() { return info; }          // nk: eigvals(S_X) and the final Rayleigh-Ritz
+  int* inf_w() { return info + nk; }         // 2 nk: cholesky of W^H W
+  int* inf_g() { return info + 3 * nk; }     // 2 nk: cholesky of G
+  int* inf_rr() { return info + 5 * nk; }    // nk: Rayleigh-Ritz eigenproblem
+  int* inf_p() { return info + 6 * nk; }     // 2 nk: cholesky of P^H P
+
+  template <class F>
+  int gemm(const std::vector<int>& S, int ta, cplx<double> alpha, cplx<double> beta, int per, F f) {
+    std::vector<GBatch> g;
+    g.reserve(S.size() * per);
+    for (int r = 0; r < per; ++r)
+      for (size_t p = 0; p < S.size(); ++p) g.push_back(f(r, (int)p, S[p]));
+    return zgemm_batch(c, ta, g, alpha, beta);
+  }
+  template <class F>
+  int elem(const std::vector<int>& S, int per, F f) {
+    std::vector<EBatch> e;
+    e.reserve(S.size() * per);
+    for (int r = 0; r < per; ++r)
+      for (size_t p = 0; p < S.size(); ++p) e.push_back(f(r, (int)p, S[p]));
+    return elem_batch(c, e);
+  }
+  int chol(int n, int nb, cplx<double>* A, int* inf) {
+    return pw_chol_inv_batch_async_dev(c, n, nb, A, n, (int64_t)n * n, inf);
+  }
+  int heev(int n, int nb, cplx<double>* A, double* w, int nvec, int* inf) {
+    if (c->eig_path == 0 && n <= small_dense_max_n()) return heev_small_batch(c, n, nb, A, n, (long)n * n, w, nvec, inf);
+    return pw_heevd_batch_async_dev(c, n, nb, A, n, (int64_t)n * n, w, inf);
+  }
+
+  // S_X = X^H H X, R = HX - X S_X (W block), eigvals(S_X) into dwx
+  int check(const std::vector<int>& S) {
+    const int ns = (int)S.size(), N = Nst;
+    int rc = gemm(S, 1, kOne, kZero, 1, [&](int, int p, int i) {
+      return gb(q(i, 0), rows[i], hq(i, 0), rows[i], sx(p), N, N, N, rows[i]);
+    });
+    if (!rc) rc = herm_fix_launch(c, N, ns, sX, N, (int64_t)N * N, 0);
+    if (!rc) rc = elem(S, 1, [&](int, int, int i) { return eb(hq(i, 0), rows[i], q(i, N), rows[i], rows[i], N); });
+    if (!rc) rc = gemm(S, 0, kMinusOne, kOne, 1, [&](int, int p, int i) {
+      return gb(q(i, 0), rows[i], sx(p), N, q(i, N), rows[i], rows[i], N, N);
+    });
+    if (!rc) rc = copy_cols(c, N * N * ns, 1, sX, (long)N * N * ns, sX2, (long)N * N * ns);
+    if (!rc) rc = heev(N, ns, sX2, dwx, 0, inf_check());
+    return rc;
+  }
+
+  // diag_LOBPCG.jl:99-203 for the k-points S (W holds R); nq = 2 Nst in the first
+  // iteration, 3 Nst after it
+  int step(const std::vector<int>& S, int nq) {
+    const int ns = (int)S.size(), N = Nst;
+    std::vector<int> ik(ns);
+    std::vector<void*> src(ns), dst(ns);
+    std::vector<int64_t> ld(ns);
+    for (int p = 0; p < ns; ++p) {
+      const int i = S[p];
+      ik[p] = iks[i];
+      src[p] = q(i, N);
+      dst[p] = hq(i, N);
+      ld[p] = rows[i];
+    }
+    auto st = [&](int p) { return sT + (long)p * nq * nq; };
+    auto sg = [&](int p) { return sG + (long)p * nq * nq; };
+    auto sm = [&](int p) { return sM + (long)p * nq * nq; };
+    // W = Kprec(R), HW
+    int rc = elem(S, 1, [&](int, int, int i) {
+      return eb(q(i, N), rows[i], q(i, N), rows[i], rows[i], N, c->kd[iks[i]].d_gk2);
+    });
+    if (!rc) rc = pw_op_H_batch_dev(c, ns, ik.data(), ispin, N, src.data(), ld.data(), dst.data(), ld.data(), hflags);
+    // W, HW <- W U^-1, HW U^-1 with U = chol(W^H W)
+    if (!rc) rc = gemm(S, 1, kOne, kZero, 1, [&](int, int p, int i) {
+      return gb(q(i, N), rows[i], q(i, N), rows[i], sc(p), N, N, N, rows[i]);
+    });
+    if (!rc) rc = chol(N, ns, sC, inf_w());
+    if (!rc) rc = gemm(S, 0, kOne, kZero, 2, [&](int r, int p, int i) {
+      return gb(r ? hq(i, N) : q(i, N), rows[i], sc(p), N, tmp(i, r * N), rows[i], rows[i], N, N);
+    });
+    if (!rc) rc = elem(S, 2, [&](int r, int, int i) {
+      return eb(tmp(i, r * N), rows[i], r ? hq(i, N) : q(i, N), rows[i], rows[i], N);
+    });
+    // T = Q^H H Q, G = Q^H Q
+    if (!rc) rc = gemm(S, 1, kOne, kZero, 2, [&](int r, int p, int i) {
+      return gb(q(i, 0), rows[i], r ? q(i, 0) : hq(i, 0), rows[i], r ? sg(p) : st(p), nq, nq, nq, rows[i]);
+    });
+    // eigen(Hermitian(T), Hermitian(G)): G = U^H U, U^-H T U^-1 y = l y, coefficients U^-1 y
+    if (!rc) rc = herm_fix_launch(c, nq, ns, sT, nq, (int64_t)nq * nq, 0);
+    if (!rc) rc = chol(nq, ns, sG, inf_g());
+    if (!rc) rc = gemm(S, 0, kOne, kZero, 1, [&](int, int p, int) { return gb(st(p), nq, sg(p), nq, sm(p), nq, nq, nq, nq); });
+    if (!rc) rc = gemm(S, 1, kOne, kZero, 1, [&](int, int p, int) { return gb(sg(p), nq, sm(p), nq, st(p), nq, nq, nq, nq); });
+    if (!rc) rc = heev(nq, ns, sT, dw, N, inf_rr());
+    if (!rc) rc = gemm(S, 0, kOne, kZero, 1, [&](int, int p, int) { return gb(sg(p), nq, st(p), nq, sm(p), nq, nq, N, nq); });
+    // X U, HQ U into Tmp
+    if (!rc) rc = gemm(S, 0, kOne, kZero, 2, [&](int r, int p, int i) {
+      return gb(r ? hq(i, 0) : q(i, 0), rows[i], sm(p), nq, tmp(i, r * N), rows[i], rows[i], N, nq);
+    });
+    if (rc) return rc;
+    if (nq == 3 * N) {
+      // P = [W P] U[Nst:, :], HP likewise, then re-orthonormalised
+      rc = gemm(S, 0, kOne, kZero, 2, [&](int r, int p, int i) {
+        return gb(r ? hq(i, N) : q(i, N), rows[i], sm(p) + N, nq, tmp(i, (2 + r) * N), rows[i], rows[i], N, 2 * N);
+      });
+      if (!rc) rc = elem(S, 2, [&](int r, int, int i) {
+        return eb(tmp(i, r * N), rows[i], r ? hq(i, 0) : q(i, 0), rows[i], rows[i], N);
+      });
+      if (!rc) rc = gemm(S, 1, kOne, kZero, 1, [&](int, int p, int i) {
+        return gb(tmp(i, 2 * N), rows[i], tmp(i, 2 * N), rows[i], sc(p), N, N, N, rows[i]);
+      });
+      if (!rc) rc = chol(N, ns, sC, inf_p());
+      if (!rc) rc = gemm(S, 0, kOne, kZero, 2, [&](int r, int p, int i) {
+        return gb(tmp(i, (2 + r) * N), rows[i], sc(p), N, r ? hq(i, 2 * N) : q(i, 2 * N), rows[i], rows[i], N, N);
+      });
+    } else {
+      // P = W, HP = HW; X, HX from Tmp
+      rc = elem(S, 4, [&](int r, int, int i) {
+        if (r == 0) return eb(q(i, N), rows[i], q(i, 2 * N), rows[i], rows[i], N);
+        if (r == 1) return eb(hq(i, N), rows[i], hq(i, 2 * N), rows[i], rows[i], N);
+        if (r == 2) return eb(tmp(i, 0), rows[i], q(i, 0), rows[i], rows[i], N);
+        return eb(tmp(i, N), rows[i], hq(i, 0), rows[i], rows[i], N);
+      });
+    }
+    return rc;
+  }
+
+  // Runs one phase: eagerly the first time its key is seen, captured into a graph the
+  // second time, replayed from the graph after that.
+  template <class F>
+  int run(int kind, int nq, const std::vector<int>& S, F enqueue) {
+    if (!use_graph) return enqueue();
+    std::vector<int> key = key0;
+    key.push_back(kind);
+    key.push_back(nq);
+    key.push_back((int)S.size());
+    key.insert(key.end(), S.begin(), S.end());
+    auto& G = c->lob_graphs;
+    int idx = -1;
+    for (int j = 0; j < (int)G.size(); ++j)
+      if (G[j].gen == c->graph_gen && G[j].key == key) {
+        idx = j;
+        break;
+      }
+    if (idx < 0) {
+      if ((int)G.size() >= kMaxLobGraphs) {
+        int old = 0;
+        for (int j = 1; j < (int)G.size(); ++j)
+          if (G[j].stamp < G[old].stamp) old = j;
+        hipStreamSynchronize(c->stream);
+        if (G[old].exec) {
+          hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(G[old].exec));
+          c->graphs_live--;
+        }
+        if (G[old].graph) hipGraphDestroy(reinterpret_cast<hipGraph_t>(G[old].graph));
+        G.erase(G.begin() + old);
+      }
+      LobGraph g;
+      g.key = std::move(key);
+      g.gen = c->graph_gen;
+      g.stamp = ++c->lob_stamp;
+      G.push_back(std::move(g));
+      return enqueue();
+    }
+    G[idx].stamp = ++c->lob_stamp;
+    if (G[idx].failed) return enqueue();
+    if (!G[idx].exec) {
+      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        G[idx].failed = true;
+        return enqueue();
+      }
+      c->capturing = true;
+      const int rc = enqueue();
+      hipGraph_t gr = nullptr;
+      e = hipStreamEndCapture(c->stream, &gr);
+      c->capturing = false;
+      hipGraphExec_t x = nullptr;
+      if (rc == PW_OK && e == hipSuccess && gr) e = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+      if (rc != PW_OK || e != hipSuccess || !x) {
+        if (gr) hipGraphDestroy(gr);
+        (void)hipGetLastError();
+        c->err.clear();
+        G[idx].failed = true;
+        return enqueue();
+      }
+      G[idx].graph = gr;
+      G[idx].exec = x;
+      c->graphs_live++;
+    }
+    return hip_check(c, hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(G[idx].exec), c->stream), "LOBPCG step graph");
+  }
+
+  // Statuses of the phases since the last read: 0 everywhere, or the first failure.
+  int check_info(const int* h) {
+    for (int s = 0; s < 8 * nk; ++s) {
+      if (h[s] == 0) continue;
+      const int region = s < nk ? 0 : s < 3 * nk ? 1 : s < 5 * nk ? 2 : s < 6 * nk ? 3 : 4;
+      if (region == 0 || region == 3)
+        return set_error(c, PW_ERR_ARG, "LOBPCG batch: eigensolve failed (" + std::to_string(h[s]) + ")");
+      return set_error(c, PW_ERR_NOT_POSDEF, "LOBPCG batch: Cholesky factorisation failed (" +
+                                                 std::string(region == 1 ? "W^H W" : region == 2 ? "G" : "P^H P") +
+                                                 ", info " + std::to_string(h[s]) + ")");
+    }
+    return PW_OK;
+  }
+};
+
+}  // namespace
+
+}  // namespace pw
+
+extern "C" int pw_diag_lobpcg_batch_dev(pw_ctx* c, int nk, const int* iks, int ispin, int nst, void* const* X,
+                                        const int64_t* ldx, double tol, int niter_max, int nstates_conv,
+                                        unsigned flags, double* evals, int* stats) {
+  if (!c) return PW_ERR_ARG;
+  hipSetDevice(c->device);
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
+  if (nk < 1 || !iks || !X || !ldx || !evals || nst < 1 || niter_max < 0 || nstates_conv < 0 || nstates_conv > nst)
+    return set_error(c, PW_ERR_ARG, "pw_diag_lobpcg_batch_dev: bad arguments");
+  if (c->capturing) return set_error(c, PW_ERR_STATE, "pw_diag_lobpcg_batch_dev synchronises: not inside a capture");
+  const int N = nst, N3 = 3 * nst;
+  BatchLobpcg B;
+  B.c = c;
+  B.nk = nk;
+  B.Nst = N;
+  B.ispin = ispin;
+  B.hflags = PW_TERM_ALL | (flags & PW_MIXED);
+  B.iks.assign(iks, iks + nk);
+  B.rows.resize(nk);
+  size_t big = 0;
+  for (int i = 0; i < nk; ++i) {
+    if (iks[i] < 0 || iks[i] >= (int)c->kd.size() || c->kd[iks[i]].ngw == 0)
+      return set_error(c, PW_ERR_STATE, "k-point " + std::to_string(iks[i]) + " is not set up");
+    B.rows[i] = c->kd[iks[i]].ngw;
+    if (3 * N > B.rows[i] || ldx[i] < B.rows[i] || !X[i])
+      return set_error(c, PW_ERR_ARG, "pw_diag_lobpcg_batch_dev: need 3 Nst <= Ngw and ldx >= Ngw for every k-point");
+    big += (size_t)B.rows[i] * 10 * N;
+  }
+  if (nstates_conv == 0) nstates_conv = N;
+  if (nk > 65535) return set_error(c, PW_ERR_ARG, "pw_diag_lobpcg_batch_dev: more than 65535 k-points");
+  // workspace: the k-points' blocks; the small stacks, the lock branch's single-k matrices,
+  // eigenvalues and status words
+  int rc = ensure(c, &c->d_lobb, &c->lobb_bytes, big * sizeof(cplx<double>));
+  if (rc) return rc;
+  const size_t s1 = (size_t)N * N, s9 = (size_t)N3 * N3;
+  const size_t ncplx = (size_t)nk * (3 * s1 + 3 * s9) + 3 * s9 + 2 * s1;
+  const size_t ndbl = (size_t)nk * (N + N3) + 2 * (size_t)N3;
+  const size_t need = ncplx * sizeof(cplx<double>) + ndbl * sizeof(double) + ((size_t)8 * nk + kInfoSlots) * sizeof(int);
+  rc = ensure(c, &c->d_lobs, &c->lobs_bytes, need);
+  if (rc) return rc;
+  {
+    cplx<double>* p = reinterpret_cast<cplx<double>*>(c->d_lobb);
+    B.blk.resize(nk);
+    for (int i = 0; i < nk; ++i) {
+      B.blk[i] = p;
+      p += (size_t)B.rows[i] * 10 * N;
+    }
+    cplx<double>* s = reinterpret_cast<cplx<double>*>(c->d_lobs);
+    B.sX = s;
+    B.sX2 = B.sX + nk * s1;
+    B.sC = B.sX2 + nk * s1;
+    B.sT = B.sC + nk * s1;
+    B.sG = B.sT + nk * s9;
+    B.sM = B.sG + nk * s9;
+    B.lSm = B.sM + nk * s9;
+    B.lGm = B.lSm + s9;
+    B.lCm = B.lGm + s9;
+    B.lS2 = B.lCm + s9;
+    B.lS3 = B.lS2 + s1;
+    double* d = reinterpret_cast<double*>(B.lS3 + s1);
+    B.dwx = d;
+    B.dw = B.dwx + (size_t)nk * N;
+    B.ldw = B.dw + (size_t)nk * N3;
+    B.lE = B.ldw + N3;
+    B.info = reinterpret_cast<int*>(B.lE + N3);
+    B.linfo = B.info + 8 * nk;
+  }
+  B.use_graph = c->lob_graph && c->eig_path == 0 && N3 <= small_dense_max_n();
+  B.key0 = {N, ispin, (int)B.hflags, c->fused_batch ? 1 : 0, nk};
+  B.key0.insert(B.key0.end(), iks, iks + nk);
+  // stale captured steps (a buffer they address was reallocated) are dropped here
+  {
+    auto& G = c->lob_graphs;
+    bool stale = false;
+    for (auto& g : G) stale = stale || g.gen != c->graph_gen;
+    if (stale) {
+      hipStreamSynchronize(c->stream);
+      for (auto& g : G)
+        if (g.gen != c->graph_gen) {
+          if (g.exec) {
+            hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(g.exec));
+            c->graphs_live--;
+          }
+          if (g.graph) hipGraphDestroy(reinterpret_cast<hipGraph_t>(g.graph));
+          g.exec = g.graph = nullptr;
+        }
+      G.erase(std::remove_if(G.begin(), G.end(), [&](const LobGraph& g) { return g.gen != c->graph_gen; }), G.end());
+      if (c->graphs_live == 0) {
+        for (void* p : c->retired) hipFree(p);
+        c->retired.clear();
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3((8 * nk + kInfoSlots + 255) / 256), dim3(256), 0, c->stream, B.info,
+                     8 * nk + kInfoSlots, 0);
+  rc = hip_check(c, hipGetLastError(), "k_fill_i32");
+  if (rc) return rc;
+
+  std::vector<int> all(nk);
+  for (int i = 0; i < nk; ++i) all[i] = i;
+  std::vector<long> ncols(nk, N);
+  std::vector<int> iters(nk, 0), nlocks(nk, 0);
+  std::vector<double> ev_old((size_t)nk * N, 0.0), ev((size_t)nk * N);
+  std::vector<int> hinfo(8 * nk + kInfoSlots);
+  const double SMALL = 10.0 * (c->lobpcg_lock_exp > 0 ? std::pow(10.0, -c->lobpcg_lock_exp)
+                                                     : std::numeric_limits<double>::epsilon());
+  // X into the blocks, HX
+  rc = B.elem(all, 1, [&](int, int, int i) {
+    return eb(reinterpret_cast<const cplx<double>*>(X[i]), ldx[i], B.q(i, 0), B.rows[i], B.rows[i], N);
+  });
+  if (!rc) {
+    std::vector<void*> src(nk), dst(nk);
+    std::vector<int64_t> ld(nk);
+    for (int i = 0; i < nk; ++i) {
+      src[i] = B.q(i, 0);
+      dst[i] = B.hq(i, 0);
+      ld[i] = B.rows[i];
+    }
+    rc = pw_op_H_batch_dev(c, nk, iks, ispin, N, src.data(), ld.data(), dst.data(), ld.data(), B.hflags);
+  }
+  if (rc) return rc;
+  auto readback = [&](int ns) -> int {
+    hipError_t e = hipMemcpyAsync(ev.data(), B.dwx, (size_t)ns * N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hinfo.data(), B.info, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "LOBPCG batch read-back");
+    int r = B.check_info(hinfo.data());
+    if (r) return r;
+    for (int s = 0; s < kInfoSlots; ++s)
+      if (hinfo[8 * nk + s] != 0)
+        return set_error(c, hinfo[8 * nk + s] == -1 ? PW_ERR_HIP : PW_ERR_NOT_POSDEF,
+                         "LOBPCG batch: lock branch factorisation or eigensolve failed (slot " + std::to_string(s) +
+                             ")");
+    return PW_OK;
+  };
+  std::vector<int> active = all;
+  for (int it = 1; it <= niter_max && !active.empty(); ++it) {
+    rc = B.run(0, 0, active, [&]() { return B.check(active); });
+    if (!rc) rc = readback((int)active.size());
+    if (rc) return rc;
+    std::vector<int> keep, locked;
+    std::vector<std::vector<int>> lk_lock, lk_act;
+    for (size_t p = 0; p < active.size(); ++p) {
+      const int i = active[p];
+      iters[i] = it;
+      int nconv = 0;
+      std::vector<int> il, ia;
+      for (int b = 0; b < N; ++b) {
+        const double v = ev[p * N + b];
+        const double dl = std::fabs(v - ev_old[(size_t)i * N + b]);
+        if (dl < tol) ++nconv;
+        (dl <= SMALL ? il : ia).push_back(b);
+        ev_old[(size_t)i * N + b] = v;
+      }
+      if (nconv >= nstates_conv) continue;
+      if (!il.empty() && !ia.empty()) {
+        locked.push_back(i);
+        lk_lock.push_back(il);
+        lk_act.push_back(ia);
+      } else {
+        keep.push_back(i);
+      }
+    }
+    // the lock branch, k-point by k-point (diag_LOBPCG.jl:125-164)
+    for (size_t j = 0; j < locked.size(); ++j) {
+      const int i = locked[j];
+      rc = kprec_apply(c, iks[i], N, B.q(i, N), B.rows[i], B.q(i, N), B.rows[i]);
+      if (rc) return rc;
+      Lobpcg L;
+      L.c = c;
+      L.ik = iks[i];
+      L.ispin = ispin;
+      L.Nb = B.rows[i];
+      L.Nst = N;
+      L.prec = (flags & PW_MIXED) ? 1 : 0;
+      L.hflags = B.hflags;
+      L.Q = B.q(i, 0);
+      L.HQ = B.hq(i, 0);
+      L.Tmp = B.tmp(i, 0);
+      L.Sm = B.lSm;
+      L.Gm = B.lGm;
+      L.Cm = B.lCm;
+      L.S2 = B.lS2;
+      L.S3 = B.lS3;
+      L.dw = B.ldw;
+      L.E = B.lE;
+      L.info = B.linfo;
+      if (c->d_lob2 && c->lob2_bytes >= (size_t)L.Nb * 5 * N * sizeof(cplx<double>))
+        L.L = reinterpret_cast<cplx<double>*>(c->d_lob2);
+      rc = L.lock_step(lk_lock[j], lk_act[j]);
+      if (rc) return rc;
+      ncols[i] += L.ncols;
+      nlocks[i]++;
+    }
+    if (!keep.empty()) {
+      const int nq = it > 1 ? 3 * N : 2 * N;
+      rc = B.run(1, nq, keep, [&]() { return B.step(keep, nq); });
+      if (rc) return rc;
+      for (int i : keep) ncols[i] += N;
+    }
+    active = keep;
+    active.insert(active.end(), locked.begin(), locked.end());
+    std::sort(active.begin(), active.end());
+  }
+  // final Rayleigh-Ritz of every k-point: S = (X^H H X + (X^H H X)^H) / 2, X <- X eigvecs(S)
+  rc = B.gemm(all, 1, kOne, kZero, 1, [&](int, int p, int i) {
+    return gb(B.q(i, 0), B.rows[i], B.hq(i, 0), B.rows[i], B.sx(p), N, N, N, B.rows[i]);
+  });
+  if (!rc) rc = herm_fix_launch(c, N, nk, B.sX, N, (int64_t)N * N, 2);
+  if (!rc) rc = B.heev(N, nk, B.sX, B.dwx, N, B.inf_check());
+  if (!rc) rc = B.gemm(all, 0, kOne, kZero, 1, [&](int, int p, int i) {
+    return gb(B.q(i, 0), B.rows[i], B.sx(p), N, reinterpret_cast<cplx<double>*>(X[i]), ldx[i], B.rows[i], N, N);
+  });
+  if (!rc) rc = readback(nk);
+  if (rc) return rc;
+  std::copy(ev.begin(), ev.begin() + (size_t)nk * N, evals);
+  if (stats)
+    for (int i = 0; i < nk; ++i) {
+      stats[3 * i] = iters[i];
+      stats[3 * i + 1] = (int)std::min<long>(ncols[i], 2147483647L);
+      stats[3 * i + 2] = nlocks[i];
+    }
+  return PW_OK;
+}

@@ -146,6 +146,7 @@ __global__ void k_vps_coef(const double* __restrict__ G, int ng, const double* _
 }
 
 static int realloc_dev(pw_ctx* c, void** p, size_t bytes) {
+  c->graph_gen++;
   hipFree(*p);
   *p = nullptr;
   if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess)

@@ -449,22 +449,45 @@ function diag_LOBPCG!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}; tol=
     return evals
 end
 
-"""diag_LOBPCG!(Ham, psiks) over the k-points and spins (diag_LOBPCG.jl:1-28), each block
-iterated in its persistent device mirror."""
+"""diag_LOBPCG!(Ham, psiks) over the k-points and spins (diag_LOBPCG.jl:1-28): the blocks of
+one spin are uploaded to their persistent device mirrors and iterated together by
+pw_diag_lobpcg_batch_dev (each k-point as diag_LOBPCG! would on its own; every operation
+one batched launch over the k-points still iterating, replayed from captured hipGraphs),
+then copied back; returns the Ritz values (Nstates x Nkpt*Nspin)."""
 function diag_LOBPCG!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc; tol=1e-5, NiterMax=100,
                       verbose=false, verbose_last=false, Nstates_conv=0)
+    ctx = context(Ham)
     pw = Ham.pw
     Nkpt = pw.gvecw.kpoints.Nkpt
     Nspin = Ham.electrons.Nspin_wf
-    evals = zeros(Float64, Ham.electrons.Nstates, Nkpt * Nspin)
+    Nst = Ham.electrons.Nstates
+    evals = zeros(Float64, Nst, Nkpt * Nspin)
     m = mirror(Ham, psiks)
-    for ispin in 1:Nspin, ik in 1:Nkpt
-        Ham.ik = ik
-        Ham.ispin = ispin
-        ikspin = ik + (ispin - 1) * Nkpt
-        evals[:, ikspin] = diag_LOBPCG!(Ham, psiks[ikspin]; tol=tol, NiterMax=NiterMax, verbose=verbose,
-                                        verbose_last=verbose_last, Nstates_conv=Nstates_conv,
-                                        dpsi=m.blocks[ikspin])
+    iks = Cint.(0:Nkpt-1)
+    for ispin in 1:Nspin
+        sel = (1:Nkpt) .+ (ispin - 1) * Nkpt
+        X = Ptr{Cvoid}[m.blocks[i] for i in sel]
+        ld = Int64[size(psiks[i], 1) for i in sel]
+        for i in sel
+            check(ctx, ccall((:pw_memcpy_h2d, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{ComplexF64}, Csize_t),
+                             ctx.h, m.blocks[i], psiks[i], 16 * length(psiks[i])))
+        end
+        ev = zeros(Float64, Nst, Nkpt)
+        stats = zeros(Cint, 3, Nkpt)
+        check(ctx, ccall((:pw_diag_lobpcg_batch_dev, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Ptr{Cint}, Cint, Cint, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Cdouble, Cint, Cint,
+                          Cuint, Ptr{Float64}, Ptr{Cint}),
+                         ctx.h, Nkpt, iks, ispin - 1, Nst, X, ld, tol, NiterMax, Nstates_conv,
+                         ctx.mixed ? PW_MIXED : UInt32(0), ev, stats))
+        for i in sel
+            check(ctx, ccall((:pw_memcpy_d2h, libpw), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Cvoid}, Csize_t),
+                             ctx.h, psiks[i], m.blocks[i], 16 * length(psiks[i])))
+        end
+        evals[:, sel] = ev
+        if verbose || verbose_last
+            println("diag_LOBPCG (device, batched): spin ", ispin, ", iterations per k-point ",
+                    extrema(stats[1, :]), ", op_H columns ", sum(stats[2, :]))
+        end
     end
     return evals
 end

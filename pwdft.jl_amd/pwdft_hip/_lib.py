@@ -26,6 +26,7 @@ PW_TUNE_PLANE_COMPRESSED = 5
 PW_TUNE_EIG_PATH = 9
 PW_TUNE_LOBPCG_LOCK = 10
 PW_TUNE_FUSED_BATCH = 11
+PW_TUNE_LOBPCG_GRAPH = 12
 
 PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
 
@@ -70,6 +71,7 @@ _PROTOS = {
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_diag_lobpcg_dev": [_vp, _i, _i, _i, _vp, _i64, _d, _i, _i, _c.c_uint, _vp, _vp],
+    "pw_diag_lobpcg_batch_dev": [_vp, _i, _vp, _i, _i, _vp, _vp, _d, _i, _i, _c.c_uint, _vp, _vp],
     "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pw_set_atoms": [_vp, _i, _vp, _vp, _i, _vp],
     "pw_set_rho_sphere": [_vp, _i, _vp, _i, _vp, _i, _vp],

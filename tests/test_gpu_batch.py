@@ -287,6 +287,85 @@ def test_lobpcg_batch_lock_branch(al):
         assert np.abs(X.conj().T @ X - np.eye(8)).max() < 1e-10
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_lobpcg_batch_graph_replay_vs_eager(al, mixed):
+    """The batched LOBPCG's phases replayed from captured hipGraphs (within a call, and from
+    graphs kept by the context across calls) give the eager run's Ritz values and vectors
+    bit for bit."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import PW_TUNE_LOBPCG_GRAPH
+    H, _ = al
+    iks = list(range(0, 182, 3))
+    runs = []
+    for graph, reps in ((0, 1), (1, 3)):
+        H.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_GRAPH, graph)
+        for rep in range(reps):
+            print(f"case Al LOBPCG batch graph={graph} rep={rep} mixed={mixed}", flush=True)
+            ps = ph.rand_BlochWavefunc(H, seed=12)
+            sub = [ps[k] for k in iks]
+            ev, st = ph.diag_LOBPCG_batch(H, sub, tol=1e-9, NiterMax=40, iks=iks, mixed=mixed)
+            runs.append((ev, [m.download() for m in sub], st))
+    H.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_GRAPH, 1)
+    ev0, X0, st0 = runs[0]
+    assert max(x["iter"] for x in st0) > 3
+    for ev, X, st in runs[1:]:
+        assert np.array_equal(ev, ev0)
+        assert st == st0
+        for a, b in zip(X, X0):
+            assert np.array_equal(a, b)
+
+
+def test_descriptor_cache_beyond_its_slots():
+    """More distinct batched-GEMM descriptor arrays than the cache has slots (256), first
+    with no captured graph (slots are reused after a wait) and then while a captured graph
+    lives (slots are added, none the graph reads is rewritten): every product equals numpy's,
+    and the graph's replay still computes its own product."""
+    import pwdft_hip as ph
+    from pwdft_hip._lib import ptr
+    ctx = ph.Context(0)
+    rng = np.random.default_rng(17)
+    m, n, k = 6, 5, 40
+    A = rand_psi(rng, k, 320 * m)       # 320 distinct column offsets -> 320 distinct arrays
+    B = rand_psi(rng, k, n)
+    dA, dB = ph.DeviceMatrix.from_host(ctx, A), ph.DeviceMatrix.from_host(ctx, B)
+    dC = ph.DeviceMatrix(ctx, m, n)
+    one, zero = np.array([1.0, 0.0]), np.array([0.0, 0.0])
+    i32 = lambda v: np.ascontiguousarray([v], dtype=np.int32)
+    i64 = lambda v: np.ascontiguousarray([v], dtype=np.int64)
+
+    def product(j):
+        pa = np.ascontiguousarray([dA.p.value + 16 * k * m * j], dtype=np.uint64)
+        pb = np.ascontiguousarray([dB.p.value], dtype=np.uint64)
+        pc = np.ascontiguousarray([dC.p.value], dtype=np.uint64)
+        ctx.call("pw_zgemm_batch_dev", 1, 1, ptr(i32(m)), ptr(i32(n)), ptr(i32(k)), ptr(one), ptr(pa), ptr(i64(k)),
+                 ptr(pb), ptr(i64(k)), ptr(zero), ptr(pc), ptr(i64(m)))
+
+    def ref(j):
+        return A[:, j * m:(j + 1) * m].conj().T @ B
+
+    print("case descriptor cache without a graph", flush=True)
+    for j in range(300):
+        product(j)
+        if j % 37 == 0 or j >= 290:
+            assert np.abs(dC.download() - ref(j)).max() < 1e-12 * np.abs(ref(j)).max(), j
+    print("case descriptor cache with a live graph", flush=True)
+    product(7)                          # its descriptor is cached before the capture
+    with ph.Graph(ctx) as g:
+        product(7)
+    for j in range(300, 320):
+        product(j)
+        assert np.abs(dC.download() - ref(j)).max() < 1e-12 * np.abs(ref(j)).max(), j
+    for j in range(0, 300, 7):
+        product(j)
+    ctx.call("pw_synchronize")
+    dC.upload(np.zeros(m * n, dtype=complex))
+    g.launch()
+    ctx.call("pw_synchronize")
+    assert np.abs(dC.download() - ref(7)).max() < 1e-12 * np.abs(ref(7)).max()
+    g.free()
+    ctx.close()
+
+
 def test_scf_batched_vs_per_k():
     """Al fcc 4x4x4 shifted (10 k-points, Fermi-Dirac smearing 0.01 Ha): the SCF with the
     batched eigensolver converges to the per-k SCF's free energy."""
