@@ -315,6 +315,16 @@ int pw_zgemm_batch_dev(pw_ctx* ctx, int transa, int nbatch, const int* m, const 
 int pw_diag_lobpcg_dev(pw_ctx* ctx, int ik, int ispin, int nst, void* X, int64_t ldx, double tol, int niter_max,
                        int nstates_conv, unsigned flags, double* evals, int* stats);
 
+/* diag_davidson!(Ham, X) for k-point ik and spin ispin (diag_davidson.jl:50-191, called per k
+ * from diag_davidson!(Ham, psiks), :12-38, which KS_solve_SCF! calls when the solver is
+ * Davidson): X (device, Ngw x nst, leading dimension ldx, orthonormal columns) is
+ * overwritten with the Ritz vectors of the last 2 nst search space, evals (host, nst) with
+ * the Ritz values.  tol, niter_max, nstates_conv and flags as pw_diag_lobpcg_dev.  stats
+ * (host, 3 ints, may be NULL): iterations, op_H columns applied, 0.  Requires 2 nst <= Ngw.
+ * Synchronises once per iteration. */
+int pw_diag_davidson_dev(pw_ctx* ctx, int ik, int ispin, int nst, void* X, int64_t ldx, double tol, int niter_max,
+                         int nstates_conv, unsigned flags, double* evals, int* stats);
+
 /* diag_LOBPCG!(Ham, psiks) over the nk k-points iks at once (diag_LOBPCG.jl:1-28 running
  * :51-225 for every k-point, KS_solve_SCF.jl:189-191): X[i] (device, Ngw[iks[i]] x nst,
  * leading dimension ldx[i], orthonormal columns) is overwritten with the Ritz vectors of

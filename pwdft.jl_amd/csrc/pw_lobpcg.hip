@@ -320,6 +320,205 @@ extern "C" int pw_diag_lobpcg_dev(pw_ctx* c, int ik, int ispin, int nst, void* X
   return PW_OK;
 }
 
+// ------------------------------------------------------------------ diag_davidson!
+//
+// pw_diag_davidson_dev: diag_davidson! (diag_davidson.jl:50-191) for one k-point and spin on
+// the device.  Q = [X R] and HQ = [HX HR] are the 2 Nst search space and its H-image; the
+// reduced matrices Hred, Sred (2 Nst square) are assembled, factorised and solved in HBM
+// (the same generalised solve as the Rayleigh-Ritz step of LOBPCG: G = U^H U, U^-H T U^-1),
+// and the host reads the Nst lowest Ritz values once per iteration for the |d lambda| test.
+// The step order is the reference's: R = X diag(evals) - HX, each column scaled by 1/|R_j|
+// when |R_j| > eps, R = Kprec(R), HR = H R, Hred = [X^H HX or diag(evals), X^H HR; ., R^H HR],
+// Sred = [I, X^H R; ., R^H R], both made Hermitian as 0.5 (A + A^H), eigen(Hred, Sred),
+// X = Q y, HX = HQ y for the Nst lowest.
+
+namespace pw {
+namespace {
+
+constexpr int DAV_NT = 256;
+
+// dw[j] = Re S[j, j] (the initial Rayleigh quotients; S has leading dimension lds)
+__global__ void k_diag_re(const cplx<double>* __restrict__ S, long lds, int n, double* __restrict__ dw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) dw[j] = S[(long)j * lds + j].x;
+}
+
+// One workgroup per column j: R_j = ev_j X_j - HX_j, then R_j /= |R_j| when eps < |R_j|
+// (diag_davidson.jl:82-89 and :102-112).
+__global__ void __launch_bounds__(DAV_NT) k_dav_resid(const cplx<double>* __restrict__ X, const cplx<double>* __restrict__ HX,
+                                                      const double* __restrict__ ev, cplx<double>* __restrict__ R,
+                                                      long ld, int nb) {
+  __shared__ double red[DAV_NT / 64];
+  const int j = blockIdx.x;
+  const double e = ev[j];
+  const cplx<double>* x = X + (long)j * ld;
+  const cplx<double>* h = HX + (long)j * ld;
+  cplx<double>* r = R + (long)j * ld;
+  double s = 0.0;
+  for (int g = threadIdx.x; g < nb; g += DAV_NT) {
+    const cplx<double> v = mk(e * x[g].x - h[g].x, e * x[g].y - h[g].y);
+    r[g] = v;
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double tot = 0.0;
+  for (int q = 0; q < DAV_NT / 64; ++q) tot += red[q];
+  const double res = sqrt(tot);
+  if (!(2.220446049250313e-16 < res)) return;
+  const double f = 1.0 / res;
+  for (int g = threadIdx.x; g < nb; g += DAV_NT) r[g] = scal(f, r[g]);
+}
+
+// Hred and Sred (order 2 n, leading dimension 2 n) from the products already in their
+// upper block column [., n:2n) (and, in the first iteration, Hred's leading n x n block):
+// the leading blocks set (Hred: first ? X^H HX : diag(ev); Sred: I), the lower triangle
+// mirrored from the upper one and every entry averaged as 0.5 (A + A^H).  One thread per
+// pair i <= j writes both entries.
+__global__ void k_dav_assemble(cplx<double>* __restrict__ H, cplx<double>* __restrict__ S, int n,
+                               const double* __restrict__ ev, int first) {
+  const int m = 2 * n;
+  const long tot = (long)m * m;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % m), j = (int)(t / m);
+    if (i > j) continue;
+    cplx<double> h, s;
+    if (j < n) {  // leading block
+      if (first) {
+        const cplx<double> u = H[(long)j * m + i], l = H[(long)i * m + j];
+        h = mk(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+      } else {
+        h = mk(i == j ? ev[i] : 0.0, 0.0);
+      }
+      s = mk(i == j ? 1.0 : 0.0, 0.0);
+    } else if (i < n) {  // upper-right block: X^H HR, X^H R (the mirror averages to itself)
+      h = H[(long)j * m + i];
+      s = S[(long)j * m + i];
+    } else {  // trailing block: R^H HR, R^H R
+      const cplx<double> hu = H[(long)j * m + i], hl = H[(long)i * m + j];
+      const cplx<double> su = S[(long)j * m + i], sl = S[(long)i * m + j];
+      h = mk(0.5 * (hu.x + hl.x), 0.5 * (hu.y - hl.y));
+      s = mk(0.5 * (su.x + sl.x), 0.5 * (su.y - sl.y));
+    }
+    if (i == j) {
+      h.y = 0.0;
+      s.y = 0.0;
+    }
+    H[(long)j * m + i] = h;
+    S[(long)j * m + i] = s;
+    H[(long)i * m + j] = mk(h.x, -h.y);
+    S[(long)i * m + j] = mk(s.x, -s.y);
+  }
+}
+
+}  // namespace
+}  // namespace pw
+
+extern "C" int pw_diag_davidson_dev(pw_ctx* c, int ik, int ispin, int nst, void* X, int64_t ldx, double tol,
+                                    int niter_max, int nstates_conv, unsigned flags, double* evals, int* stats) {
+  if (!c) return PW_ERR_ARG;
+  hipSetDevice(c->device);
+  if (c->N == 0) return set_error(c, PW_ERR_STATE, "pw_set_grid has not been called");
+  if (ik < 0 || ik >= (int)c->kd.size() || c->kd[ik].ngw == 0)
+    return set_error(c, PW_ERR_STATE, "k-point " + std::to_string(ik) + " is not set up");
+  const int Nb = c->kd[ik].ngw;
+  if (nst < 1 || !X || ldx < Nb || !evals || niter_max < 0 || nstates_conv < 0 || nstates_conv > nst || 2 * nst > Nb)
+    return set_error(c, PW_ERR_ARG, "pw_diag_davidson_dev: bad arguments (need 1 <= Nst, 2 Nst <= Ngw)");
+  if (c->capturing) return set_error(c, PW_ERR_STATE, "pw_diag_davidson_dev synchronises: not inside a graph capture");
+  if (nstates_conv == 0) nstates_conv = nst;
+  const int Nst = nst, n2 = 2 * nst;
+  // workspace: Q, HQ, Tmp (Nb x 2 Nst each), Hred, Sred, Cm (n2^2), S2 (Nst^2), dw, E (n2
+  // doubles), status words
+  const size_t big = (size_t)Nb * 6 * Nst, small = 3 * (size_t)n2 * n2 + (size_t)Nst * Nst;
+  const size_t need = (big + small) * sizeof(cplx<double>) + 2 * (size_t)n2 * sizeof(double) + kInfoSlots * sizeof(int);
+  int rc = ensure(c, &c->d_lob, &c->lob_bytes, need);
+  if (rc) return rc;
+  Lobpcg L;
+  L.c = c;
+  L.ik = ik;
+  L.ispin = ispin;
+  L.Nb = Nb;
+  L.Nst = Nst;
+  L.prec = (flags & PW_MIXED) ? 1 : 0;
+  L.hflags = PW_TERM_ALL | (flags & PW_MIXED);
+  L.Q = reinterpret_cast<cplx<double>*>(c->d_lob);
+  L.HQ = L.Q + (size_t)Nb * n2;
+  L.Tmp = L.HQ + (size_t)Nb * n2;
+  L.Sm = L.Tmp + (size_t)Nb * n2;   // Hred
+  L.Gm = L.Sm + (size_t)n2 * n2;    // Sred
+  L.Cm = L.Gm + (size_t)n2 * n2;
+  L.S2 = L.Cm + (size_t)n2 * n2;
+  L.S3 = nullptr;
+  L.dw = reinterpret_cast<double*>(L.S2 + (size_t)Nst * Nst);
+  L.E = L.dw + n2;
+  L.info = reinterpret_cast<int*>(L.E + n2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, L.info, kInfoSlots, 0);
+  rc = hip_check(c, hipGetLastError(), "k_fill_i32");
+  if (rc) return rc;
+  cplx<double>* x = L.q(0);
+  cplx<double>* r = L.q(Nst);
+  cplx<double>* hx = L.hq(0);
+  cplx<double>* hr = L.hq(Nst);
+  const int prec = L.prec;
+  // HX, evals = Re diag(X^H HX), R = X diag(evals) - HX (scaled)
+  rc = copy_cols(c, Nb, Nst, reinterpret_cast<const cplx<double>*>(X), ldx, x, Nb);
+  if (!rc) rc = L.opH(x, hx, Nst);
+  if (!rc) rc = L.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, L.S2, Nst, prec);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_diag_re, dim3((Nst + 255) / 256), dim3(256), 0, c->stream, L.S2, (long)Nst, Nst, L.dw);
+  hipLaunchKernelGGL(k_dav_resid, dim3(Nst), dim3(DAV_NT), 0, c->stream, x, hx, L.dw, r, (long)Nb, Nb);
+  rc = hip_check(c, hipGetLastError(), "Davidson residual");
+  std::vector<double> ev(Nst), ev_old(Nst);
+  std::vector<int> hinfo(kInfoSlots);
+  if (!rc) rc = read_iteration(c, L.dw, Nst, L.info, ev_old.data(), hinfo.data());
+  if (rc) return rc;
+  ev = ev_old;
+  int it_done = 0;
+  const int nblk = (int)std::min<long>(((long)n2 * n2 + 255) / 256, 1024);
+  for (int it = 1; it <= niter_max; ++it) {
+    it_done = it;
+    rc = kprec_apply(c, ik, Nst, r, Nb, r, Nb);
+    if (!rc) rc = L.opH(r, hr, Nst);
+    // upper block column of Hred (X^H HR; R^H HR) and Sred (X^H R; R^H R); Sred in FP64
+    if (!rc) rc = L.gemm(1, n2, Nst, Nb, L.Q, Nb, hr, Nb, L.Sm + (long)n2 * Nst, n2, prec);
+    if (!rc) rc = L.gemm(1, n2, Nst, Nb, L.Q, Nb, r, Nb, L.Gm + (long)n2 * Nst, n2, 0);
+    if (!rc && it == 1) rc = L.gemm(1, Nst, Nst, Nb, x, Nb, hx, Nb, L.Sm, n2, prec);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dav_assemble, dim3(nblk), dim3(256), 0, c->stream, L.Sm, L.Gm, Nst, L.dw, it == 1 ? 1 : 0);
+    rc = hip_check(c, hipGetLastError(), "k_dav_assemble");
+    if (!rc) rc = L.rr_solve(n2, Nst, 2);
+    // X = Q y, HX = HQ y
+    if (!rc) rc = L.gemm(0, Nb, Nst, n2, L.Q, Nb, L.Cm, n2, L.tmp(0), Nb, prec);
+    if (!rc) rc = L.gemm(0, Nb, Nst, n2, L.HQ, Nb, L.Cm, n2, L.tmp(Nst), Nb, prec);
+    if (!rc) rc = L.copy(Nst, L.tmp(0), x);
+    if (!rc) rc = L.copy(Nst, L.tmp(Nst), hx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dav_resid, dim3(Nst), dim3(DAV_NT), 0, c->stream, x, hx, L.dw, r, (long)Nb, Nb);
+    rc = hip_check(c, hipGetLastError(), "Davidson residual");
+    if (!rc) rc = read_iteration(c, L.dw, Nst, L.info, ev.data(), hinfo.data());
+    if (rc) return rc;
+    int nconv = 0;
+    for (int i = 0; i < Nst; ++i)
+      if (std::fabs(ev[i] - ev_old[i]) < tol) ++nconv;
+    if (nconv >= nstates_conv) break;
+    ev_old = ev;
+  }
+  rc = copy_cols(c, Nb, Nst, x, Nb, reinterpret_cast<cplx<double>*>(X), ldx);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_check(c, e, "Davidson final copy");
+  }
+  if (rc) return rc;
+  std::copy(ev.begin(), ev.end(), evals);
+  if (stats) {
+    stats[0] = it_done;
+    stats[1] = (int)std::min<long>(L.ncols, 2147483647L);
+    stats[2] = 0;
+  }
+  return PW_OK;
+}
+
 // ------------------------------------------------------------------ all k-points at once
 //
 // pw_diag_lobpcg_batch_dev runs the iteration above for many k-points in lockstep.  Every

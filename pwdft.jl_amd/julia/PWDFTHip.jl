@@ -9,10 +9,11 @@ route op_H / op_K / op_V_loc / op_V_Ps_nloc / calc_rhoe! (with symmetrize_rhoe!)
 update! (Poisson, VWN or PBE, V_Total) through the C ABI.  The op_H family also runs on the
 device for norm-conserving `Hamiltonian{PsPot_UPF}` (single grid, collinear), whose
 nonlocal term couples each atom's projectors through its dense Deeq block.  The reference's own
-KS_solve_SCF! and diag_davidson! then run unchanged on top of them (they call op_H,
-calc_rhoe! and update!).  diag_LOBPCG! itself runs in the library (pw_diag_lobpcg_dev): each
-k-point's block is uploaded once into its device mirror, every iteration's op_H, Gram
-products and small dense problems stay on the device, and the Ritz vectors come back once.  A HIP context is attached to each Hamiltonian the first
+KS_solve_SCF! then runs unchanged on top of them (it calls op_H, calc_rhoe! and update!).
+diag_LOBPCG! and diag_davidson! themselves run in the library (pw_diag_lobpcg_batch_dev,
+pw_diag_lobpcg_dev, pw_diag_davidson_dev): each k-point's block is uploaded once into its
+device mirror, every iteration's op_H, Gram products and small dense problems stay on the
+device, and the Ritz vectors come back once.  A HIP context is attached to each Hamiltonian the first
 time it is used, keyed by object identity; the device update! installs V_Total as the op_H
 potential itself, and `sync_potential!(Ham)` re-uploads Ham.potentials.Total for code that
 writes it directly.
@@ -25,7 +26,7 @@ module PWDFTHip
 using PWDFT
 import PWDFT: op_H, op_K, op_V_loc, op_V_Ps_nloc, op_H!, op_K!, op_V_loc!, op_V_Ps_nloc!,
               calc_rhoe!, update!, calc_forces_Ps_loc, calc_forces_Ps_nloc, Hamiltonian, PsPot_GTH,
-              PsPot_UPF, BlochWavefunc, diag_LOBPCG!
+              PsPot_UPF, BlochWavefunc, diag_LOBPCG!, diag_davidson!
 
 # Hamiltonians whose op_H family runs on the device: GTH, and norm-conserving UPF (the
 # nonlocal term then couples each atom's projectors through its dense Deeq block)
@@ -488,6 +489,59 @@ function diag_LOBPCG!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc; tol=1e-
             println("diag_LOBPCG (device, batched): spin ", ispin, ", iterations per k-point ",
                     extrema(stats[1, :]), ", op_H columns ", sum(stats[2, :]))
         end
+    end
+    return evals
+end
+
+"""diag_davidson!(Ham, psi) for the current Ham.ik / Ham.ispin, on the device
+(diag_davidson.jl:50-191 through pw_diag_davidson_dev): psi is overwritten with the Ritz
+vectors; returns the Ritz values."""
+function diag_davidson!(Ham::Hamiltonian{PsPot_GTH}, psi::Matrix{ComplexF64}; tol=1e-5, NiterMax=100,
+                        verbose=false, verbose_last=false, Nstates_conv=0, dpsi::Ptr{Cvoid}=C_NULL)
+    ctx = context(Ham)
+    Ngw, Nst = size(psi)
+    own = dpsi == C_NULL
+    if own
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ctx, ccall((:pw_malloc, libpw), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}), ctx.h, 16 * Ngw * Nst, r))
+        dpsi = r[]
+    end
+    evals = zeros(Float64, Nst)
+    stats = zeros(Cint, 3)
+    try
+        check(ctx, ccall((:pw_memcpy_h2d, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{ComplexF64}, Csize_t),
+                         ctx.h, dpsi, psi, 16 * Ngw * Nst))
+        check(ctx, ccall((:pw_diag_davidson_dev, libpw), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Int64, Cdouble, Cint, Cint, Cuint, Ptr{Float64},
+                          Ptr{Cint}),
+                         ctx.h, Ham.ik - 1, Ham.ispin - 1, Nst, dpsi, Ngw, tol, NiterMax, Nstates_conv,
+                         ctx.mixed ? PW_MIXED : UInt32(0), evals, stats))
+        check(ctx, ccall((:pw_memcpy_d2h, libpw), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Cvoid}, Csize_t),
+                         ctx.h, psi, dpsi, 16 * Ngw * Nst))
+    finally
+        own && ccall((:pw_free, libpw), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, dpsi)
+    end
+    if verbose || verbose_last
+        println("diag_davidson (device): ", stats[1], " iterations, ", stats[2], " op_H columns")
+    end
+    return evals
+end
+
+"""diag_davidson!(Ham, psiks) over the k-points and spins (diag_davidson.jl:12-38): each block
+iterated on the device in its persistent device mirror; returns the Ritz values
+(Nstates x Nkpt*Nspin)."""
+function diag_davidson!(Ham::Hamiltonian{PsPot_GTH}, psiks::BlochWavefunc; tol=1e-5, NiterMax=100,
+                        verbose=false, verbose_last=false, Nstates_conv=0)
+    Nkpt = Ham.pw.gvecw.kpoints.Nkpt
+    Nspin = Ham.electrons.Nspin_wf
+    evals = zeros(Float64, Ham.electrons.Nstates, Nkpt * Nspin)
+    m = mirror(Ham, psiks)
+    for ispin in 1:Nspin, ik in 1:Nkpt
+        Ham.ik = ik
+        Ham.ispin = ispin
+        i = ik + (ispin - 1) * Nkpt
+        evals[:, i] = diag_davidson!(Ham, psiks[i]; tol=tol, NiterMax=NiterMax, verbose=verbose,
+                                     verbose_last=verbose_last, Nstates_conv=Nstates_conv, dpsi=m.blocks[i])
     end
     return evals
 end

@@ -71,6 +71,7 @@ _PROTOS = {
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_diag_lobpcg_dev": [_vp, _i, _i, _i, _vp, _i64, _d, _i, _i, _c.c_uint, _vp, _vp],
+    "pw_diag_davidson_dev": [_vp, _i, _i, _i, _vp, _i64, _d, _i, _i, _c.c_uint, _vp, _vp],
     "pw_diag_lobpcg_batch_dev": [_vp, _i, _vp, _i, _i, _vp, _vp, _d, _i, _i, _c.c_uint, _vp, _vp],
     "pw_zgemm_batch_dev": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pw_set_atoms": [_vp, _i, _vp, _vp, _i, _vp],

@@ -317,10 +317,32 @@ def _lobpcg_lock_step(D, Q, HQ, Tmp, Nb, Nst, ilock, iact, scratch=None):
     return 2 * na + 2 * Nst
 
 
-def diag_davidson_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None):
+def _davidson_device(Ham, X, tol, NiterMax, Nstates_conv, mixed):
+    """diag_davidson! through the C ABI (pw_diag_davidson_dev): the search space, its
+    H-image, the reduced matrices and their generalised eigenproblem stay in HBM; the host
+    receives the Ritz values once per iteration for the |dlambda| test."""
+    Nst = X.cols
+    ev = np.zeros(Nst)
+    st = np.zeros(3, dtype=np.int32)
+    Ham.ctx.call("pw_diag_davidson_dev", Ham.ik, Ham.ispin, Nst, X.p, X.rows, float(tol), int(NiterMax),
+                 int(Nstates_conv), PW_MIXED if mixed else 0, ev.ctypes.data_as(ctypes.c_void_p),
+                 st.ctypes.data_as(ctypes.c_void_p))
+    return ev, {"iter": int(st[0]), "ncols": int(st[1])}
+
+
+def diag_davidson_(Ham, X, tol=1e-5, NiterMax=100, Nstates_conv=0, mixed=None, dense=None):
     """diag_davidson! for the current Ham.ik (diag_davidson.jl:50-191).  Q = [X R] and
-    HQ = [HX HR] give the reduced matrices of the 2Nst search space in two products."""
+    HQ = [HX HR] give the reduced matrices of the 2Nst search space in two products.
+    dense = "device" (default) runs the whole solver in the library
+    (pw_diag_davidson_dev); "host" solves the reduced eigenproblem with LAPACK on the host
+    as the reference does, the Ngw-sized products staying on the device."""
     mixed = Ham.mixed if mixed is None else mixed
+    if (DENSE if dense is None else dense) == "device":
+        if Nstates_conv == 0:
+            Nstates_conv = X.cols
+        if mixed:
+            tol = max(tol, MIXED_TOL_FLOOR)
+        return _davidson_device(Ham, X, tol, NiterMax, Nstates_conv, mixed)
     D = _Dense(Ham, mixed)
     Nb, Nst = X.rows, X.cols
     if Nstates_conv == 0:

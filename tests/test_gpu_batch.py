@@ -330,15 +330,17 @@ def test_descriptor_cache_beyond_its_slots():
     dA, dB = ph.DeviceMatrix.from_host(ctx, A), ph.DeviceMatrix.from_host(ctx, B)
     dC = ph.DeviceMatrix(ctx, m, n)
     one, zero = np.array([1.0, 0.0]), np.array([0.0, 0.0])
-    i32 = lambda v: np.ascontiguousarray([v], dtype=np.int32)
-    i64 = lambda v: np.ascontiguousarray([v], dtype=np.int64)
+    # the size arrays stay referenced for the whole call (a temporary's buffer may be
+    # freed, and reused, before the library reads it)
+    mm, nn, kk = (np.array([v], dtype=np.int32) for v in (m, n, k))
+    lda, ldc = np.array([k], dtype=np.int64), np.array([m], dtype=np.int64)
 
     def product(j):
         pa = np.ascontiguousarray([dA.p.value + 16 * k * m * j], dtype=np.uint64)
         pb = np.ascontiguousarray([dB.p.value], dtype=np.uint64)
         pc = np.ascontiguousarray([dC.p.value], dtype=np.uint64)
-        ctx.call("pw_zgemm_batch_dev", 1, 1, ptr(i32(m)), ptr(i32(n)), ptr(i32(k)), ptr(one), ptr(pa), ptr(i64(k)),
-                 ptr(pb), ptr(i64(k)), ptr(zero), ptr(pc), ptr(i64(m)))
+        ctx.call("pw_zgemm_batch_dev", 1, 1, ptr(mm), ptr(nn), ptr(kk), ptr(one), ptr(pa), ptr(lda),
+                 ptr(pb), ptr(lda), ptr(zero), ptr(pc), ptr(ldc))
 
     def ref(j):
         return A[:, j * m:(j + 1) * m].conj().T @ B

@@ -33,9 +33,13 @@ SHIFT8 = np.array([[0.041, -0.027, 0.033, -0.012, 0.018, -0.045, 0.006, 0.029],
 
 
 def _oracle(symbols, cart, L, psp, ecut, **kw):
+    """Oracle Hamiltonian from Cartesian positions given 3 x Natoms (the reference's layout);
+    the oracle's Atoms takes one row per atom."""
     from oracle.energy import Hamiltonian
     from oracle.setup import Atoms, PsPot_GTH
-    return Hamiltonian(Atoms(symbols, cart, L, frac=False), [PsPot_GTH(os.path.join(PSP, psp))], ecut, **kw)
+    pos = np.asarray(cart, dtype=float)
+    assert pos.shape == (3, len(symbols))
+    return Hamiltonian(Atoms(symbols, pos.T, L, frac=False), [PsPot_GTH(os.path.join(PSP, psp))], ecut, **kw)
 
 
 # ----------------------------------------------------------------------------- Si64
