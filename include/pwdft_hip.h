@@ -124,6 +124,18 @@ int pw_op_H(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldp
 /* Same on device pointers. */
 int pw_op_H_dev(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldpsi, void* Hpsi,
                 int64_t ldh, unsigned flags);
+/* pw_op_H_dev with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP stream capture, serving loops that forbid
+ * hipMalloc).  Every workspace (the FFT work array, beta^H psi, the projector products'
+ * split-K partial sums) lives in one fixed 64 MiB device array per GPU, held until the
+ * process ends; the bands go through the FFT
+ * passes in as many launches as that array needs.  Rejected with PW_ERR_ARG: a k-point
+ * whose Nbeta x nb block of beta^H psi exceeds 8 MiB, or one of whose bands needs more
+ * than 48 MiB of FFT work array.  Results equal pw_op_H_dev's to rounding (the split of
+ * the projector products' K may differ).  Calls on one GPU share the array: calls that
+ * overlap in time on one GPU (several contexts or threads) race. */
+int pw_op_H_dev_noalloc(pw_ctx* ctx, int ik, int ispin, int nb, const void* psi, int64_t ldpsi, void* Hpsi,
+                        int64_t ldh, unsigned flags);
 
 /* op_H over several k-points in one launch per stage (the k loop of op_H(Ham, psiks) for a
  * BlochWavefunc, op_H.jl:29-50): k-point iks[i] applies to psi[i] (Ngw[iks[i]] x nb, leading

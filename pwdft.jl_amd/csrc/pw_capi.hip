@@ -24,6 +24,9 @@ int hip_check(pw_ctx* c, hipError_t e, const char* what) {
 
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need) {
   if (need <= *have && *buf) return PW_OK;
+  if (c->no_alloc)
+    return set_error(c, PW_ERR_ARG, "a workspace would grow inside a no-allocation call (" + std::to_string(need) +
+                                        " bytes needed, " + std::to_string(*have) + " available)");
   if (c->capturing)
     return set_error(c, PW_ERR_STATE, "a workspace would grow during graph capture; run the sequence once first");
   if (*buf) {
@@ -290,6 +293,10 @@ int pw_ctx_create(pw_ctx** out, int device) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
+    return PW_ERR_HIP;
+  }
+  if (noalloc_init(c) != PW_OK) {
+    pw_ctx_destroy(c);
     return PW_ERR_HIP;
   }
   *out = c;

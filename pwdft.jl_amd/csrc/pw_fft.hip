@@ -986,7 +986,9 @@ static int vloc_run(pw_ctx* c, KData& k, const T* V, int nb, const cplx<double>*
   }
   if (!lanes_ok(c->dim[0]) || !lanes_ok(c->dim[1]) || !lanes_ok(c->dim[2]))
     return set_error(c, PW_ERR_ARG, "FFT line too long for one wavefront");
-  const int bb = std::max(1, std::min(nb, c->band_batch));
+  int bb = std::max(1, std::min(nb, c->band_batch));
+  // a no-allocation call takes as many bands per launch as its fixed work array holds
+  if (c->no_alloc) bb = std::max(1, std::min<int>(bb, (int)(c->work_bytes / ((size_t)c->n3 * k.ncol * esz))));
   const size_t wbytes = (size_t)bb * c->n3 * k.ncol * esz;
   int rc = ensure(c, &c->d_work, &c->work_bytes, wbytes);
   if (rc) return rc;

@@ -430,6 +430,8 @@ static int zgemm_launch(pw_ctx* c, int transa, int M, int N, int K, cplx<double>
   if (tiles < target_wg && ksteps > 8) {
     nz = (int)std::min<long>((target_wg + tiles - 1) / tiles, ksteps / 8);
     nz = std::max(8, (nz + 7) / 8 * 8);  // whole rounds over the 8 XCDs
+    // a no-allocation call splits K only as far as its fixed partial-sum array holds
+    if (c->no_alloc) nz = std::max<int>(1, std::min<long>(nz, (long)(c->gpart_bytes / ((size_t)M * N * 16))));
   }
   int kchunk = GBK;
   if (K > 0) {

@@ -232,6 +232,8 @@ struct pw_ctx {
   // stream capture into hipGraphs (pw_graph_*): while capturing, nothing may allocate,
   // upload or synchronise
   bool capturing = false;
+  // inside pw_op_H_dev_noalloc: workspaces point into the fixed array and must not grow
+  bool no_alloc = false;
   std::vector<void*> graphs;       // hipGraphExec_t per id (nullptr once freed)
   std::vector<void*> graph_defs;   // hipGraph_t per id
   int graphs_live = 0;
@@ -286,6 +288,9 @@ int set_error(pw_ctx* c, int code, const std::string& msg);
 void free_lob_graphs(pw_ctx* c);
 int hip_check(pw_ctx* c, hipError_t e, const char* what);
 int ensure(pw_ctx* c, void** buf, size_t* have, size_t need);
+// Looks up the device address of the no-allocation workspace (pw_noalloc.hip) for the
+// context's device, once per device; called by pw_ctx_create with that device current.
+int noalloc_init(pw_ctx* c);
 
 // Hpsi of the bands psi at k-point ik, spin ispin, into hpsi (terms and precision by flags)
 // on the context stream (pw_capi.hip).

@@ -10,7 +10,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
-SOURCES = ["pw_capi.hip", "pw_fft.hip", "pw_gemm.hip", "pw_forces.hip", "pw_batch.hip", "pw_dense.hip", "pw_scf.hip", "pw_proj.hip", "pw_eig.hip", "pw_lobpcg.hip", "pw_small.hip"]
+SOURCES = ["pw_capi.hip", "pw_fft.hip", "pw_gemm.hip", "pw_forces.hip", "pw_batch.hip", "pw_dense.hip", "pw_scf.hip", "pw_proj.hip", "pw_eig.hip", "pw_lobpcg.hip", "pw_small.hip", "pw_noalloc.hip"]
 OUT = os.path.join(HERE, "libpwdft_hip.so")
 
 HIPCC_FLAGS = [

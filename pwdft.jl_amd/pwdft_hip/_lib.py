@@ -50,6 +50,7 @@ _PROTOS = {
     "pw_set_projectors": [_vp, _i, _i, _vp, _i64],
     "pw_op_H": [_vp, _i, _i, _i, _vp, _i64, _vp, _i64, _c.c_uint],
     "pw_op_H_dev": [_vp, _i, _i, _i, _vp, _i64, _vp, _i64, _c.c_uint],
+    "pw_op_H_dev_noalloc": [_vp, _i, _i, _i, _vp, _i64, _vp, _i64, _c.c_uint],
     "pw_kprec_dev": [_vp, _i, _i, _vp, _i64, _vp, _i64],
     "pw_betaNL_psi_dev": [_vp, _i, _i, _vp, _i64, _vp, _i64, _c.c_uint],
     "pw_rhoe_zero": [_vp],
