@@ -89,9 +89,12 @@ enum {
                                    per (k-point, band) holding the band's whole grid in LDS,
                                    wherever the grid fits (20^3 in FP64); 0: the three-stage
                                    column / plane / column launches and batched GEMMs */
-  PW_TUNE_LOBPCG_GRAPH = 12     /* pw_diag_lobpcg_batch_dev: 1 (default) replay each step from
+  PW_TUNE_LOBPCG_GRAPH = 12,    /* pw_diag_lobpcg_batch_dev: 1 (default) replay each step from
                                    a captured hipGraph once it has run eagerly (orders 3 Nst <=
                                    64); 0: every step enqueued eagerly */
+  PW_TUNE_FUSED_NT = 13         /* threads per workgroup of the fused batched op_H: 256, 512 or
+                                   1024; 0 (default): the workgroups one CU's LDS holds share
+                                   1024 threads */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

@@ -13,8 +13,9 @@
 //     owner of column i + 1 updates that column first and publishes v_{i+1} before it
 //     updates the rest of its columns (look-ahead).
 //  2. The eigenpairs of T through rocSOLVER's divide and conquer (ZSTEDC).
-//  3. k_apply_q: eigenvectors of A = Q (eigenvectors of T), the reflectors applied to the
-//     wanted columns only, two columns per wavefront, staged in LDS.
+//  3. k_larft + k_apply_q_wy: eigenvectors of A = Q (eigenvectors of T), the reflectors
+//     applied by blocks of 32 in compact WY form (I - V T V^H) to the wanted columns only,
+//     four columns per workgroup staged in LDS.
 //
 // Only the upper triangle of A is read, as Julia's Hermitian(A) does.  Cross-workgroup
 // waits are bounded: a wait that exceeds its spin limit sets a status word, every other
@@ -39,11 +40,14 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Thread 0: wait until *ctr >= target (acquire); false once the status word is set or the
-// spin limit is reached (then it sets the status word itself).
+// Thread 0: wait until *ctr >= target; false once the status word is set or the spin limit
+// is reached (then it sets the status word itself).  The counter is polled with relaxed
+// loads and a single acquire fence follows the successful poll: an acquire load in the
+// loop would invalidate the caches on every poll, slowing the workgroup that is about to
+// publish the next value.
 __device__ inline bool wait_geq(unsigned* ctr, unsigned target, int* status) {
   long spins = 0;
-  while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+  while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
     if (++spins > HT_SPIN) {
       __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -51,6 +55,7 @@ __device__ inline bool wait_geq(unsigned* ctr, unsigned target, int* status) {
     }
     __builtin_amdgcn_s_sleep(1);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
@@ -104,9 +109,11 @@ __device__ void hetrd_reflector(const cplx<double>* col, int i, int n, cplx<doub
     ee[i] = beta;
     dd[i] = col[i].x;
   }
+  // every thread's stores are made visible at agent scope before the barrier, so the flag
+  // that follows it needs no release of its own
   __threadfence();
   __syncthreads();
-  if (tid == 0) __hip_atomic_store(&sync[0], (unsigned)(i + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) __hip_atomic_store(&sync[0], (unsigned)(i + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A22 -= v w^H + w v^H on local column lc (global column j), rows i+1..n-1.
@@ -215,7 +222,7 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
     }
     __threadfence();
     __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // D: gather p, form w, update
     if (tid == 0) sh.ok = wait_geq(&sync[1], (unsigned)P * (unsigned)(i + 1), status) ? 1 : 0;
     __syncthreads();
@@ -249,46 +256,124 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
   if ((n - 1) % P == w && tid == 0) dd[n - 1] = Acol[(long)((n - 1) / P) * n + (n - 1)].x;
 }
 
-// C[:, c] <- Q C[:, c] = H(0) (H(1) (... H(n-2) C[:, c])) for the columns c < ncols; each
-// wavefront owns two columns, staged in LDS.
-__global__ void __launch_bounds__(HT_NT) k_apply_q(const cplx<double>* __restrict__ V, long ldv,
-                                                   const cplx<double>* __restrict__ tau, int n,
-                                                   cplx<double>* __restrict__ C, long ldc, int ncols) {
-  extern __shared__ char smem_raw[];
-  constexpr int CPW = 2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  cplx<double>* S = reinterpret_cast<cplx<double>*>(smem_raw) + (long)wave * CPW * n;
-  const int c0 = (blockIdx.x * (HT_NT / 64) + wave) * CPW;
-  if (c0 >= ncols) return;
-  const int nc = min(CPW, ncols - c0);
-  for (int q = 0; q < nc; ++q)
-    for (int r = lane; r < n; r += 64) S[q * n + r] = C[(long)(c0 + q) * ldc + r];
-  for (int i = n - 2; i >= 0; --i) {
-    const cplx<double> t = tau[i];
-    if (t.x == 0.0 && t.y == 0.0) continue;
-    const cplx<double>* v = V + (long)i * ldv;
-    double sr[CPW] = {0.0, 0.0}, si[CPW] = {0.0, 0.0};
-    for (int r = i + 1 + lane; r < n; r += 64) {
-      const cplx<double> vr = v[r];
-      for (int q = 0; q < CPW; ++q) {
-        if (q < nc) {
-          const cplx<double> cr = S[q * n + r];
-          // conj(v) * c
-          sr[q] += vr.x * cr.x + vr.y * cr.y;
-          si[q] += vr.x * cr.y - vr.y * cr.x;
-        }
-      }
+// Back-transform by blocks of WY_NB reflectors (compact WY form, LAPACK ZLARFT / ZLARFB,
+// forward, column-wise): H(i0) H(i0+1) ... H(i0+nr-1) = I - V_b T_b V_b^H with T_b upper
+// triangular, T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] (V_b[:, 0:j]^H v_j).  Rows of
+// v_i above i + 1 are not stored: they are zero, and v_i[i + 1] = 1.
+constexpr int WY_NB = 32;
+constexpr int WY_NT_T = 1024;   // k_larft: one workgroup per block of reflectors
+constexpr int WY_CW = 4;        // k_apply_q_wy: columns of C per workgroup
+
+__global__ void __launch_bounds__(WY_NT_T) k_larft(const cplx<double>* __restrict__ V, long ldv,
+                                                  const cplx<double>* __restrict__ tau, int n,
+                                                  cplx<double>* __restrict__ Tout) {
+  __shared__ cplx<double> S[WY_NB][WY_NB + 1];
+  __shared__ cplx<double> T[WY_NB][WY_NB + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = WY_NT_T / 64;
+  const int i0 = blockIdx.x * WY_NB;
+  const int nr = min(WY_NB, n - 1 - i0);
+  // S[l][j] = v_l^H v_j for l < j (rows from i0 + j + 1, where v_j starts with its 1)
+  for (int p = wave; p < nr * nr; p += nwave) {
+    const int l = p % nr, j = p / nr;
+    if (l >= j) continue;
+    const cplx<double>* vl = V + (long)(i0 + l) * ldv;
+    const cplx<double>* vj = V + (long)(i0 + j) * ldv;
+    const int r0 = i0 + j + 1;
+    double sr = 0.0, si = 0.0;
+    for (int r = r0 + lane; r < n; r += 64) {
+      const cplx<double> a = vl[r];
+      const cplx<double> b = r == r0 ? mk(1.0, 0.0) : vj[r];
+      sr += a.x * b.x + a.y * b.y;
+      si += a.x * b.y - a.y * b.x;
     }
-    cplx<double> f[CPW];
-    for (int q = 0; q < CPW; ++q) f[q] = cmul(t, mk(wave_sum(sr[q]), wave_sum(si[q])));
-    for (int r = i + 1 + lane; r < n; r += 64) {
-      const cplx<double> vr = v[r];
-      for (int q = 0; q < CPW; ++q)
-        if (q < nc) S[q * n + r] = S[q * n + r] - cmul(vr, f[q]);
-    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) S[l][j] = mk(sr, si);
   }
-  for (int q = 0; q < nc; ++q)
-    for (int r = lane; r < n; r += 64) C[(long)(c0 + q) * ldc + r] = S[q * n + r];
+  for (int t = tid; t < WY_NB * (WY_NB + 1); t += WY_NT_T) T[t / (WY_NB + 1)][t % (WY_NB + 1)] = mk(0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < nr; ++j) {
+    const cplx<double> tj = tau[i0 + j];
+    if (tid < j) {
+      cplx<double> acc = mk(0.0, 0.0);
+      for (int l = tid; l < j; ++l) acc = acc + cmul(T[tid][l], S[l][j]);
+      T[tid][j] = mk(-(tj.x * acc.x - tj.y * acc.y), -(tj.x * acc.y + tj.y * acc.x));
+    }
+    if (tid == 0) T[j][j] = tj;
+    __syncthreads();
+  }
+  cplx<double>* to = Tout + (long)blockIdx.x * WY_NB * WY_NB;
+  for (int t = tid; t < WY_NB * WY_NB; t += WY_NT_T) to[t] = T[t % WY_NB][t / WY_NB];
+}
+
+// C[:, c] <- Q C[:, c] for the columns c < ncols, Q = B_0 B_1 ... (B_b = I - V_b T_b V_b^H):
+// the blocks are applied last to first to WY_CW columns per workgroup staged in LDS.
+__global__ void __launch_bounds__(HT_NT) k_apply_q_wy(const cplx<double>* __restrict__ V, long ldv,
+                                                     const cplx<double>* __restrict__ Tb, int n,
+                                                     cplx<double>* __restrict__ C, long ldc, int ncols) {
+  extern __shared__ char smem_raw[];
+  cplx<double>* Cs = reinterpret_cast<cplx<double>*>(smem_raw);   // [WY_CW][n]
+  __shared__ cplx<double> Ts[WY_NB][WY_NB + 1];
+  __shared__ cplx<double> W[WY_NB][WY_CW], W2[WY_NB][WY_CW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = HT_NT / 64;
+  const int c0 = blockIdx.x * WY_CW;
+  const int nc = min(WY_CW, ncols - c0);
+  for (int t = tid; t < WY_CW * n; t += HT_NT) {
+    const int q = t / n, r = t % n;
+    Cs[t] = q < nc ? C[(long)(c0 + q) * ldc + r] : mk(0.0, 0.0);
+  }
+  const int nblk = (n - 1 + WY_NB - 1) / WY_NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int i0 = b * WY_NB;
+    const int nr = min(WY_NB, n - 1 - i0);
+    for (int t = tid; t < WY_NB * WY_NB; t += HT_NT) Ts[t % WY_NB][t / WY_NB] = Tb[(long)b * WY_NB * WY_NB + t];
+    __syncthreads();
+    // W = V_b^H C
+    for (int p = wave; p < nr * WY_CW; p += nwave) {
+      const int l = p % nr, q = p / nr;
+      const cplx<double>* vl = V + (long)(i0 + l) * ldv;
+      const cplx<double>* cq = Cs + (long)q * n;
+      const int r0 = i0 + l + 1;
+      double sr = 0.0, si = 0.0;
+      for (int r = r0 + lane; r < n; r += 64) {
+        const cplx<double> a = r == r0 ? mk(1.0, 0.0) : vl[r];
+        const cplx<double> x = cq[r];
+        sr += a.x * x.x + a.y * x.y;
+        si += a.x * x.y - a.y * x.x;
+      }
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      if (lane == 0) W[l][q] = mk(sr, si);
+    }
+    __syncthreads();
+    // W2 = T_b W
+    for (int t = tid; t < nr * WY_CW; t += HT_NT) {
+      const int k = t % nr, q = t / nr;
+      cplx<double> acc = mk(0.0, 0.0);
+      for (int l = k; l < nr; ++l) acc = acc + cmul(Ts[k][l], W[l][q]);
+      W2[k][q] = acc;
+    }
+    __syncthreads();
+    // C -= V_b W2 (rows from i0 + 1; v_l contributes from row i0 + l + 1)
+    for (int r = i0 + 1 + tid; r < n; r += HT_NT) {
+      cplx<double> acc[WY_CW];
+#pragma unroll
+      for (int q = 0; q < WY_CW; ++q) acc[q] = mk(0.0, 0.0);
+      const int lmax = min(nr - 1, r - i0 - 1);
+      for (int l = 0; l <= lmax; ++l) {
+        const cplx<double> a = r == i0 + l + 1 ? mk(1.0, 0.0) : V[(long)(i0 + l) * ldv + r];
+#pragma unroll
+        for (int q = 0; q < WY_CW; ++q) acc[q] = acc[q] + cmul(a, W2[l][q]);
+      }
+#pragma unroll
+      for (int q = 0; q < WY_CW; ++q) Cs[(long)q * n + r] = Cs[(long)q * n + r] - acc[q];
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < nc * n; t += HT_NT) {
+    const int q = t / n, r = t % n;
+    C[(long)(c0 + q) * ldc + r] = Cs[t];
+  }
 }
 
 __global__ void k_merge_status(int* info, const int* status) {
@@ -345,12 +430,16 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   if (P <= 0) return set_error(c, PW_ERR_ARG, "heevd_dist: order " + std::to_string(n) + " out of range");
   // workspace: V (n x n), tau (n), pg (2n) complex; e (n), sp (4P) doubles; sync (2) + status
   const size_t vbytes = (size_t)n * n * sizeof(cplx<double>);
-  const size_t need = vbytes + 3 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
+  const int nblk = (n - 1 + WY_NB - 1) / WY_NB;
+  const size_t tbytes = (size_t)nblk * WY_NB * WY_NB * sizeof(cplx<double>);
+  const size_t need =
+      vbytes + tbytes + 3 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
   int rc = ensure(c, &c->d_eig, &c->eig_bytes, need);
   if (rc) return rc;
   char* base = reinterpret_cast<char*>(c->d_eig);
   cplx<double>* V = reinterpret_cast<cplx<double>*>(base);
-  cplx<double>* tau = V + (size_t)n * n;
+  cplx<double>* Tb = V + (size_t)n * n;
+  cplx<double>* tau = Tb + (size_t)nblk * WY_NB * WY_NB;
   cplx<double>* pg = tau + n;
   double* E = reinterpret_cast<double*>(pg + 2 * (size_t)n);
   double* sp = E + n;
@@ -369,17 +458,21 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   if (e != hipSuccess) return hip_check(c, e, "k_hetrd_dist (cooperative launch)");
   // the timeout status of the reduction is folded into d_info below
   rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);
-  rocblas_status st = rocsolver_zstedc(h, nvec > 0 ? rocblas_evect_tridiagonal : rocblas_evect_none, n, d_w, E,
+  // ZSTEDC with the tridiagonal's eigenvectors even when only values are wanted: without
+  // them rocSOLVER runs its single-thread QL iteration (ZSTERF), which costs ~13 ms at n = 160
+  rocblas_status st = rocsolver_zstedc(h, rocblas_evect_tridiagonal, n, d_w, E,
                                        reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_info);
   if (st != rocblas_status_success)
     return set_error(c, PW_ERR_HIP, "zstedc: rocSOLVER status " + std::to_string((int)st));
   if (nvec > 0) {
-    const int waves = (nvec + 1) / 2;
-    const int blocks = (waves + HT_NT / 64 - 1) / (HT_NT / 64);
-    const size_t sh = (size_t)(HT_NT / 64) * 2 * n * sizeof(cplx<double>);
-    hipLaunchKernelGGL(k_apply_q, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, tau, n, A, lda, nvec);
+    hipLaunchKernelGGL(k_larft, dim3(nblk), dim3(WY_NT_T), 0, c->stream, V, ldv, tau, n, Tb);
     e = hipGetLastError();
-    if (e != hipSuccess) return hip_check(c, e, "k_apply_q");
+    if (e != hipSuccess) return hip_check(c, e, "k_larft");
+    const int blocks = (nvec + WY_CW - 1) / WY_CW;
+    const size_t sh = (size_t)WY_CW * n * sizeof(cplx<double>);
+    hipLaunchKernelGGL(k_apply_q_wy, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, Tb, n, A, lda, nvec);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(c, e, "k_apply_q_wy");
   }
   // d_info = -1 when the reduction timed out (one tiny kernel keeps it on the stream)
   hipLaunchKernelGGL(k_merge_status, dim3(1), dim3(1), 0, c->stream, d_info, status);

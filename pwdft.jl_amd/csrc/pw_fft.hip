@@ -1145,7 +1145,6 @@ struct FusedK {
   int ngw, ncol, nxact, nbeta;
 };
 
-constexpr int FUSED_NT = 256;
 
 template <typename T>
 __device__ inline void fused_rows_vloc(cplx<T>* lds, int nrows, int ps, const FftDim& dx, const cplx<T>* tx,
@@ -1162,8 +1161,8 @@ __device__ inline void fused_rows_vloc(cplx<T>* lds, int nrows, int ps, const Ff
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(FUSED_NT)
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
 k_op_H_fused(const FusedK* __restrict__ kd, int nb, const T* __restrict__ V, FftDim dx, FftDim dy, FftDim dz,
              const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz,
              int do_v, int do_k, int do_nl, int acc) {
@@ -1294,10 +1293,27 @@ static int fused_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, cons
   const cplx<T>* tw[3];
   for (int d = 0; d < 3; ++d)
     tw[d] = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[d] : c->d_tw32[d]);
-  hipLaunchKernelGGL(k_op_H_fused<T>, dim3((unsigned)(nk * nb)), dim3(FUSED_NT), lds, c->stream,
-                     reinterpret_cast<const FusedK*>(dptr), nb, V, c->dim[0], c->dim[1], c->dim[2], tw[0], tw[1],
-                     tw[2], (terms & PW_TERM_VLOC) ? 1 : 0, (terms & PW_TERM_K) ? 1 : 0, (terms & PW_TERM_NLOC) ? 1 : 0,
-                     (flags & PW_ACCUMULATE) ? 1 : 0);
+  // Threads per workgroup: the workgroups that fit a CU's LDS together hold 1024 threads
+  // (FP64 20^3: one 134 KB box per CU, 16 wavefronts share its line FFTs), unless the
+  // tuning knob fixes the width.
+  int nt = c->fused_nt;
+  if (nt != 256 && nt != 512 && nt != 1024) {
+    const int per_cu = std::max<int>(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
+    nt = per_cu >= 4 ? 256 : per_cu >= 2 ? 512 : 1024;
+  }
+  const dim3 grid((unsigned)(nk * nb));
+  const FusedK* dk = reinterpret_cast<const FusedK*>(dptr);
+  const int dv = (terms & PW_TERM_VLOC) ? 1 : 0, dkin = (terms & PW_TERM_K) ? 1 : 0;
+  const int dnl = (terms & PW_TERM_NLOC) ? 1 : 0, acc = (flags & PW_ACCUMULATE) ? 1 : 0;
+  if (nt == 1024)
+    hipLaunchKernelGGL((k_op_H_fused<T, 1024>), grid, dim3(1024), lds, c->stream, dk, nb, V, c->dim[0], c->dim[1],
+                       c->dim[2], tw[0], tw[1], tw[2], dv, dkin, dnl, acc);
+  else if (nt == 512)
+    hipLaunchKernelGGL((k_op_H_fused<T, 512>), grid, dim3(512), lds, c->stream, dk, nb, V, c->dim[0], c->dim[1],
+                       c->dim[2], tw[0], tw[1], tw[2], dv, dkin, dnl, acc);
+  else
+    hipLaunchKernelGGL((k_op_H_fused<T, 256>), grid, dim3(256), lds, c->stream, dk, nb, V, c->dim[0], c->dim[1],
+                       c->dim[2], tw[0], tw[1], tw[2], dv, dkin, dnl, acc);
   return hip_check(c, hipGetLastError(), "k_op_H_fused");
 }
 

@@ -256,6 +256,7 @@ struct pw_ctx {
   // a captured step whose generation differs is discarded, never replayed
   unsigned graph_gen = 0;
   bool fused_batch = true;                            // batched op_H: one fused launch where the box fits LDS
+  int fused_nt = 0;                                   // threads per fused workgroup (0: by LDS, 256/512/1024)
   // density side of an SCF iteration (pw_scf.hip)
   double* d_vps = nullptr;                            // V_Ps_loc(r) [N]
   void* d_sgrid = nullptr; size_t sgrid_bytes = 0;    // grid fields of the potential update
