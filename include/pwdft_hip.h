@@ -92,9 +92,14 @@ enum {
   PW_TUNE_LOBPCG_GRAPH = 12,    /* pw_diag_lobpcg_batch_dev: 1 (default) replay each step from
                                    a captured hipGraph once it has run eagerly (orders 3 Nst <=
                                    64); 0: every step enqueued eagerly */
-  PW_TUNE_FUSED_NT = 13         /* threads per workgroup of the fused batched op_H: 256, 512 or
+  PW_TUNE_FUSED_NT = 13,        /* threads per workgroup of the fused batched op_H: 256, 512 or
                                    1024; 0 (default): the workgroups one CU's LDS holds share
                                    1024 threads */
+  PW_TUNE_EIG_SYNC = 14         /* the device tridiagonal reduction's cross-workgroup exchanges:
+                                   1 (default) device-coherent atomic accesses, each producer
+                                   waiting for its stores before the flag, no cache flushes
+                                   (n = 480: 10.8 ms, n = 960: 21.4 ms); 0 plain accesses
+                                   ordered by agent-scope fences (12.2, 33.4 ms) */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

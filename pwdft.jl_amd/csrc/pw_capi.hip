@@ -384,6 +384,10 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
     case PW_TUNE_FUSED_BATCH:
       c->fused_batch = value != 0;
       return PW_OK;
+    case PW_TUNE_EIG_SYNC:
+      if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver sync mode must be 0..1");
+      c->eig_light = value;
+      return PW_OK;
     case PW_TUNE_FUSED_NT:
       if (value != 0 && value != 256 && value != 512 && value != 1024)
         return set_error(c, PW_ERR_ARG, "pw_tune_set: fused workgroup width must be 0, 256, 512 or 1024");

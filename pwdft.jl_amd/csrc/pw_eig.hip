@@ -19,7 +19,10 @@
 //
 // Only the upper triangle of A is read, as Julia's Hermitian(A) does.  Cross-workgroup
 // waits are bounded: a wait that exceeds its spin limit sets a status word, every other
-// wait then ends at once, and the host reports the failure.
+// wait then ends at once, and the host reports the failure.  A step costs ~17 us at
+// n = 480 (26 workgroups): about eight round trips to device-coherent memory on its
+// critical path (flag, v_i, p segments, partial sums, the next reflector), which bounds
+// this one-dimensional distribution.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -42,9 +45,8 @@ __device__ inline double wave_sum(double v) {
 
 // Thread 0: wait until *ctr >= target; false once the status word is set or the spin limit
 // is reached (then it sets the status word itself).  The counter is polled with relaxed
-// loads and a single acquire fence follows the successful poll: an acquire load in the
-// loop would invalidate the caches on every poll, slowing the workgroup that is about to
-// publish the next value.
+// loads (an acquire load would invalidate the caches on every poll); every thread orders its
+// later reads after the barrier that follows (xacquire).
 __device__ inline bool wait_geq(unsigned* ctr, unsigned target, int* status) {
   long spins = 0;
   while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -55,8 +57,51 @@ __device__ inline bool wait_geq(unsigned* ctr, unsigned target, int* status) {
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
+}
+
+// Values exchanged between the workgroups of the reduction (v_i, tau_i, the p segments and
+// partial sums).  With LIGHT they are written and read as relaxed agent-scope atomics
+// (device-coherent accesses that bypass the non-coherent caches), and a producer waits for
+// its own stores to complete (s_waitcnt) before the barrier that precedes the flag store, so
+// no L2 write-back or invalidation is needed; otherwise plain accesses ordered by agent-scope
+// fences.
+template <bool LIGHT>
+__device__ inline void xstore(cplx<double>* p, cplx<double> v) {
+  if (LIGHT) {
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    *p = v;
+  }
+}
+template <bool LIGHT>
+__device__ inline void xstore(double* p, double v) {
+  if (LIGHT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+template <bool LIGHT>
+__device__ inline cplx<double> xload(const cplx<double>* p) {
+  if (LIGHT)
+    return mk(__hip_atomic_load(&p->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+              __hip_atomic_load(&p->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  return *p;
+}
+template <bool LIGHT>
+__device__ inline double xload(const double* p) {
+  if (LIGHT) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return *p;
+}
+// Every thread, after its exchanged stores and before the barrier that precedes a flag.
+template <bool LIGHT>
+__device__ inline void xpublish() {
+  if (LIGHT) __builtin_amdgcn_s_waitcnt(0);
+  else __threadfence();
+}
+// Every thread, after the barrier that follows a successful wait.
+template <bool LIGHT>
+__device__ inline void xacquire() {
+  if (!LIGHT) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Block-wide sum (every thread gets the result); red holds HT_NT / 64 doubles.
@@ -79,6 +124,7 @@ struct HetrdShared {
 // Householder vector of column i (owner only; column in LDS, rows i+1..n-1 below the
 // diagonal): ZLARFG on (alpha = A[i+1, i], x = A[i+2:n, i]).  Writes v_i (v[i+1] = 1) to
 // V[:, i], tau_i, e_i = beta, d_i = Re A[i, i], then publishes sync[0] = i + 1.
+template <bool LIGHT>
 __device__ void hetrd_reflector(const cplx<double>* col, int i, int n, cplx<double>* V, long ldv,
                                 cplx<double>* tau, double* dd, double* ee, unsigned* sync, double* red) {
   const int tid = threadIdx.x;
@@ -103,15 +149,15 @@ __device__ void hetrd_reflector(const cplx<double>* col, int i, int n, cplx<doub
     scale = mk(ar / den, -ai / den);
   }
   cplx<double>* v = V + (long)i * ldv;
-  for (int r = i + 1 + tid; r < n; r += HT_NT) v[r] = (r == i + 1) ? mk(1.0, 0.0) : cmul(col[r], scale);
+  for (int r = i + 1 + tid; r < n; r += HT_NT) xstore<LIGHT>(v + r, (r == i + 1) ? mk(1.0, 0.0) : cmul(col[r], scale));
   if (tid == 0) {
-    tau[i] = mk(tr, ti);
+    xstore<LIGHT>(tau + i, mk(tr, ti));
     ee[i] = beta;
     dd[i] = col[i].x;
   }
   // every thread's stores are made visible at agent scope before the barrier, so the flag
   // that follows it needs no release of its own
-  __threadfence();
+  xpublish<LIGHT>();
   __syncthreads();
   if (tid == 0) __hip_atomic_store(&sync[0], (unsigned)(i + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -129,6 +175,7 @@ __device__ inline void hetrd_update_col(cplx<double>* col, int j, int i, int n, 
 
 // sync[0]: index + 1 of the last published reflector; sync[1]: arrivals of the p gather.
 // pg: 2 x n published p (by step parity), sp: 2 x P x 2 partial sums s = sum conj(p_j) v_j.
+template <bool LIGHT>
 __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __restrict__ A, long lda, int n,
                                                       int ncl_max, cplx<double>* __restrict__ V, long ldv,
                                                       cplx<double>* __restrict__ tau, double* __restrict__ dd,
@@ -161,7 +208,7 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
     Acol[(long)lc * n + r] = a;
   }
   __syncthreads();
-  if (w == 0) hetrd_reflector(Acol, 0, n, V, ldv, tau, dd, ee, sync, red);
+  if (w == 0) hetrd_reflector<LIGHT>(Acol, 0, n, V, ldv, tau, dd, ee, sync, red);
 
   for (int i = 0; i <= n - 2; ++i) {
     const int par = i & 1;
@@ -169,10 +216,10 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
     if (tid == 0) sh.ok = wait_geq(&sync[0], (unsigned)(i + 1), status) ? 1 : 0;
     __syncthreads();
     if (!sh.ok) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int r = i + 1 + tid; r < n; r += HT_NT) vb[r] = V[(long)i * ldv + r];
+    xacquire<LIGHT>();
+    for (int r = i + 1 + tid; r < n; r += HT_NT) vb[r] = xload<LIGHT>(V + (long)i * ldv + r);
     if (tid == 0) {
-      const cplx<double> t = tau[i];
+      const cplx<double> t = xload<LIGHT>(tau + i);
       sh.tr = t.x;
       sh.ti = t.y;
     }
@@ -195,7 +242,7 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
       ai = wave_sum(ai);
       const cplx<double> p = cmul(taui, mk(ar, ai));
       if (lane == 0) {
-        pg[(long)par * n + j] = p;
+        xstore<LIGHT>(pg + (long)par * n + j, p);
         const cplx<double> v = vb[j];
         // conj(p) * v
         spr += p.x * v.x + p.y * v.y;
@@ -216,35 +263,35 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
           a += red[q];
           b += red2[q];
         }
-        sp[((long)par * P + w) * 2 + 0] = a;
-        sp[((long)par * P + w) * 2 + 1] = b;
+        xstore<LIGHT>(sp + ((long)par * P + w) * 2 + 0, a);
+        xstore<LIGHT>(sp + ((long)par * P + w) * 2 + 1, b);
       }
     }
-    __threadfence();
+    xpublish<LIGHT>();
     __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // D: gather p, form w, update
     if (tid == 0) sh.ok = wait_geq(&sync[1], (unsigned)P * (unsigned)(i + 1), status) ? 1 : 0;
     __syncthreads();
     if (!sh.ok) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    xacquire<LIGHT>();
     double sr = 0.0, si = 0.0;
     for (int q = tid; q < P; q += HT_NT) {
-      sr += sp[((long)par * P + q) * 2 + 0];
-      si += sp[((long)par * P + q) * 2 + 1];
+      sr += xload<LIGHT>(sp + ((long)par * P + q) * 2 + 0);
+      si += xload<LIGHT>(sp + ((long)par * P + q) * 2 + 1);
     }
     sr = block_sum(sr, red);
     si = block_sum(si, red);
     // alpha2 = -1/2 tau s
     const cplx<double> alpha2 = cmul(mk(-0.5 * taui.x, -0.5 * taui.y), mk(sr, si));
-    for (int r = i + 1 + tid; r < n; r += HT_NT) wb[r] = pg[(long)par * n + r] + cmul(alpha2, vb[r]);
+    for (int r = i + 1 + tid; r < n; r += HT_NT) wb[r] = xload<LIGHT>(pg + (long)par * n + r) + cmul(alpha2, vb[r]);
     __syncthreads();
     int done = -1;
     if (i + 1 <= n - 2 && (i + 1) % P == w) {
       const int lc = (i + 1) / P;
       hetrd_update_col(Acol + (long)lc * n, i + 1, i, n, vb, wb);
       __syncthreads();
-      hetrd_reflector(Acol + (long)lc * n, i + 1, n, V, ldv, tau, dd, ee, sync, red);
+      hetrd_reflector<LIGHT>(Acol + (long)lc * n, i + 1, n, V, ldv, tau, dd, ee, sync, red);
       done = lc;
     }
     for (int lc = lc0; lc < ncl; ++lc) {
@@ -454,7 +501,8 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   double* dd = d_w;
   void* args[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
                   (void*)&dd, (void*)&E, (void*)&pg, (void*)&sp, (void*)&sync, (void*)&status};
-  e = hipLaunchCooperativeKernel((const void*)k_hetrd_dist, dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
+  const void* kern = c->eig_light ? (const void*)k_hetrd_dist<true> : (const void*)k_hetrd_dist<false>;
+  e = hipLaunchCooperativeKernel(kern, dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "k_hetrd_dist (cooperative launch)");
   // the timeout status of the reduction is folded into d_info below
   rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);

@@ -9,6 +9,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 PW_TUNE_EIG_PATH = 9
+PW_TUNE_EIG_SYNC = 14
 
 
 @pytest.fixture(scope="module")
@@ -78,3 +79,24 @@ def test_heevd_values_only_and_rocsolver_path(ctx):
     wv = np.zeros(480)
     ctx.call("pw_heev_values_dev", 480, dA.p, 480, wv.ctypes.data_as(ctypes.c_void_p))
     assert np.abs(wv - w1).max() < 1e-11 * np.abs(w1).max()
+
+
+@pytest.mark.parametrize("n", [33, 160, 480, 960])
+def test_heevd_fenced_exchanges(ctx, n):
+    """The reduction with its cross-workgroup exchanges ordered by agent-scope fences
+    (PW_TUNE_EIG_SYNC = 0) against numpy and against the default device-coherent exchanges."""
+    rng = np.random.default_rng(n + 7)
+    A = _herm(rng, n, "random")
+    print(f"case heevd n={n} fenced exchanges", flush=True)
+    ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, 0)
+    try:
+        w, X = _eig(ctx, A)
+    finally:
+        ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, 1)
+    w0, _ = _eig(ctx, A)
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(w - ref).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+    assert np.abs(w - w0).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+    assert np.abs(X.conj().T @ X - np.eye(n)).max() < 1e-12 * max(1.0, np.sqrt(n) / 4)
+    assert np.abs(A @ X - X * w).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
