@@ -421,6 +421,10 @@ def main():
                     help="also run the Si216 PBE leg (108^3, 480 bands; adds about a minute)")
     ap.add_argument("--no-forces", dest="forces", action="store_false",
                     help="skip the Si8 mixed-vs-FP64 force leg")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="run the library's default launch parameters instead of the autotuner's "
+                         "(pwdft_hip/autotune.py: FFT radix mode, plane kernel, column chunk, band "
+                         "split, GEMM tiles, second stream; tuned once per precision before timing)")
     ap.add_argument("--no-overlap", dest="overlap", action="store_false",
                     help="run op_H's nonlocal products on the main stream (kernels in sequence), "
                          "e.g. under a kernel-trace profiler")
@@ -441,18 +445,37 @@ def main():
         H.ctx.call("pw_synchronize")
         a.free()
         b.free()
+    # launch parameters per precision: the autotuner's (timed here, outside every timed
+    # region, or read from its cache), else the library's defaults with all bands per launch
+    from pwdft_hip import autotune as at
+    tuned = {}
+    for mixed in (False, True):
+        tag = "mixed" if mixed else "fp64"
+        if args.autotune:
+            t0 = time.perf_counter()
+            settings, source = at.autotune(H, nbands=NBANDS, mixed=mixed)
+            tuned[tag] = dict(settings, source=source, seconds=round(time.perf_counter() - t0, 1))
+            print(f"# autotune {tag} ({source}): {json.dumps(settings)}", file=sys.stderr, flush=True)
+        else:
+            tuned[tag] = dict(at.DEFAULTS, band_batch=NBANDS)
+
+    def use(mixed):
+        at.apply_settings(H.ctx, tuned["mixed" if mixed else "fp64"])
+        if not args.overlap:
+            H.ctx.call("pw_tune_set", 6, 0)
+
     # accuracy of the mixed path: band energies <psi|H|psi>
     res = {}
     from pwdft_hip.solvers import _Dense
     for mixed in (False, True):
-        # headline: nonlocal products on the second stream beside the FFT passes, no
-        # per-kernel events; then the per-kernel durations for the roofline with the
-        # kernels in sequence on one stream (overlapped kernels share the CUs)
-        H.ctx.call("pw_tune_set", 6, 1 if args.overlap else 0)
+        # headline: the tuned launch parameters, no per-kernel events; then the per-kernel
+        # durations for the roofline with the kernels in sequence on one stream (overlapped
+        # kernels share the CUs)
+        use(mixed)
         sec, wall, _ = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=False)
         H.ctx.call("pw_tune_set", 6, 0)
         _, _, prof = time_op_H(H, X, Y, args.steps, args.warmup, mixed, profile=True)
-        H.ctx.call("pw_tune_set", 6, 1 if args.overlap else 0)
+        use(mixed)
         D = _Dense(H, False)
         e = np.real(np.diag(D.gram(X.p, Y.p, NBANDS, NBANDS, X.rows)))
         res["mixed" if mixed else "fp64"] = {"sec": sec, "wall": wall, "eband": e, "prof": prof}
@@ -465,6 +488,7 @@ def main():
     rhoe = {}
     for mixed in (False, True):
         tag = "mixed" if mixed else "fp64"
+        use(mixed)
         rhoe[tag] = time_rhoe(H, X, args.steps, args.warmup, mixed)
         print(f"# calc_rhoe {tag}: {rhoe[tag]['ms_per_step']:.3f} ms per 160 bands", file=sys.stderr, flush=True)
     Natoms = H.atoms.Natoms
@@ -477,6 +501,7 @@ def main():
     if args.solver_iters > 0:
         for mixed in (False, True):
             tag = "mixed" if mixed else "fp64"
+            use(mixed)
             lob[tag] = time_lobpcg(H, X, args.solver_iters, mixed)
             print(f"# LOBPCG {tag}: {lob[tag]['s_per_iter'] * 1e3:.2f} ms/iteration, "
                   f"{lob[tag]['band_apps_per_s']:.0f} band-applications/s", file=sys.stderr, flush=True)
@@ -484,6 +509,7 @@ def main():
     if args.scf_iters > 0:
         for mixed in (False, True):
             tag = "mixed" if mixed else "fp64"
+            use(mixed)
             scf[tag] = time_scf(H, args.scf_iters, mixed)
             print(f"# SCF {tag}: {scf[tag]['s_per_iter']:.3f} s/iteration", file=sys.stderr, flush=True)
     batched = time_batched_k(args.steps, args.warmup) if args.batched_k else None
@@ -510,8 +536,9 @@ def main():
         "fp64": {"value": NBANDS / t_64, "ms_per_step": t_64 * 1e3},
         "mixed_over_fp64": t_64 / t_mix,
         "dE_per_atom_mixed_vs_fp64": dE,
-        "op_H_streams": "nonlocal products on a second stream beside the FFT passes" if args.overlap
-                        else "one stream",
+        "op_H_streams": ("nonlocal products on a second stream beside the FFT passes"
+                         if args.overlap and tuned["mixed"].get("overlap", 1) else "one stream"),
+        "autotune": tuned,
         "roofline": roofline(res["mixed"]["prof"], H, True),
         "roofline_fp64": roofline(res["fp64"]["prof"], H, False),
         "lobpcg": lob or None,

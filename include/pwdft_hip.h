@@ -95,11 +95,14 @@ enum {
   PW_TUNE_FUSED_NT = 13,        /* threads per workgroup of the fused batched op_H: 256, 512 or
                                    1024; 0 (default): the workgroups one CU's LDS holds share
                                    1024 threads */
-  PW_TUNE_EIG_SYNC = 14         /* the device tridiagonal reduction's cross-workgroup exchanges:
+  PW_TUNE_EIG_SYNC = 14,        /* the device tridiagonal reduction's cross-workgroup exchanges:
                                    1 (default) device-coherent atomic accesses, each producer
                                    waiting for its stores before the flag, no cache flushes
                                    (n = 480: 10.8 ms, n = 960: 21.4 ms); 0 plain accesses
                                    ordered by agent-scope fences (12.2, 33.4 ms) */
+  PW_TUNE_FFT_ORDER = 15,       /* radix plan of the FFT lines: 0 (default) largest radices
+                                   first (72 = 9 x 8, compile-time specialised); 1 the same
+                                   radices in reverse order; 2 radices 5, 4, 3, 2 only */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

@@ -50,14 +50,21 @@ int ensure(pw_ctx* c, void** buf, size_t* have, size_t need) {
   return PW_OK;
 }
 
-static bool factorize(int n, FftDim& d) {
+// Radix passes of a length-n line.  mode 0: largest radices first (9, 8, 5, 4, 3, 2; the
+// compile-time plans of 20, 27, 45, 72, 108 are of this form); 1: the same radices in
+// reverse order; 2: radices 5, 4, 3, 2 only (more passes, more lanes per line).
+static bool factorize(int n, FftDim& d, int mode = 0) {
   d.n = n;
   d.npass = 0;
   int rem = n;
-  const int order[6] = {9, 8, 5, 4, 3, 2};
+  const int big[6] = {9, 8, 5, 4, 3, 2};
+  const int small[6] = {5, 4, 3, 2, 0, 0};
+  const int* order = mode == 2 ? small : big;
   while (rem > 1) {
     bool found = false;
-    for (int r : order) {
+    for (int q = 0; q < 6; ++q) {
+      const int r = order[q];
+      if (r == 0) break;
       if (rem % r == 0) {
         if (d.npass >= 8) return false;
         d.radix[d.npass++] = r;
@@ -71,6 +78,18 @@ static bool factorize(int n, FftDim& d) {
   if (d.npass == 0) {  // n == 1: identity
     d.radix[d.npass++] = 1;
     return false;
+  }
+  if (mode == 1) std::reverse(d.radix, d.radix + d.npass);
+  return true;
+}
+
+// FFT plans of the three grid dimensions for a radix mode (factorize); false when a
+// dimension has no plan that fits one wavefront.
+static bool plan_grid(const int ns[3], int mode, FftDim dims[3]) {
+  for (int d = 0; d < 3; ++d) {
+    if (!factorize(ns[d], dims[d], mode)) return false;
+    plan_lanes(dims[d], 12);
+    if (dims[d].gpw < 1 || dims[d].L > 64) return false;
   }
   return true;
 }
@@ -384,6 +403,19 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
     case PW_TUNE_FUSED_BATCH:
       c->fused_batch = value != 0;
       return PW_OK;
+    case PW_TUNE_FFT_ORDER: {
+      if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: FFT radix mode must be 0..2");
+      if (c->N > 0) {
+        const int ns[3] = {c->n1, c->n2, c->n3};
+        FftDim dims[3];
+        if (!plan_grid(ns, value, dims))
+          return set_error(c, PW_ERR_ARG, "pw_tune_set: no FFT plan of this radix mode fits the grid");
+        for (int d = 0; d < 3; ++d) c->dim[d] = dims[d];
+        c->graph_gen++;   // captured launches carry the old plans
+      }
+      c->fft_order = value;
+      return PW_OK;
+    }
     case PW_TUNE_EIG_SYNC:
       if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver sync mode must be 0..1");
       c->eig_light = value;
@@ -422,6 +454,14 @@ int pw_set_grid(pw_ctx* c, int n1, int n2, int n3, double omega) {
     plan_lanes(dims[d], 12);
     if (dims[d].gpw < 1)
       return set_error(c, PW_ERR_ARG, "pw_set_grid: size " + std::to_string(ns[d]) + " too long for one wavefront");
+  }
+  // a radix mode chosen by the tuner applies to the new grid where it has plans
+  if (c->fft_order != 0) {
+    FftDim alt[3];
+    if (plan_grid(ns, c->fft_order, alt))
+      for (int d = 0; d < 3; ++d) dims[d] = alt[d];
+    else
+      c->fft_order = 0;
   }
   for (auto& k : c->kd) free_k(k);
   c->kd.clear();

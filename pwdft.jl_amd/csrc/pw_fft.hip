@@ -313,6 +313,72 @@ __device__ inline void wave_line_fft(cplx<T>* lds, int base, int es, const FftDi
   }
 }
 
+// One x-row of the local-potential product, row = F(V * F^-1(row)), for the two-pass
+// compile-time plans N = R0 R1.  The inverse transform runs the radices R0 then R1 and the
+// forward one R1 then R0.  The inverse's last pass leaves butterfly j holding the elements
+// j + R0 r, exactly the inputs of the forward's first pass: the row stays in registers
+// between the two transforms and V multiplies it there, so a row costs three LDS round
+// trips instead of four.  vrow: V/N of the row (length N), read only by active lanes.
+template <typename T, int N, int R0, int R1>
+__device__ __attribute__((noinline)) void row_vloc_static(LDS_AS cplx<T>* lds, int base,
+                                                          const LDS_AS cplx<T>* tw, int q, bool active,
+                                                          const T* __restrict__ vrow) {
+  constexpr int L = lanes_for(N, R0, R1, 1, 12);
+  constexpr int NB = N / R1;   // butterflies of the radix-R1 passes (= R0)
+  constexpr int MB = (NB + L - 1) / L;
+  // inverse, radix R0 (NS = 1)
+  spass<T, +1, N, 1, R0, L>(lds, base, 1, tw, q, active, nullptr, false);
+  // inverse, radix R1 (NS = R0, the last pass): butterfly j reads and leaves j + NB r
+  cplx<T> v[MB][R1];
+  bool on[MB];
+#pragma unroll
+  for (int s = 0; s < MB; ++s) {
+    const int j = q + s * L;
+    on[s] = active && (MB * L == NB || j < NB);
+    if (on[s]) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[s][r] = lds[base + j + r * NB];
+#pragma unroll
+      for (int r = 1; r < R1; ++r) v[s][r] = cmul(v[s][r], tw[j * r]);
+      DFT<T, +1, R1>::run(v[s]);
+      // times V/N, then the forward transform's first pass (radix R1, NS = 1) in registers
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[s][r] = scal(vrow[j + r * NB], v[s][r]);
+      DFT<T, -1, R1>::run(v[s]);
+    }
+  }
+  wave_fence();
+#pragma unroll
+  for (int s = 0; s < MB; ++s) {
+    if (on[s]) {
+      const int j = q + s * L;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) lds[base + j * R1 + r] = v[s][r];
+    }
+  }
+  wave_fence();
+  // forward, radix R0 (NS = R1)
+  spass<T, -1, N, R1, R0, L>(lds, base, 1, tw, q, active, nullptr, false);
+}
+
+// An x-row of the local-potential product through the fused two-pass routine where the
+// row length has a compile-time plan of two passes, else as two line transforms.
+template <typename T>
+__device__ inline void wave_row_vloc(cplx<T>* lds, int base, const FftDim& d, const cplx<T>* __restrict__ tw, int q,
+                                     bool active, const T* vrow) {
+  LDS_AS cplx<T>* l3 = (LDS_AS cplx<T>*)lds;
+  const LDS_AS cplx<T>* t3 = (const LDS_AS cplx<T>*)tw;
+  switch (d.plan) {
+    case 72: row_vloc_static<T, 72, 9, 8>(l3, base, t3, q, active, vrow); return;
+    case 27: row_vloc_static<T, 27, 9, 3>(l3, base, t3, q, active, vrow); return;
+    case 45: row_vloc_static<T, 45, 9, 5>(l3, base, t3, q, active, vrow); return;
+    case 20: row_vloc_static<T, 20, 5, 4>(l3, base, t3, q, active, vrow); return;
+    default: break;
+  }
+  wave_line_fft<T, +1, FFT_ME>(lds, base, 1, d, tw, q, active, nullptr);
+  wave_line_fft<T, -1, FFT_ME>(lds, base, 1, d, tw, q, active, active ? vrow : nullptr);
+}
+
 // All `nlines` lines of a workgroup, distributed over its wavefronts; line t starts at
 // lds[lbase ? lbase[t] : t * lstride].  Wavefronts run their lines independently.
 template <typename T, int S, int ME>
@@ -423,8 +489,7 @@ __device__ inline void plane_vloc_body(cplx<T>* __restrict__ w, const T* __restr
       const int row = t0 + g;
       const bool active = g < dx.gpw && row < n2;
       const int base = active ? row * ps : 0;
-      wave_line_fft<T, +1, FFT_ME>(lds, base, 1, dx, tx, q, active, nullptr);
-      wave_line_fft<T, -1, FFT_ME>(lds, base, 1, dx, tx, q, active, active ? v + (long)row * n1 : nullptr);
+      wave_row_vloc<T>(lds, base, dx, tx, q, active, v + (long)(active ? row : 0) * n1);
     }
   }
   __syncthreads();
@@ -1156,8 +1221,7 @@ __device__ inline void fused_rows_vloc(cplx<T>* lds, int nrows, int ps, const Ff
     const int row = t0 + g;
     const bool active = g < dx.gpw && row < nrows;
     const int base = active ? row * ps : 0;
-    wave_line_fft<T, +1, FFT_ME>(lds, base, 1, dx, tx, q, active, nullptr);
-    wave_line_fft<T, -1, FFT_ME>(lds, base, 1, dx, tx, q, active, active ? V + (long)row * n1 : nullptr);
+    wave_row_vloc<T>(lds, base, dx, tx, q, active, V + (long)(active ? row : 0) * n1);
   }
 }
 

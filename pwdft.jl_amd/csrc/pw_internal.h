@@ -257,6 +257,7 @@ struct pw_ctx {
   // a captured step whose generation differs is discarded, never replayed
   unsigned graph_gen = 0;
   bool fused_batch = true;                            // batched op_H: one fused launch where the box fits LDS
+  int fft_order = 0;                                  // radix mode of the FFT plans (pw_tune_set PW_TUNE_FFT_ORDER)
   int fused_nt = 0;                                   // threads per fused workgroup (0: by LDS, 256/512/1024)
   // density side of an SCF iteration (pw_scf.hip)
   double* d_vps = nullptr;                            // V_Ps_loc(r) [N]

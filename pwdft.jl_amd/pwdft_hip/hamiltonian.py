@@ -98,7 +98,7 @@ class Hamiltonian:
     def __init__(self, atoms, pspfiles, ecutwfc, meshk=(1, 1, 1), shiftk=(0, 0, 0), Ns_=None,
                  kpoints=None, rots=None, extra_states=None, Nstates=None, use_smearing=False,
                  smearing_kT=0.0, xcfunc="VWN", device=0, band_batch=64, mixed=False,
-                 use_symmetry=True, time_reversal=True):
+                 use_symmetry=True, time_reversal=True, autotune=False):
         if len(pspfiles) != atoms.Nspecies:
             raise ValueError(f"Length of pspfiles is not equal to Nspecies={atoms.Nspecies}")
         self.pspots = [p if isinstance(p, PsPot_GTH) else PsPot_GTH(p) for p in pspfiles]
@@ -132,10 +132,17 @@ class Hamiltonian:
         self._rho_scratch = None
         self.potentials = Potentials(self.ctx, self.pw.Npoints)
         self.E_NN = calc_E_NN(atoms, self.Zvals)
-        self.ctx.call("pw_set_band_batch", int(band_batch))
+        self.band_batch = int(band_batch)
+        self.ctx.call("pw_set_band_batch", self.band_batch)
         self._upload_static()          # also builds V_Ps_loc and betaNL on the device
         self.potentials.Total[:, 0] = self.potentials.Ps_loc
         self._upload_vloc()
+        # op_H launch parameters for this shape (autotune.py): from the tuning cache, or timed
+        # once on the device and cached; (settings, "cache" | "tuned"), or None
+        self.tuned = None
+        if autotune:
+            from .autotune import autotune as _autotune
+            self.tuned = _autotune(self, mixed=mixed)
 
     # ---- device state ---------------------------------------------------------------
     def _upload_static(self):

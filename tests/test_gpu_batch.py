@@ -303,7 +303,10 @@ def test_lobpcg_batch_graph_replay_vs_eager(al, mixed):
             print(f"case Al LOBPCG batch graph={graph} rep={rep} mixed={mixed}", flush=True)
             ps = ph.rand_BlochWavefunc(H, seed=12)
             sub = [ps[k] for k in iks]
-            ev, st = ph.diag_LOBPCG_batch(H, sub, tol=1e-9, NiterMax=40, iks=iks, mixed=mixed)
+            # a bounded number of iterations: run to full convergence, the mixed path's P
+            # block can turn rank-deficient (Cholesky of P^H P fails, as the reference's
+            # cholesky() would)
+            ev, st = ph.diag_LOBPCG_batch(H, sub, tol=1e-9, NiterMax=12, iks=iks, mixed=mixed)
             runs.append((ev, [m.download() for m in sub], st))
     H.ctx.call("pw_tune_set", PW_TUNE_LOBPCG_GRAPH, 1)
     ev0, X0, st0 = runs[0]

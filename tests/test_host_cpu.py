@@ -222,3 +222,29 @@ def test_packed_fp32_guard_catches_packed_instructions(tmp_path):
     with pytest.raises(RuntimeError, match="packed FP32"):
         B.check_no_packed_fp32(str(bad))
     assert B.check_no_packed_fp32(str(good)) == 1
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    """The tuning cache: shape keys, merged writes, and settings applied in dependency order
+    (the FFT radix mode first) -- host logic only, on a stand-in context."""
+    from pwdft_hip import autotune as at
+    path = str(tmp_path / "sub" / "tune.json")
+    k1 = at.shape_key((72, 72, 72), 23871, 320, 160, True)
+    k2 = at.shape_key((20, 20, 20), 316, 4, 8, False)
+    assert k1 == "72x72x72/ngw23871/nbeta320/nb160/mixed" and k1 != k2
+    assert at.load_cache(path) == {}
+    at.save_cache({k1: {"fft_order": 1, "band_batch": 96}}, path)
+    at.save_cache({k2: {"fft_order": 0}}, path)
+    d = at.load_cache(path)
+    assert d[k1]["band_batch"] == 96 and d[k2]["fft_order"] == 0
+
+    class Ctx:
+        calls = []
+
+        def call(self, name, *args):
+            self.calls.append((name, args))
+
+    ctx = Ctx()
+    at.apply_settings(ctx, {"band_batch": 32, "gemm_nn_shape": 4, "fft_order": 2, "unknown": 7})
+    assert ctx.calls == [("pw_tune_set", (at.TUNE_FFT_ORDER, 2)), ("pw_tune_set", (at.TUNE_BAND_BATCH, 32)),
+                         ("pw_tune_set", (at.TUNE_GEMM_NN_SHAPE, 4))]
