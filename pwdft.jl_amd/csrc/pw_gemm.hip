@@ -170,25 +170,43 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
   for (int k0 = kbeg; k0 < kend; k0 += SBK) {
     const bool more = k0 + SBK < kend;
     if (more) load_stage(k0 + SBK);  // in flight during the MFMA phase
-#pragma unroll
-    for (int q = 0; q < (2 * SBK) / 4; ++q) {
+    // operand fragments of k-step q: read from LDS one step ahead of the MFMAs that use
+    // them (two register sets), so the LDS latency overlaps the previous step's MFMAs
+    // Ã's sign pattern depends on the lane and the row tile only (k-step parity is lg's)
+    constexpr int NQ = (2 * SBK) / 4;
+    TC a[2][4], b[2][2];
+    auto frag = [&](int q, int p) {
       const int kp = 4 * q + lg;
       const int kc = kp >> 1, t = kp & 1;
-      TC a[4], b[2];
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
         const int ip = wm * 64 + rt * 16 + Mfma<TC>::row(li);
-        const int sgn = ip & 1, ml = ip >> 1;
-        const TC av = As[buf][ml][2 * kc + (sgn ^ t)];
-        const bool neg = (TRANSA == 0) ? (sgn == 0 && t == 1) : (sgn == 1 && t == 0);
-        a[rt] = neg ? -av : av;
+        a[p][rt] = As[buf][ip >> 1][2 * kc + ((ip & 1) ^ t)];
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) b[ct] = Bs[buf][wn * 32 + ct * 16 + li][kp];
+      for (int ct = 0; ct < 2; ++ct) b[p][ct] = Bs[buf][wn * 32 + ct * 16 + li][kp];
+    };
+    bool neg[4];
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
+    for (int rt = 0; rt < 4; ++rt) {
+      const int sgn = (wm * 64 + rt * 16 + Mfma<TC>::row(li)) & 1, t = lg & 1;
+      neg[rt] = (TRANSA == 0) ? (sgn == 0 && t == 1) : (sgn == 1 && t == 0);
+    }
+    frag(0, 0);
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = Mfma<TC>::run(a[rt], b[ct], acc[rt][ct]);
+    for (int q = 0; q < NQ; ++q) {
+      const int p = q & 1;
+      if (q + 1 < NQ) frag(q + 1, p ^ 1);
+      // keep the next step's LDS reads ahead of this step's MFMAs (the scheduler would
+      // otherwise sink them to their use), so their latency overlaps these MFMAs
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const TC av = neg[rt] ? -a[p][rt] : a[p][rt];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = Mfma<TC>::run(av, b[p][ct], acc[rt][ct]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
     // one barrier per stage: the other buffer is written while this one is read, and no
     // wave writes this buffer again before every wave has passed this barrier
