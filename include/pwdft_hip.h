@@ -86,15 +86,19 @@ enum {
   PW_TUNE_LOBPCG_LOCK = 10,     /* lock threshold of pw_diag_lobpcg_dev: 0 (default) the
                                    reference's 10 eps; k > 0: 10 * 10^-k (tests of the branch) */
   PW_TUNE_FUSED_BATCH = 11,     /* pw_op_H_batch_dev: 1 (default) one fused launch, a workgroup
-                                   per (k-point, band) holding the band's whole grid in LDS,
-                                   wherever the grid fits (20^3 in FP64); 0: the three-stage
-                                   column / plane / column launches and batched GEMMs */
+                                   per (k-point, band), wherever the band's grid fits one
+                                   workgroup's LDS (20^3 in FP64); on a cubic 20^3 grid its line
+                                   transforms run in registers on a box kept at the active x
+                                   only; 2: the fused launch with the whole box in LDS and
+                                   wavefront-shared line transforms; 0: the three-stage column /
+                                   plane / column launches and batched GEMMs */
   PW_TUNE_LOBPCG_GRAPH = 12,    /* pw_diag_lobpcg_batch_dev: 1 (default) replay each step from
                                    a captured hipGraph once it has run eagerly (orders 3 Nst <=
                                    64); 0: every step enqueued eagerly */
   PW_TUNE_FUSED_NT = 13,        /* threads per workgroup of the fused batched op_H: 256, 512 or
                                    1024; 0 (default): the workgroups one CU's LDS holds share
-                                   1024 threads */
+                                   1024 threads (whole-box kernel), 256 (register-line kernel,
+                                   which takes 256 or 512) */
   PW_TUNE_EIG_SYNC = 14,        /* the device tridiagonal reduction's cross-workgroup exchanges:
                                    1 (default) device-coherent atomic accesses, each producer
                                    waiting for its stores before the flag, no cache flushes

@@ -401,7 +401,8 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       c->overlap = value != 0;
       return PW_OK;
     case PW_TUNE_FUSED_BATCH:
-      c->fused_batch = value != 0;
+      if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: fused batch mode must be 0..2");
+      c->fused_batch = value;
       return PW_OK;
     case PW_TUNE_FFT_ORDER: {
       if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: FFT radix mode must be 0..2");

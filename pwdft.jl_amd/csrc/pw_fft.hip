@@ -1381,10 +1381,300 @@ static int fused_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, cons
   return hip_check(c, hipGetLastError(), "k_op_H_fused");
 }
 
+// ------------------------------------------------ fused op_H with line transforms in registers
+// The same operator as k_op_H_fused for cubic grids N^3 with a two-pass register plan
+// (N = R0 R1), built for occupancy: every 1-D transform is done by one lane on a line held in
+// its registers (load N entries, two radix passes, store), and the box is kept in LDS only at
+// the x values that carry data, as [z][a][y] (a: index of x among the k-point's active x,
+// y stride 1, padded to an odd length).  The x-rows, which need all N x, are expanded in
+// registers: zero at the inactive x, inverse transform, * V/N, forward transform, and only
+// the active x are stored back.  The FP64 box of a 20^3 grid with 9 active x takes 60 KB
+// instead of 134 KB, so two workgroups share a CU and hide each other's memory latency.
+
+// X[k] = sum_n x[n] exp(S 2 pi i n k / N) of a line in registers, n = R1 n1 + n2,
+// k = k1 + R0 k2: R1 transforms of length R0 over n1, twiddles exp(S 2 pi i n2 k1 / N), R0
+// transforms of length R1 over n2.  tw[t] = exp(+2 pi i t / N).
+template <typename T, int S, int N, int R0, int R1>
+__device__ inline void fft_regs(cplx<T> (&v)[N], const LDS_AS cplx<T>* tw) {
+  static_assert(R0 * R1 == N, "two-pass plan");
+  cplx<T> y[R1][R0];
+#pragma unroll
+  for (int n2 = 0; n2 < R1; ++n2) {
+    cplx<T> t[R0];
+#pragma unroll
+    for (int n1 = 0; n1 < R0; ++n1) t[n1] = v[R1 * n1 + n2];
+    DFT<T, S, R0>::run(t);
+#pragma unroll
+    for (int k1 = 0; k1 < R0; ++k1) {
+      if (n2 > 0 && k1 > 0) {
+        const cplx<T> w = tw[n2 * k1];
+        t[k1] = S > 0 ? cmul(t[k1], w) : cmulc(t[k1], w);
+      }
+      y[n2][k1] = t[k1];
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R0; ++k1) {
+    cplx<T> t[R1];
+#pragma unroll
+    for (int n2 = 0; n2 < R1; ++n2) t[n2] = y[n2][k1];
+    DFT<T, S, R1>::run(t);
+#pragma unroll
+    for (int k2 = 0; k2 < R1; ++k2) v[k1 + R0 * k2] = t[k2];
+  }
+}
+
+// Lines [l0, nl) of the workgroup, one per thread: line l starts at lds[base(l)], stride es.
+template <typename T, int S, int N, int R0, int R1, int NT, typename Base>
+__device__ inline void reg_lines(LDS_AS cplx<T>* lds, int nl, int es, const LDS_AS cplx<T>* tw, Base base) {
+  for (int l = threadIdx.x; l < nl; l += NT) {
+    const int b0 = base(l);
+    cplx<T> v[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = lds[b0 + i * es];
+    fft_regs<T, S, N, R0, R1>(v, tw);
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[b0 + i * es] = v[i];
+  }
+}
+
+// One workgroup per (k-point, band).  LDS: box [N][nxa][YS] (sized for nxa_max), twiddles
+// (N), per-wavefront partial sums of beta^H psi (NT/64 x nbeta_max), column bases (ncol_max
+// ints), x -> a map (N ints).  Every global read that does not depend on the transforms is
+// issued before them: the first GPT entries of the sphere per thread (their psi, box slot,
+// kinetic, nonlocal and accumulated terms) stay in registers from the start to the final
+// store, so the transforms run between one round of reads and one round of writes.
+template <typename T, int N, int R0, int R1, int NT>
+__global__ void __launch_bounds__(NT)
+k_op_H_reg(const FusedK* __restrict__ kd, int nb, const T* __restrict__ V, const cplx<T>* __restrict__ twg,
+           int nxa_max, int nbeta_max, int do_v, int do_k, int do_nl, int acc) {
+  constexpr int YS = (N % 2 == 0) ? N + 1 : N;   // odd_stride(N): y stride of the box, and gpos's x stride
+  constexpr int GPT = 2;                          // sphere entries per thread held in registers
+  constexpr int NW = NT / 64;
+  extern __shared__ char smem_raw[];
+  const int kk = blockIdx.x / nb, b = blockIdx.x - kk * nb;
+  const FusedK F = kd[kk];
+  const int nxa = F.nxact;
+  const int zs = nxa * YS;   // one z-slab of the box
+  const int nbeta = do_nl ? F.nbeta : 0;
+  LDS_AS cplx<T>* box = (LDS_AS cplx<T>*)smem_raw;
+  LDS_AS cplx<T>* tw = box + (size_t)N * nxa_max * YS;
+  cplx<T>* red = (cplx<T>*)(tw + N);
+  int* cb = reinterpret_cast<int*>(red + (size_t)NW * (nbeta_max > 0 ? nbeta_max : 1));
+  int* xa = cb + (F.ncol > 0 ? F.ncol : 1);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const cplx<double>* p = F.psi + (long)b * F.ldpsi;
+  cplx<double>* h = F.hpsi + (long)b * F.ldh;
+  const cplx<T>* be = reinterpret_cast<const cplx<T>*>(F.beta);
+  const cplx<T>* bd = reinterpret_cast<const cplx<T>*>(F.betaD);
+
+  // global reads of the set-up, issued together
+  int gq[GPT];
+  cplx<double> pv[GPT], hv[GPT];
+#pragma unroll
+  for (int i = 0; i < GPT; ++i) {
+    const int g = tid + i * NT;
+    const bool in = g < F.ngw;
+    gq[i] = in ? F.gpos[g] : 0;
+    pv[i] = in ? p[g] : mk(0.0, 0.0);
+    hv[i] = (in && acc) ? h[g] : mk(0.0, 0.0);
+  }
+  const int xv = tid < nxa ? F.xact[tid] : 0;
+  const cplx<T> twv = tid < N ? twg[tid] : mk((T)0, (T)0);
+  // psi row g -> box slot: gpos = x + YS (y + N z)
+  auto slot_of = [&](int q) {
+    const int x = q % YS, r = q / YS;
+    return (r / N) * zs + xa[x] * YS + (r % N);
+  };
+  if (do_v) {
+    for (int i = tid; i < N * zs; i += NT) box[i] = mk((T)0, (T)0);
+    if (tid < N) tw[tid] = twv;
+  }
+  // beta^H psi: each thread's entries times every projector, summed over each wavefront
+  for (int j = 0; j < nbeta; ++j) {
+    T sr = 0, si = 0;
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int g = tid + i * NT;
+      if (g < F.ngw) {
+        const cplx<T> bg = be[(long)j * F.ngw + g];
+        const cplx<T> x = cvt<T>(pv[i]);
+        sr += bg.x * x.x + bg.y * x.y;
+        si += bg.x * x.y - bg.y * x.x;
+      }
+    }
+    for (int g = tid + GPT * NT; g < F.ngw; g += NT) {
+      const cplx<T> bg = be[(long)j * F.ngw + g];
+      const cplx<T> x = cvt<T>(p[g]);
+      sr += bg.x * x.x + bg.y * x.y;
+      si += bg.x * x.y - bg.y * x.x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      sr += __shfl_xor(sr, o, 64);
+      si += __shfl_xor(si, o, 64);
+    }
+    if (lane == 0) red[wave * nbeta + j] = mk(sr, si);
+  }
+  __syncthreads();
+  if (do_v && tid < nxa) xa[xv] = tid;
+  // kinetic and nonlocal terms of the held entries (B_j summed from the wavefront partials)
+#pragma unroll
+  for (int i = 0; i < GPT; ++i) {
+    const int g = tid + i * NT;
+    if (g < F.ngw) {
+      if (do_k) hv[i] = hv[i] + scal(0.5 * F.gk2[g], pv[i]);
+      if (nbeta > 0) {
+        cplx<T> s = mk((T)0, (T)0);
+        for (int j = 0; j < nbeta; ++j) {
+          cplx<T> B = red[j];
+          for (int w = 1; w < NW; ++w) B = B + red[w * nbeta + j];
+          s = s + cmul(bd[(long)j * F.ngw + g], B);
+        }
+        hv[i] = hv[i] + cvt<double>(s);
+      }
+    }
+  }
+  if (do_v) {
+    __syncthreads();
+    // column (i, j) -> (a, y) = (xa[i], j)
+    for (int c = tid; c < F.ncol; c += NT) {
+      const int cij = F.col_ij[c];
+      cb[c] = xa[cij % YS] * YS + cij / YS;
+    }
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      if (tid + i * NT < F.ngw) {
+        gq[i] = slot_of(gq[i]);
+        box[gq[i]] = cvt<T>(pv[i]);
+      }
+    }
+    for (int g = tid + GPT * NT; g < F.ngw; g += NT) box[slot_of(F.gpos[g])] = cvt<T>(p[g]);
+    __syncthreads();
+    // inverse: z-lines of the sphere's columns, y-lines at the active x
+    reg_lines<T, +1, N, R0, R1, NT>(box, F.ncol, zs, tw, [&](int l) { return cb[l]; });
+    __syncthreads();
+    reg_lines<T, +1, N, R0, R1, NT>(box, N * nxa, 1, tw, [&](int l) { return l * YS; });
+    __syncthreads();
+    // x-rows (y, z): expand the active x, inverse, * V/N, forward, keep the active x
+    unsigned xmask = 0;
+    for (int a = 0; a < nxa; ++a) xmask |= 1u << F.xact[a];
+    for (int r = tid; r < N * N; r += NT) {
+      const int z = r / N, y = r - z * N;
+      const int b0 = z * zs + y;
+      const T* vr = V + (long)r * N;
+      cplx<T> v[N];
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const int a = __popc(xmask & ((1u << x) - 1u));
+        v[x] = ((xmask >> x) & 1u) ? box[b0 + a * YS] : mk((T)0, (T)0);
+      }
+      fft_regs<T, +1, N, R0, R1>(v, tw);
+#pragma unroll
+      for (int x = 0; x < N; ++x) v[x] = scal(vr[x], v[x]);
+      fft_regs<T, -1, N, R0, R1>(v, tw);
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const int a = __popc(xmask & ((1u << x) - 1u));
+        if ((xmask >> x) & 1u) box[b0 + a * YS] = v[x];
+      }
+    }
+    __syncthreads();
+    // forward: y-lines, then z-lines
+    reg_lines<T, -1, N, R0, R1, NT>(box, N * nxa, 1, tw, [&](int l) { return l * YS; });
+    __syncthreads();
+    reg_lines<T, -1, N, R0, R1, NT>(box, F.ncol, zs, tw, [&](int l) { return cb[l]; });
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < GPT; ++i) {
+    const int g = tid + i * NT;
+    if (g < F.ngw) h[g] = do_v ? hv[i] + cvt<double>(box[gq[i]]) : hv[i];
+  }
+  // entries beyond the held ones
+  for (int g = tid + GPT * NT; g < F.ngw; g += NT) {
+    const cplx<double> x = p[g];
+    cplx<double> r = acc ? h[g] : mk(0.0, 0.0);
+    if (do_v) r = r + cvt<double>(box[slot_of(F.gpos[g])]);
+    if (do_k) r = r + scal(0.5 * F.gk2[g], x);
+    if (nbeta > 0) {
+      cplx<T> s = mk((T)0, (T)0);
+      for (int j = 0; j < nbeta; ++j) {
+        cplx<T> B = red[j];
+        for (int w = 1; w < NW; ++w) B = B + red[w * nbeta + j];
+        s = s + cmul(bd[(long)j * F.ngw + g], B);
+      }
+      r = r + cvt<double>(s);
+    }
+    h[g] = r;
+  }
+}
+
+// The register-line kernel for a cubic 20^3 grid (radix plan 5 x 4); false where it does not apply.
+static bool reg_plan(const pw_ctx* c) { return c->n1 == 20 && c->n2 == 20 && c->n3 == 20; }
+
+template <typename T>
+static int fused_reg_run(pw_ctx* c, int nk, const int* iks, const T* V, int nb, const cplx<double>* const* psi,
+                         const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
+  constexpr int N = 20, YS = 21;
+  const unsigned terms = flags & PW_TERM_ALL;
+  std::vector<FusedK> kb(nk);
+  int nxa_max = 1, ncol_max = 1, nbeta_max = 0;
+  for (int kk = 0; kk < nk; ++kk) {
+    const KData& k = c->kd[iks[kk]];
+    FusedK d{};
+    d.psi = psi[kk];
+    d.hpsi = hpsi[kk];
+    d.ldpsi = ldpsi[kk];
+    d.ldh = ldh[kk];
+    d.gpos = k.d_gpos;
+    d.gk2 = k.d_gk2;
+    d.col_ij = k.d_col_ij;
+    d.xact = k.d_xact;
+    d.nbeta = (terms & PW_TERM_NLOC) ? k.nbeta : 0;
+    d.beta = sizeof(T) == 8 ? k.d_beta64 : k.d_beta32;
+    d.betaD = sizeof(T) == 8 ? k.d_betaD64 : k.d_betaD32;
+    d.ngw = k.ngw;
+    d.ncol = k.ncol;
+    d.nxact = k.nxact;
+    if (d.nbeta > 0 && (!d.beta || !d.betaD)) return set_error(c, PW_ERR_STATE, "op_H fused: projectors not set");
+    if (k.nxact < 1 || k.nxact > N) return set_error(c, PW_ERR_STATE, "op_H fused: bad active-x count");
+    kb[kk] = d;
+    nxa_max = std::max(nxa_max, k.nxact);
+    ncol_max = std::max(ncol_max, k.ncol);
+    nbeta_max = std::max(nbeta_max, d.nbeta);
+  }
+  const int nw = (c->fused_nt == 512 ? 512 : 256) / 64;
+  const size_t lds = ((size_t)N * nxa_max * YS + N + (size_t)nw * std::max(nbeta_max, 1)) * sizeof(cplx<T>) +
+                     (size_t)(ncol_max + N) * sizeof(int);
+  if (lds > 160 * 1024) return set_error(c, PW_ERR_ARG, "op_H fused: the compressed box does not fit in LDS");
+  if ((long)nk * nb > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "op_H fused: too many bands for one launch");
+  const void* dptr = nullptr;
+  int rc = desc_get(c, kb.data(), kb.size() * sizeof(FusedK), &dptr);
+  if (rc) return rc;
+  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(sizeof(T) == 8 ? c->d_tw64[0] : c->d_tw32[0]);
+  const FusedK* dk = reinterpret_cast<const FusedK*>(dptr);
+  const int dv = (terms & PW_TERM_VLOC) ? 1 : 0, dkin = (terms & PW_TERM_K) ? 1 : 0;
+  const int dnl = (terms & PW_TERM_NLOC) ? 1 : 0, acc = (flags & PW_ACCUMULATE) ? 1 : 0;
+  const dim3 grid((unsigned)(nk * nb));
+  // 256 threads (the 400 x-rows in two rounds) unless the tuning knob asks for 512
+  if (c->fused_nt == 512)
+    hipLaunchKernelGGL((k_op_H_reg<T, N, 5, 4, 512>), grid, dim3(512), lds, c->stream, dk, nb, V, tw, nxa_max,
+                       nbeta_max, dv, dkin, dnl, acc);
+  else
+    hipLaunchKernelGGL((k_op_H_reg<T, N, 5, 4, 256>), grid, dim3(256), lds, c->stream, dk, nb, V, tw, nxa_max,
+                       nbeta_max, dv, dkin, dnl, acc);
+  return hip_check(c, hipGetLastError(), "k_op_H_reg");
+}
+
 int op_H_fused_batch(pw_ctx* c, int nk, const int* iks, int ispin, int nb, const cplx<double>* const* psi,
                      const int64_t* ldpsi, cplx<double>* const* hpsi, const int64_t* ldh, unsigned flags) {
   if (!op_H_fused_fits(c, (flags & PW_MIXED) != 0)) return set_error(c, PW_ERR_ARG, "op_H fused: grid too large");
-  if (flags & PW_MIXED) return fused_run<float>(c, nk, iks, c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
+  const bool reg = c->fused_batch == 1 && reg_plan(c);
+  if (flags & PW_MIXED) {
+    if (reg) return fused_reg_run<float>(c, nk, iks, c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
+    return fused_run<float>(c, nk, iks, c->d_v32[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
+  }
+  if (reg) return fused_reg_run<double>(c, nk, iks, c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
   return fused_run<double>(c, nk, iks, c->d_v64[ispin], nb, psi, ldpsi, hpsi, ldh, flags);
 }
 

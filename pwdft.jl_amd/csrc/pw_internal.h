@@ -256,7 +256,8 @@ struct pw_ctx {
   // bumped whenever device memory that a captured step may address is allocated or freed;
   // a captured step whose generation differs is discarded, never replayed
   unsigned graph_gen = 0;
-  bool fused_batch = true;                            // batched op_H: one fused launch where the box fits LDS
+  int fused_batch = 1;                                // batched op_H: 1 fused launch (register-line kernel where it
+                                                      // applies), 2 fused launch with the whole box in LDS, 0 three stages
   int fft_order = 0;                                  // radix mode of the FFT plans (pw_tune_set PW_TUNE_FFT_ORDER)
   int fused_nt = 0;                                   // threads per fused workgroup (0: by LDS, 256/512/1024)
   // density side of an SCF iteration (pw_scf.hip)

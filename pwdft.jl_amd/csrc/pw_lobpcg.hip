@@ -886,7 +886,7 @@ extern "C" int pw_diag_lobpcg_batch_dev(pw_ctx* c, int nk, const int* iks, int i
     B.linfo = B.info + 8 * nk;
   }
   B.use_graph = c->lob_graph && c->eig_path == 0 && N3 <= small_dense_max_n();
-  B.key0 = {N, ispin, (int)B.hflags, c->fused_batch ? 1 : 0, nk};
+  B.key0 = {N, ispin, (int)B.hflags, c->fused_batch, nk};
   B.key0.insert(B.key0.end(), iks, iks + nk);
   // stale captured steps (a buffer they address was reallocated) are dropped here
   {

@@ -421,25 +421,33 @@ def test_op_H_batch_graph_replay(al, mixed):
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_op_H_batch_fused_vs_three_stage(al, mixed):
-    """The fused launch (one workgroup per (k-point, band), the band's 20^3 box in LDS) and
-    the three-stage launches (column / plane / column kernels with the intermediate in HBM,
-    batched GEMMs for the nonlocal term) give the same op_H, per term and accumulated."""
+    """The fused launches (one workgroup per (k-point, band): the register-line kernel on the
+    20^3 box compressed to the active x, at 256 and 512 threads, and the kernel with the whole
+    box in LDS) and the three-stage launches (column / plane / column kernels with the
+    intermediate in HBM, batched GEMMs for the nonlocal term) give the same op_H, per term
+    and accumulated."""
     import pwdft_hip as ph
     from pwdft_hip._lib import (PW_ACCUMULATE, PW_MIXED, PW_TERM_ALL, PW_TERM_K, PW_TERM_NLOC, PW_TERM_VLOC,
-                                PW_TUNE_FUSED_BATCH)
+                                PW_TUNE_FUSED_BATCH, PW_TUNE_FUSED_NT)
     H, host = al
     dev = [ph.DeviceMatrix.from_host(H.ctx, a) for a in host]
     base = [ph.DeviceMatrix.from_host(H.ctx, 0.5 * a) for a in host]
     tol = 2e-5 if mixed else 1e-13
-    for terms in (PW_TERM_ALL, PW_TERM_VLOC, PW_TERM_K | PW_TERM_VLOC, PW_TERM_NLOC):
-        for acc in (0, PW_ACCUMULATE):
-            res = {}
-            for fused in (1, 0):
-                H.ctx.call("pw_tune_set", PW_TUNE_FUSED_BATCH, fused)
-                out = [ph.DeviceMatrix.from_host(H.ctx, b.download()) for b in base]
-                print(f"case fused={fused} terms={terms} acc={acc} mixed={mixed}", flush=True)
-                ph.op_H_batch(H, dev, out=out, terms=terms | (PW_MIXED if mixed else 0), accumulate=bool(acc))
-                res[fused] = [o.download() for o in out]
-            H.ctx.call("pw_tune_set", PW_TUNE_FUSED_BATCH, 1)
-            worst = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(res[1], res[0]))
-            assert worst < tol, (terms, acc, worst)
+    modes = [("reg", 1, 0), ("reg512", 1, 512), ("box", 2, 0), ("three-stage", 0, 0)]
+    try:
+        for terms in (PW_TERM_ALL, PW_TERM_VLOC, PW_TERM_K | PW_TERM_VLOC, PW_TERM_NLOC):
+            for acc in (0, PW_ACCUMULATE):
+                res = {}
+                for name, fused, nt in modes:
+                    H.ctx.call("pw_tune_set", PW_TUNE_FUSED_BATCH, fused)
+                    H.ctx.call("pw_tune_set", PW_TUNE_FUSED_NT, nt)
+                    out = [ph.DeviceMatrix.from_host(H.ctx, b.download()) for b in base]
+                    print(f"case {name} terms={terms} acc={acc} mixed={mixed}", flush=True)
+                    ph.op_H_batch(H, dev, out=out, terms=terms | (PW_MIXED if mixed else 0), accumulate=bool(acc))
+                    res[name] = [o.download() for o in out]
+                for name in ("reg", "reg512", "box"):
+                    worst = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(res[name], res["three-stage"]))
+                    assert worst < tol, (name, terms, acc, worst)
+    finally:
+        H.ctx.call("pw_tune_set", PW_TUNE_FUSED_BATCH, 1)
+        H.ctx.call("pw_tune_set", PW_TUNE_FUSED_NT, 0)
