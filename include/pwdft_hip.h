@@ -107,6 +107,11 @@ enum {
   PW_TUNE_FFT_ORDER = 15,       /* radix plan of the FFT lines: 0 (default) largest radices
                                    first (72 = 9 x 8, compile-time specialised); 1 the same
                                    radices in reverse order; 2 radices 5, 4, 3, 2 only */
+  PW_TUNE_BATCH_GEMM_SMALL = 16 /* pw_zgemm_batch_dev and the batched solvers: 1 (default) the
+                                   small-product kernels (A B with k <= 64 and n <= 32: one
+                                   thread per row; A^H B with m, n <= 32: one workgroup per
+                                   product) where every product of the batch fits them; 0: the
+                                   16 x 16-tile kernel for all */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

@@ -36,8 +36,13 @@ __global__ void __launch_bounds__(256) k_zgemm_batch(const GBatch* __restrict__ 
   const int i0 = (t % tm) * BT, j0 = (t / tm) * BT;
   const int tid = threadIdx.x, tx = tid % BT, ty = tid / BT;
   cplx<double> acc = mk(0.0, 0.0);
-  for (int k0 = 0; k0 < g.k; k0 += BK) {
-    for (int e = tid; e < BK * BT; e += 256) {
+  // the next chunk's elements are loaded into registers while the current one is multiplied
+  constexpr int EPT = BK * BT / 256;
+  cplx<double> ra[EPT], rb[EPT];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < EPT; ++s) {
+      const int e = tid + 256 * s;
       int kc, i;
       if (TA) {
         kc = e % BK;
@@ -47,15 +52,24 @@ __global__ void __launch_bounds__(256) k_zgemm_batch(const GBatch* __restrict__ 
         kc = e / BT;
       }
       const int kk = k0 + kc, ii = i0 + i;
-      cplx<double> a = mk(0.0, 0.0);
-      if (kk < g.k && ii < g.m) a = TA ? g.A[(long)ii * g.lda + kk] : g.A[(long)kk * g.lda + ii];
-      As[kc][i] = a;
-      const int kb = e % BK, j = e / BK, jj = j0 + j;
-      cplx<double> b = mk(0.0, 0.0);
-      if (k0 + kb < g.k && jj < g.n) b = g.B[(long)jj * g.ldb + k0 + kb];
-      Bs[kb][j] = b;
+      ra[s] = mk(0.0, 0.0);
+      if (kk < g.k && ii < g.m) ra[s] = TA ? g.A[(long)ii * g.lda + kk] : g.A[(long)kk * g.lda + ii];
+      const int kb = e % BK, jj = j0 + e / BK;
+      rb[s] = mk(0.0, 0.0);
+      if (k0 + kb < g.k && jj < g.n) rb[s] = g.B[(long)jj * g.ldb + k0 + kb];
+    }
+  };
+  if (g.k > 0) load(0);
+  for (int k0 = 0; k0 < g.k; k0 += BK) {
+#pragma unroll
+    for (int s = 0; s < EPT; ++s) {
+      const int e = tid + 256 * s;
+      if (TA) As[e % BK][e / BK] = ra[s];
+      else As[e / BT][e % BT] = ra[s];
+      Bs[e % BK][e / BK] = rb[s];
     }
     __syncthreads();
+    if (k0 + BK < g.k) load(k0 + BK);
 #pragma unroll 8
     for (int kc = 0; kc < BK; ++kc) {
       const cplx<double> a = As[kc][tx], b = Bs[kc][ty];
@@ -75,6 +89,120 @@ __global__ void __launch_bounds__(256) k_zgemm_batch(const GBatch* __restrict__ 
     cplx<double> r = cmul(alpha, acc);
     if (use_beta) r = r + cmul(beta, *cp);
     *cp = r;
+  }
+}
+
+// C = alpha A B + beta C for products with a short inner dimension (k <= SK_KMAX) and few
+// columns (n <= SK_NMAX): one thread per row of C, 256 rows per workgroup; B (k x n) is
+// staged in LDS and read as a broadcast, the row's k entries of A are loaded together (one
+// memory latency per workgroup instead of one per k-chunk).  Workgroup w takes row block
+// w % bpp of product w / bpp (bpp: row blocks of the tallest product); blocks past a
+// product's rows return at once.
+constexpr int SK_KMAX = 64, SK_NMAX = 32, SK_NC = 8;
+__global__ void __launch_bounds__(256) k_zgemm_batch_nn_rows(const GBatch* __restrict__ gb, int bpp,
+                                                             cplx<double> alpha, cplx<double> beta, int use_beta) {
+  __shared__ cplx<double> Bs[SK_KMAX * SK_NMAX];
+  const GBatch& g = gb[blockIdx.x / bpp];
+  const int rb = blockIdx.x % bpp;
+  if (rb * 256 >= g.m) return;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < g.k * g.n; e += 256) {
+    const int kk = e % g.k, j = e / g.k;
+    Bs[kk * SK_NMAX + j] = g.B[(long)j * g.ldb + kk];
+  }
+  __syncthreads();
+  const int i = rb * 256 + tid;
+  if (i >= g.m) return;
+  for (int j0 = 0; j0 < g.n; j0 += SK_NC) {
+    cplx<double> acc[SK_NC];
+#pragma unroll
+    for (int jj = 0; jj < SK_NC; ++jj) acc[jj] = mk(0.0, 0.0);
+#pragma unroll 8
+    for (int kk = 0; kk < g.k; ++kk) {
+      const cplx<double> a = g.A[(long)kk * g.lda + i];
+#pragma unroll
+      for (int jj = 0; jj < SK_NC; ++jj) {
+        const cplx<double> b = Bs[kk * SK_NMAX + j0 + jj];   // columns past n hold stale values, never stored
+        acc[jj].x += a.x * b.x - a.y * b.y;
+        acc[jj].y += a.x * b.y + a.y * b.x;
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < SK_NC; ++jj) {
+      const int j = j0 + jj;
+      if (j < g.n) {
+        cplx<double>* cp = g.C + (long)j * g.ldc + i;
+        cplx<double> r = cmul(alpha, acc[jj]);
+        if (use_beta) r = r + cmul(beta, *cp);
+        *cp = r;
+      }
+    }
+  }
+}
+
+// C = alpha A^H B + beta C for small products (m, n <= SC_MAX) with a long inner dimension:
+// one workgroup per product, every output of C owned by one thread (up to SC_OPT each), the
+// k-chunks of A and B staged through LDS with the next chunk loaded into registers while
+// the current one is multiplied.
+constexpr int SC_MAX = 32, SC_KC = 32, SC_OPT = SC_MAX * SC_MAX / 256;
+__global__ void __launch_bounds__(256) k_zgemm_batch_cn_small(const GBatch* __restrict__ gb, cplx<double> alpha,
+                                                              cplx<double> beta, int use_beta) {
+  __shared__ cplx<double> As[SC_KC][SC_MAX + 1];
+  __shared__ cplx<double> Bs[SC_KC][SC_MAX + 1];
+  const GBatch& g = gb[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int m = g.m, n = g.n;
+  cplx<double> acc[SC_OPT];
+#pragma unroll
+  for (int o = 0; o < SC_OPT; ++o) acc[o] = mk(0.0, 0.0);
+  constexpr int EPT = SC_KC * SC_MAX / 256;   // staged elements per thread and operand
+  cplx<double> ra[EPT], rb[EPT];
+  // element e of a chunk: k offset e % SC_KC (contiguous in memory), column e / SC_KC
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < EPT; ++s) {
+      const int e = tid + 256 * s, kc = e % SC_KC, col = e / SC_KC, kk = k0 + kc;
+      ra[s] = (kk < g.k && col < m) ? g.A[(long)col * g.lda + kk] : mk(0.0, 0.0);
+      rb[s] = (kk < g.k && col < n) ? g.B[(long)col * g.ldb + kk] : mk(0.0, 0.0);
+    }
+  };
+  if (g.k > 0) load(0);
+  for (int k0 = 0; k0 < g.k; k0 += SC_KC) {
+#pragma unroll
+    for (int s = 0; s < EPT; ++s) {
+      const int e = tid + 256 * s;
+      As[e % SC_KC][e / SC_KC] = ra[s];
+      Bs[e % SC_KC][e / SC_KC] = rb[s];
+    }
+    __syncthreads();
+    if (k0 + SC_KC < g.k) load(k0 + SC_KC);
+#pragma unroll
+    for (int o = 0; o < SC_OPT; ++o) {
+      const int q = tid + 256 * o;
+      if (q < m * n) {
+        const int i = q % m, j = q / m;
+        cplx<double> x = acc[o];
+#pragma unroll 8
+        for (int kc = 0; kc < SC_KC; ++kc) {
+          const cplx<double> a = As[kc][i], b = Bs[kc][j];   // conj(a) * b
+          x.x += a.x * b.x + a.y * b.y;
+          x.y += a.x * b.y - a.y * b.x;
+        }
+        acc[o] = x;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int o = 0; o < SC_OPT; ++o) {
+    const int q = tid + 256 * o;
+    if (q < m * n) {
+      const int i = q % m, j = q / m;
+      cplx<double>* cp = g.C + (long)j * g.ldc + i;
+      cplx<double> r = cmul(alpha, acc[o]);
+      if (use_beta) r = r + cmul(beta, *cp);
+      *cp = r;
+    }
   }
 }
 
@@ -142,21 +270,42 @@ void free_desc(pw_ctx* c) {
 int zgemm_batch(pw_ctx* c, int transa, const std::vector<GBatch>& g0, cplx<double> alpha, cplx<double> beta) {
   std::vector<GBatch> g;
   g.reserve(g0.size());
-  long tiles = 0;
+  int maxm = 0, maxn = 0, maxk = 0;
   for (const GBatch& x : g0) {
     if (x.m <= 0 || x.n <= 0) continue;
-    GBatch y = x;
-    y.tile0 = (int)tiles;
-    tiles += (long)((x.m + BT - 1) / BT) * ((x.n + BT - 1) / BT);
-    g.push_back(y);
+    g.push_back(x);
+    maxm = std::max(maxm, x.m);
+    maxn = std::max(maxn, x.n);
+    maxk = std::max(maxk, x.k);
   }
   if (g.empty()) return PW_OK;
+  const int ub = (beta.x != 0.0 || beta.y != 0.0) ? 1 : 0;
+  // the shapes of the block eigensolvers' products: A B with a short inner dimension and few
+  // columns (the Ritz-vector updates), A^H B with a small result (the Gram matrices)
+  const int kind = (!transa && maxk <= SK_KMAX && maxn <= SK_NMAX && c->batch_gemm_small) ? 1
+                   : (transa && maxm <= SC_MAX && maxn <= SC_MAX && c->batch_gemm_small) ? 2 : 0;
+  long tiles = 0;
+  const int bpp = (maxm + 255) / 256;   // row blocks per product (kind 1)
+  for (GBatch& y : g) {
+    y.tile0 = (int)tiles;
+    if (kind == 1) tiles += bpp;
+    else if (kind == 2) tiles += 1;
+    else tiles += (long)((y.m + BT - 1) / BT) * ((y.n + BT - 1) / BT);
+  }
   if (tiles > 0x7fffffffL) return set_error(c, PW_ERR_ARG, "batched GEMM too large for one launch");
   const void* dptr = nullptr;
   int rc = desc_get(c, g.data(), g.size() * sizeof(GBatch), &dptr);
   if (rc) return rc;
   const GBatch* d = reinterpret_cast<const GBatch*>(dptr);
-  const int ub = (beta.x != 0.0 || beta.y != 0.0) ? 1 : 0;
+  if (kind == 1) {
+    hipLaunchKernelGGL(k_zgemm_batch_nn_rows, dim3((unsigned)tiles), dim3(256), 0, c->stream, d, bpp, alpha, beta,
+                       ub);
+    return hip_check(c, hipGetLastError(), "k_zgemm_batch_nn_rows");
+  }
+  if (kind == 2) {
+    hipLaunchKernelGGL(k_zgemm_batch_cn_small, dim3((unsigned)tiles), dim3(256), 0, c->stream, d, alpha, beta, ub);
+    return hip_check(c, hipGetLastError(), "k_zgemm_batch_cn_small");
+  }
   if (transa)
     hipLaunchKernelGGL(k_zgemm_batch<1>, dim3((unsigned)tiles), dim3(256), 0, c->stream, d, (int)g.size(), alpha,
                        beta, ub);

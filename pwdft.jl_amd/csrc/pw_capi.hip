@@ -417,6 +417,10 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       c->fft_order = value;
       return PW_OK;
     }
+    case PW_TUNE_BATCH_GEMM_SMALL:
+      if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: batched GEMM mode must be 0..1");
+      c->batch_gemm_small = value;
+      return PW_OK;
     case PW_TUNE_EIG_SYNC:
       if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver sync mode must be 0..1");
       c->eig_light = value;

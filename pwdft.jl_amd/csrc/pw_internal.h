@@ -259,6 +259,7 @@ struct pw_ctx {
   int fused_batch = 1;                                // batched op_H: 1 fused launch (register-line kernel where it
                                                       // applies), 2 fused launch with the whole box in LDS, 0 three stages
   int fft_order = 0;                                  // radix mode of the FFT plans (pw_tune_set PW_TUNE_FFT_ORDER)
+  int batch_gemm_small = 1;                           // batched GEMM: small-product kernels where they apply
   int fused_nt = 0;                                   // threads per fused workgroup (0: by LDS, 256/512/1024)
   // density side of an SCF iteration (pw_scf.hip)
   double* d_vps = nullptr;                            // V_Ps_loc(r) [N]

@@ -7,8 +7,8 @@
 // into a graph whole.
 //
 //  - k_heev_jacobi: Hermitian eigenpairs by cyclic Jacobi rotations in tournament order
-//    (n/2 disjoint rotations per round, all applied at once: columns, then rows), a pair
-//    rotated while |A_pq| > max(1e-15 sqrt|A_pp A_qq|, 1e-17 ||A||_F), until a sweep rotates
+//    (n/2 disjoint rotations per round, all applied at once), a pair
+//    rotated while |A_pq| > max(1e-15 sqrt|A_pp A_qq|, 2 eps ||A||_F), until a sweep rotates
 //    no pair; eigenvalues sorted ascending with their vectors.  Complex 2x2 step: the pair (p, q) with A_pq = |g| e^{i phi} is made
 //    real by diag(1, e^{-i phi}) and diagonalised by the classical rotation
 //    t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = (A_qq - A_pp) / (2 |g|).
@@ -40,22 +40,34 @@ __device__ inline double sm_block_sum(double v, double* red) {
 
 // Matrix b of the stack at A + b * stride (leading dimension lda): eigenvalues (ascending)
 // into w + b * n, the nvec lowest eigenvectors into the matrix's first nvec columns.
+// A <- J^H A J is applied per 2 x 2 block (rows of pair k, columns of pair k'; the pairs of
+// a round are disjoint, so every element belongs to one block): the columns' rotation, then
+// the rows', as a two-pass form would apply them, in one pass beside the update of V.
+__device__ inline void jac_rot(cplx<double>& x0, cplx<double>& x1, double c, double s, cplx<double> ph) {
+  // (x0, x1) <- (c x0 - s ph x1, s x0 + c ph x1)
+  const cplx<double> xp = x0, xq = cmul(x1, ph);
+  x0 = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
+  x1 = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+}
+
 __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict__ A, long lda, long stride, int n,
-                                                       double* __restrict__ w, int nvec, int* __restrict__ info) {
+                                                    double* __restrict__ w, int nvec, int* __restrict__ info) {
   extern __shared__ char smem_raw[];
   cplx<double>* a = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n
   cplx<double>* v = a + n * n;                                     // n x n
   __shared__ double rc[SM_NMAX / 2], rs[SM_NMAX / 2];
   __shared__ cplx<double> rph[SM_NMAX / 2];       // e^{-i phi} of each pair
-  __shared__ int rp[SM_NMAX / 2], rq[SM_NMAX / 2];
-  __shared__ double red[SM_NT / 64];
+  __shared__ int rp[SM_NMAX / 2], rq[SM_NMAX / 2];   // rq < 0: the index paired with the dummy player
   __shared__ double dsort[SM_NMAX];
   __shared__ int rank[SM_NMAX];
   cplx<double>* M = A + (long)blockIdx.x * stride;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int nn = n * n;
+  __shared__ double red[SM_NT / 64];
+  __shared__ int flag;
   // Hermitian(M): upper triangle, real diagonal
-  for (int t = tid; t < nn; t += SM_NT) {
+  double fro = 0.0;
+  for (int t = lane; t < nn; t += SM_NT) {
     const int i = t % n, j = t / n;
     cplx<double> x;
     if (i < j) x = M[(long)j * lda + i];
@@ -63,27 +75,29 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
     else { const cplx<double> u = M[(long)i * lda + j]; x = mk(u.x, -u.y); }
     a[i + n * j] = x;
     v[i + n * j] = mk(i == j ? 1.0 : 0.0, 0.0);
+    fro += x.x * x.x + x.y * x.y;
   }
-  __syncthreads();
-  double fro = 0.0;
-  for (int t = tid; t < nn; t += SM_NT) fro += a[t].x * a[t].x + a[t].y * a[t].y;
   fro = sm_block_sum(fro, red);
-  const double gfloor = 1e-17 * sqrt(fro);
+  // below 2 eps ||A||_F an off-diagonal entry is rounding noise of the rotations themselves
+  // (rotating it changes eigenvalues by less than the backward error of the decomposition);
+  // a tighter floor keeps pairs with small or disparate diagonals rotating on noise for
+  // sweep after sweep
+  const double gfloor = 2.0 * 2.220446049250313e-16 * sqrt(fro);
   const int m = (n + 1) / 2 * 2;     // players of the tournament (one dummy when n is odd)
   const int npair = m / 2;
-  int sweep = 0;
   bool converged = false;
-  for (; sweep < SM_SWEEPS; ++sweep) {
-    double nrot = 0.0;
+  for (int sweep = 0; sweep < SM_SWEEPS; ++sweep) {
+    if (lane == 0) flag = 0;
+    int nrot = 0;
     for (int r = 0; r < m - 1; ++r) {
-      if (tid < npair) {
+      if (lane < npair) {
         int p, q;
-        if (tid == 0) {
+        if (lane == 0) {
           p = m - 1;
           q = r;
         } else {
-          p = (r + tid) % (m - 1);
-          q = (r - tid + (m - 1)) % (m - 1);
+          p = (r + lane) % (m - 1);
+          q = (r - lane + (m - 1)) % (m - 1);
         }
         if (p > q) { const int t = p; p = q; q = t; }
         double c = 1.0, s = 0.0;
@@ -93,7 +107,7 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
           const double ag = sqrt(g.x * g.x + g.y * g.y);
           const double app = a[p + n * p].x, aqq = a[q + n * q].x;
           if (ag > 1e-15 * sqrt(fabs(app * aqq)) && ag > gfloor) {
-            nrot += 1.0;
+            ++nrot;
             const double tau = (aqq - app) / (2.0 * ag);
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -101,69 +115,82 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
             ph = mk(g.x / ag, -g.y / ag);   // e^{-i phi}
           }
         } else {
-          p = q = -1;
+          q = -1;   // p meets the dummy: not rotated this round
         }
-        rp[tid] = p;
-        rq[tid] = q;
-        rc[tid] = c;
-        rs[tid] = s;
-        rph[tid] = ph;
+        rp[lane] = p;
+        rq[lane] = q;
+        rc[lane] = c;
+        rs[lane] = s;
+        rph[lane] = ph;
       }
       __syncthreads();
-      // columns: A <- A J, V <- V J with J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] on (p, q)
-      for (int t = tid; t < npair * n; t += SM_NT) {
-        const int k = t / n, i = t % n;
-        const int p = rp[k], q = rq[k];
-        if (p < 0 || rs[k] == 0.0) continue;
-        const double c = rc[k], s = rs[k];
-        const cplx<double> ph = rph[k];
-        {
-          const cplx<double> xp = a[i + n * p], xq = cmul(a[i + n * q], ph);
-          a[i + n * p] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
-          a[i + n * q] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+      // A <- J^H A J, block (rows of pair k, columns of pair k2)
+      for (int t = lane; t < npair * npair; t += SM_NT) {
+        const int k = t % npair, k2 = t / npair;
+        const int p = rp[k], q = rq[k], p2 = rp[k2], q2 = rq[k2];
+        const double s = rs[k], s2 = rs[k2];
+        if (s == 0.0 && s2 == 0.0) continue;
+        const bool qv = q >= 0, q2v = q2 >= 0;
+        cplx<double> b00 = a[p + n * p2];
+        cplx<double> b01 = q2v ? a[p + n * q2] : mk(0.0, 0.0);
+        cplx<double> b10 = qv ? a[q + n * p2] : mk(0.0, 0.0);
+        cplx<double> b11 = (qv && q2v) ? a[q + n * q2] : mk(0.0, 0.0);
+        if (s2 != 0.0) {   // columns p2, q2
+          const double c2 = rc[k2];
+          const cplx<double> ph2 = rph[k2];
+          jac_rot(b00, b01, c2, s2, ph2);
+          jac_rot(b10, b11, c2, s2, ph2);
         }
-        {
-          const cplx<double> xp = v[i + n * p], xq = cmul(v[i + n * q], ph);
-          v[i + n * p] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
-          v[i + n * q] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+        if (s != 0.0) {    // rows p, q: e^{i phi}
+          const double c1 = rc[k];
+          const cplx<double> phc = mk(rph[k].x, -rph[k].y);
+          jac_rot(b00, b10, c1, s, phc);
+          jac_rot(b01, b11, c1, s, phc);
         }
+        a[p + n * p2] = b00;
+        if (q2v) a[p + n * q2] = b01;
+        if (qv) a[q + n * p2] = b10;
+        if (qv && q2v) a[q + n * q2] = b11;
       }
-      __syncthreads();
-      // rows: A <- J^H A, rows p, q: [c A_p - s e^{i phi} A_q, s A_p + c e^{i phi} A_q]
-      for (int t = tid; t < npair * n; t += SM_NT) {
-        const int k = t / n, j = t % n;
-        const int p = rp[k], q = rq[k];
-        if (p < 0 || rs[k] == 0.0) continue;
-        const double c = rc[k], s = rs[k];
-        const cplx<double> phc = mk(rph[k].x, -rph[k].y);   // e^{i phi}
-        const cplx<double> xp = a[p + n * j], xq = cmul(a[q + n * j], phc);
-        a[p + n * j] = mk(c * xp.x - s * xq.x, c * xp.y - s * xq.y);
-        a[q + n * j] = mk(s * xp.x + c * xq.x, s * xp.y + c * xq.y);
+      // V <- V J, columns of pair k2
+      for (int t = lane; t < npair * n; t += SM_NT) {
+        const int k2 = t / n, i = t % n;
+        const int p2 = rp[k2], q2 = rq[k2];
+        const double s2 = rs[k2];
+        if (q2 < 0 || s2 == 0.0) continue;
+        cplx<double> x0 = v[i + n * p2], x1 = v[i + n * q2];
+        jac_rot(x0, x1, rc[k2], s2, rph[k2]);
+        v[i + n * p2] = x0;
+        v[i + n * q2] = x1;
       }
       __syncthreads();
     }
-    if (sm_block_sum(nrot, red) == 0.0) {
+    if (nrot) flag = 1;   // any thread that rotated this sweep
+    __syncthreads();
+    const int more = flag;
+    __syncthreads();
+    if (!more) {
       converged = true;
       break;
     }
   }
   // eigenvalues: the diagonal; sort ascending (ties by index) with the vectors
-  if (tid < n) dsort[tid] = a[tid + n * tid].x;
+  if (lane < n) dsort[lane] = a[lane + n * lane].x;
   __syncthreads();
-  if (tid < n) {
-    const double d = dsort[tid];
+  if (lane < n) {
+    const double d = dsort[lane];
     int r = 0;
-    for (int j = 0; j < n; ++j) r += (dsort[j] < d || (dsort[j] == d && j < tid)) ? 1 : 0;
-    rank[tid] = r;
+    for (int j = 0; j < n; ++j) r += (dsort[j] < d || (dsort[j] == d && j < lane)) ? 1 : 0;
+    rank[lane] = r;
     w[(long)blockIdx.x * n + r] = d;
   }
   __syncthreads();
-  for (int t = tid; t < nn; t += SM_NT) {
+  for (int t = lane; t < nn; t += SM_NT) {
     const int i = t % n, j = t / n;
     const int r = rank[j];
     if (r < nvec) M[(long)r * lda + i] = v[i + n * j];
   }
-  if (tid == 0 && info) info[blockIdx.x] = converged ? 0 : 1;
+  if (lane == 0 && info) info[blockIdx.x] = converged ? 0 : 1;
 }
 
 // Matrix b of the stack: inv(chol(Hermitian(A)).U) in place, strict lower triangle zeroed;

@@ -28,6 +28,7 @@ PW_TUNE_LOBPCG_LOCK = 10
 PW_TUNE_FUSED_BATCH = 11
 PW_TUNE_LOBPCG_GRAPH = 12
 PW_TUNE_FUSED_NT = 13
+PW_TUNE_BATCH_GEMM_SMALL = 16
 
 PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
 

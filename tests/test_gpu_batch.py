@@ -89,12 +89,27 @@ def test_op_H_batch_terms_and_accumulate(al):
         assert np.abs(a.download() - f).max() < 1e-13 * np.abs(f).max()
 
 
-def test_zgemm_batch_vs_numpy():
+# batches of (m, n, k): mixed sizes (the 16 x 16-tile kernel), and batches whose every
+# product fits the small-product kernels (A B: k <= 64, n <= 32; A^H B: m, n <= 32)
+ZGEMM_BATCHES = {
+    0: [[(5, 8, 300), (24, 24, 311), (1, 1, 1), (17, 33, 5), (300, 8, 24), (40, 16, 4634)],
+        [(300, 8, 24), (5, 8, 16), (1, 1, 1), (517, 32, 64), (17, 3, 5), (256, 9, 33)]],
+    1: [[(5, 8, 300), (24, 24, 311), (1, 1, 1), (17, 33, 5), (300, 8, 24), (40, 16, 4634)],
+        [(24, 24, 311), (5, 8, 300), (1, 1, 1), (32, 32, 1000), (17, 3, 5), (8, 24, 33)]],
+}
+
+
+@pytest.mark.parametrize("small", [1, 0])
+def test_zgemm_batch_vs_numpy(small):
+    """pw_zgemm_batch_dev against numpy (1e-12), with the small-product kernels allowed (1)
+    and not (0), on batches that fit them and batches that do not."""
     import pwdft_hip as ph
+    from pwdft_hip._lib import PW_TUNE_BATCH_GEMM_SMALL
     ctx = ph.Context(0)
+    ctx.call("pw_tune_set", PW_TUNE_BATCH_GEMM_SMALL, small)
     rng = np.random.default_rng(4)
-    for transa in (0, 1):
-        shapes = [(5, 8, 300), (24, 24, 311), (1, 1, 1), (17, 33, 5), (300, 8, 24), (40, 16, 4634)]
+    for transa, shapes in [(t, sh) for t in (0, 1) for sh in ZGEMM_BATCHES[t]]:
+        print(f"case small={small} transa={transa} shapes={shapes}", flush=True)
         As, Bs, Cs, refs = [], [], [], []
         alpha = np.array([0.7, -0.2])
         beta = np.array([0.3, 0.1])
@@ -148,6 +163,36 @@ def test_heevd_batch_vs_numpy(n, nbatch):
         assert np.abs(A[k] @ V[k] - V[k] * w[k][None, :]).max() < 1e-11 * max(1.0, np.abs(ref).max())
     with pytest.raises(PWError):
         c.call("pw_heevd_batch_dev", n, nbatch, d.p, n, n * n - 1, ptr(w))
+    d.free()
+    c.close()
+
+
+def test_heevd_batch_graded_spectrum():
+    """Matrices like the Rayleigh-Ritz matrices of a batched LOBPCG step: eigenvalues near
+    zero beside ones of order 1-10 (so the diagonal products that scale the Jacobi threshold
+    are tiny for some pairs).  The batched eigensolver converges (a non-converged matrix
+    raises) and matches numpy to the same tolerance as random spectra."""
+    from pwdft_hip._lib import Context, DeviceMatrix, ptr
+    c = Context(0)
+    rng = np.random.default_rng(77)
+    n, nbatch = 24, 64
+    lam = np.concatenate([[-3e-7, 1e-9, 2e-6, -1e-4], np.linspace(-0.3, 0.8, 12), np.geomspace(1.0, 9.0, 8)])
+    A = np.empty((nbatch, n, n), dtype=complex)
+    for k in range(nbatch):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        A[k] = (Q * rng.permutation(lam)[None, :]) @ Q.conj().T
+        A[k] = 0.5 * (A[k] + A[k].conj().T)
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    d = DeviceMatrix.from_host(c, buf.reshape(-1, 1))
+    w = np.empty((nbatch, n))
+    print("case heevd_batch graded spectrum", flush=True)
+    c.call("pw_heevd_batch_dev", n, nbatch, d.p, n, n * n, ptr(w))
+    V = np.transpose(d.download().reshape(nbatch, n, n), (0, 2, 1))
+    ref = np.linalg.eigvalsh(A)
+    assert np.abs(w - ref).max() < 1e-12 * np.abs(ref).max()
+    for k in range(nbatch):
+        assert np.abs(V[k].conj().T @ V[k] - np.eye(n)).max() < 1e-12
+        assert np.abs(A[k] @ V[k] - V[k] * w[k][None, :]).max() < 1e-11 * np.abs(ref).max()
     d.free()
     c.close()
 
