@@ -100,18 +100,27 @@ enum {
                                    1024 threads (whole-box kernel), 256 (register-line kernel,
                                    which takes 256 or 512) */
   PW_TUNE_EIG_SYNC = 14,        /* the device tridiagonal reduction's cross-workgroup exchanges:
-                                   1 (default) device-coherent atomic accesses, each producer
-                                   waiting for its stores before the flag, no cache flushes
-                                   (n = 480: 10.8 ms, n = 960: 21.4 ms); 0 plain accesses
+                                   3 (default) by order: 2 up to n = 640, else 1;
+                                   2 one exchange per column (the owner of the next
+                                   column publishes it with its p segments and every
+                                   workgroup forms the next reflector itself), device-coherent
+                                   atomic accesses; 1 two exchanges per column (the owner
+                                   publishes the next reflector), device-coherent atomic
+                                   accesses, each producer waiting for its stores before the
+                                   flag (n = 480: 10.8 ms, n = 960: 21.4 ms); 0 two exchanges
                                    ordered by agent-scope fences (12.2, 33.4 ms) */
   PW_TUNE_FFT_ORDER = 15,       /* radix plan of the FFT lines: 0 (default) largest radices
                                    first (72 = 9 x 8, compile-time specialised); 1 the same
                                    radices in reverse order; 2 radices 5, 4, 3, 2 only */
-  PW_TUNE_BATCH_GEMM_SMALL = 16 /* pw_zgemm_batch_dev and the batched solvers: 1 (default) the
+  PW_TUNE_BATCH_GEMM_SMALL = 16, /* pw_zgemm_batch_dev and the batched solvers: 1 (default) the
                                    small-product kernels (A B with k <= 64 and n <= 32: one
                                    thread per row; A^H B with m, n <= 32: one workgroup per
                                    product) where every product of the batch fits them; 0: the
                                    16 x 16-tile kernel for all */
+  PW_TUNE_EIG_WG = 17           /* the device tridiagonal reduction: at least this many
+                                   workgroups share the matrix (default 64: n = 320 5.2 -> 4.2
+                                   ms, n = 480 7.5 -> 6.7 ms for the whole eigensolve); 0: the
+                                   fewest whose column shares fit in LDS, at least 8 */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

@@ -421,8 +421,12 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: batched GEMM mode must be 0..1");
       c->batch_gemm_small = value;
       return PW_OK;
+    case PW_TUNE_EIG_WG:
+      if (value < 0 || value > 256) return set_error(c, PW_ERR_ARG, "pw_tune_set: reduction workgroups must be 0..256");
+      c->eig_min_wg = value;
+      return PW_OK;
     case PW_TUNE_EIG_SYNC:
-      if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver sync mode must be 0..1");
+      if (value < 0 || value > 3) return set_error(c, PW_ERR_ARG, "pw_tune_set: eigensolver sync mode must be 0..3");
       c->eig_light = value;
       return PW_OK;
     case PW_TUNE_FUSED_NT:

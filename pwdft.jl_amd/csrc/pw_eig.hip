@@ -119,6 +119,7 @@ __device__ inline double block_sum(double v, double* red) {
 struct HetrdShared {
   int ok;
   double tr, ti;  // tau_i
+  double beta;
 };
 
 // Householder vector of column i (owner only; column in LDS, rows i+1..n-1 below the
@@ -303,6 +304,193 @@ __global__ void __launch_bounds__(HT_NT) k_hetrd_dist(const cplx<double>* __rest
   if ((n - 1) % P == w && tid == 0) dd[n - 1] = Acol[(long)((n - 1) / P) * n + (n - 1)].x;
 }
 
+// ZLARFG in place on col[i+1 .. n-1] (alpha = col[i+1], x = col[i+2 ..]): col becomes v_i
+// (col[i+1] = 1), tau_i and beta into *sh.  Every thread of the workgroup calls it; it ends
+// with a barrier.
+__device__ void hetrd_reflector_ip(cplx<double>* col, int i, int n, double* red, HetrdShared* sh) {
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int r = i + 2 + tid; r < n; r += HT_NT) s += col[r].x * col[r].x + col[r].y * col[r].y;
+  const double xn2 = block_sum(s, red);
+  const cplx<double> alpha = col[i + 1];
+  double beta, tr, ti;
+  cplx<double> scale;
+  if (xn2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tr = ti = 0.0;
+    scale = mk(0.0, 0.0);
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    tr = (beta - alpha.x) / beta;
+    ti = -alpha.y / beta;
+    const double ar = alpha.x - beta, ai = alpha.y;
+    const double den = ar * ar + ai * ai;
+    scale = mk(ar / den, -ai / den);
+  }
+  __syncthreads();   // every thread has read alpha before col[i + 1] is overwritten
+  for (int r = i + 1 + tid; r < n; r += HT_NT) col[r] = (r == i + 1) ? mk(1.0, 0.0) : cmul(col[r], scale);
+  if (tid == 0) {
+    sh->tr = tr;
+    sh->ti = ti;
+    sh->beta = beta;
+  }
+  __syncthreads();
+}
+
+// The same reduction with one cross-workgroup exchange per column (PW_TUNE_EIG_SYNC 2).
+// Beside its p segments, the owner of column i + 1 publishes that column as it stands before
+// step i's update; after the gather every workgroup applies step i's rank-2 update to that
+// one column and forms v_{i+1} itself -- the same data and the same arithmetic in every
+// workgroup, so the same bits -- instead of waiting for the owner to publish the reflector.
+// v_0 comes from column 0 of A, which every workgroup reads.  Workgroup 0 writes V, tau, d
+// and e for the eigenvector back-transform and ZSTEDC.
+// pg: 2 x n published p, sp: 2 x P x 2 partial sums, cg: 2 x n published next column (all by
+// step parity); sync[1]: arrivals.
+template <bool LIGHT>
+__global__ void __launch_bounds__(HT_NT) k_hetrd_dist1(const cplx<double>* __restrict__ A, long lda, int n,
+                                                       int ncl_max, cplx<double>* __restrict__ V, long ldv,
+                                                       cplx<double>* __restrict__ tau, double* __restrict__ dd,
+                                                       double* __restrict__ ee, cplx<double>* pg, double* sp,
+                                                       cplx<double>* cg, unsigned* sync, int* status) {
+  extern __shared__ char smem_raw[];
+  const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  cplx<double>* Acol = reinterpret_cast<cplx<double>*>(smem_raw);
+  cplx<double>* vb = Acol + (size_t)ncl_max * n;   // v_i
+  cplx<double>* wb = vb + n;                        // w_i
+  cplx<double>* vn = wb + n;                        // column i + 1, then v_{i+1}
+  double* red = reinterpret_cast<double*>(vn + n);
+  __shared__ HetrdShared sh;
+  const int ncl = w < n ? (n - 1 - w) / P + 1 : 0;
+
+  for (long t = tid; t < (long)ncl * n; t += HT_NT) {
+    const int lc = (int)(t / n), r = (int)(t % n);
+    const int j = w + lc * P;
+    cplx<double> a;
+    if (r < j) {
+      a = A[(long)j * lda + r];
+    } else if (r == j) {
+      a = mk(A[(long)j * lda + r].x, 0.0);
+    } else {
+      const cplx<double> u = A[(long)r * lda + j];
+      a = mk(u.x, -u.y);
+    }
+    Acol[(long)lc * n + r] = a;
+  }
+  // column 0 of Hermitian(A) below the diagonal (conjugated row 0 of the upper triangle)
+  for (int r = 1 + tid; r < n; r += HT_NT) {
+    const cplx<double> u = A[(long)r * lda];
+    vn[r] = mk(u.x, -u.y);
+  }
+  __syncthreads();
+  hetrd_reflector_ip(vn, 0, n, red, &sh);
+  if (w == 0) {
+    for (int r = 1 + tid; r < n; r += HT_NT) V[r] = vn[r];
+    if (tid == 0) {
+      tau[0] = mk(sh.tr, sh.ti);
+      ee[0] = sh.beta;
+      dd[0] = A[0].x;
+    }
+  }
+  {
+    cplx<double>* t = vb;
+    vb = vn;
+    vn = t;
+  }
+
+  for (int i = 0; i <= n - 2; ++i) {
+    const int par = i & 1;
+    const cplx<double> taui = mk(sh.tr, sh.ti);
+    // p_j = tau A22[:, j]^H v for the owned columns j > i; the owner of column i + 1
+    // publishes it as well
+    const int lc0 = (i + 1 <= w) ? 0 : (i + 1 - w + P - 1) / P;
+    double spr = 0.0, spi = 0.0;
+    for (int lc = lc0 + wave; lc < ncl; lc += HT_NT / 64) {
+      const int j = w + lc * P;
+      const cplx<double>* col = Acol + (long)lc * n;
+      double ar = 0.0, ai = 0.0;
+      for (int r = i + 1 + lane; r < n; r += 64) {
+        const cplx<double> a = col[r], v = vb[r];
+        ar += a.x * v.x + a.y * v.y;
+        ai += a.x * v.y - a.y * v.x;
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      const cplx<double> p = cmul(taui, mk(ar, ai));
+      if (lane == 0) {
+        xstore<LIGHT>(pg + (long)par * n + j, p);
+        const cplx<double> v = vb[j];
+        spr += p.x * v.x + p.y * v.y;
+        spi += p.x * v.y - p.y * v.x;
+      }
+    }
+    if (i + 1 <= n - 2 && (i + 1) % P == w) {
+      const cplx<double>* col = Acol + (long)((i + 1) / P) * n;
+      for (int r = i + 1 + tid; r < n; r += HT_NT) xstore<LIGHT>(cg + (long)par * n + r, col[r]);
+    }
+    {
+      double* red2 = red + HT_NT / 64;
+      __syncthreads();
+      if (lane == 0) {
+        red[wave] = spr;
+        red2[wave] = spi;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < HT_NT / 64; ++q) {
+          a += red[q];
+          b += red2[q];
+        }
+        xstore<LIGHT>(sp + ((long)par * P + w) * 2 + 0, a);
+        xstore<LIGHT>(sp + ((long)par * P + w) * 2 + 1, b);
+      }
+    }
+    xpublish<LIGHT>();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) sh.ok = wait_geq(&sync[1], (unsigned)P * (unsigned)(i + 1), status) ? 1 : 0;
+    __syncthreads();
+    if (!sh.ok) return;
+    xacquire<LIGHT>();
+    double sr = 0.0, si = 0.0;
+    for (int q = tid; q < P; q += HT_NT) {
+      sr += xload<LIGHT>(sp + ((long)par * P + q) * 2 + 0);
+      si += xload<LIGHT>(sp + ((long)par * P + q) * 2 + 1);
+    }
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    const cplx<double> alpha2 = cmul(mk(-0.5 * taui.x, -0.5 * taui.y), mk(sr, si));
+    for (int r = i + 1 + tid; r < n; r += HT_NT) wb[r] = xload<LIGHT>(pg + (long)par * n + r) + cmul(alpha2, vb[r]);
+    const bool more = i + 1 <= n - 2;
+    if (more)
+      for (int r = i + 1 + tid; r < n; r += HT_NT) vn[r] = xload<LIGHT>(cg + (long)par * n + r);
+    __syncthreads();
+    // step i's update of the owned columns j > i, and of the next column in every workgroup
+    for (int lc = lc0; lc < ncl; ++lc) hetrd_update_col(Acol + (long)lc * n, w + lc * P, i, n, vb, wb);
+    if (more) {
+      hetrd_update_col(vn, i + 1, i, n, vb, wb);
+      __syncthreads();
+      const double d1 = vn[i + 1].x;
+      hetrd_reflector_ip(vn, i + 1, n, red, &sh);
+      if (w == 0) {
+        for (int r = i + 2 + tid; r < n; r += HT_NT) V[(long)(i + 1) * ldv + r] = vn[r];
+        if (tid == 0) {
+          tau[i + 1] = mk(sh.tr, sh.ti);
+          ee[i + 1] = sh.beta;
+          dd[i + 1] = d1;
+        }
+      }
+      cplx<double>* t = vb;
+      vb = vn;
+      vn = t;
+    }
+    __syncthreads();
+  }
+  if ((n - 1) % P == w && tid == 0) dd[n - 1] = Acol[(long)((n - 1) / P) * n + (n - 1)].x;
+}
+
 // Back-transform by blocks of WY_NB reflectors (compact WY form, LAPACK ZLARFT / ZLARFB,
 // forward, column-wise): H(i0) H(i0+1) ... H(i0+nr-1) = I - V_b T_b V_b^H with T_b upper
 // triangular, T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] (V_b[:, 0:j]^H v_j).  Rows of
@@ -433,15 +621,17 @@ __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
 }
 
 // Workgroups of the distributed reduction for order n: the fewest whose column share,
-// the two staging vectors and the reduction scratch fit in LDS; 0 when n is out of range.
-int hetrd_workgroups(int n, int* ncl_max, size_t* lds_bytes) {
+// the three staging vectors and the reduction scratch fit in LDS; 0 when n is out of range.
+int hetrd_workgroups(int n, int* ncl_max, size_t* lds_bytes, int min_wg) {
   if (n < 2) return 0;
-  const size_t extra = 2 * (size_t)n * sizeof(cplx<double>) + 2 * (HT_NT / 64) * sizeof(double);
+  const size_t extra = 3 * (size_t)n * sizeof(cplx<double>) + 2 * (HT_NT / 64) * sizeof(double);
   if (extra >= HT_LDS) return 0;
   const long per = (long)((HT_LDS - extra) / ((size_t)n * sizeof(cplx<double>)));
   if (per < 1) return 0;
   int P = (int)((n + per - 1) / per);
-  P = std::max(P, std::min(n, 8));   // at least a few workgroups share the matrix-vector work
+  // more workgroups than LDS needs: each holds fewer columns, so the per-column work of
+  // every step (its share of p = A v and of the rank-2 update) shrinks
+  P = std::max(P, std::min(n, std::max(8, min_wg)));
   if (P > 256) return 0;
   const int ncl = (n + P - 1) / P;
   *ncl_max = ncl;
@@ -456,7 +646,7 @@ int heevd_dist_max_n() {
     const int mid = (lo + hi) / 2;
     int ncl;
     size_t b;
-    if (hetrd_workgroups(mid, &ncl, &b) > 0) {
+    if (hetrd_workgroups(mid, &ncl, &b, 0) > 0) {
       best = mid;
       lo = mid + 1;
     } else {
@@ -473,14 +663,15 @@ int heevd_dist_max_n() {
 int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info) {
   int ncl = 0;
   size_t lds = 0;
-  const int P = hetrd_workgroups(n, &ncl, &lds);
+  const int P = hetrd_workgroups(n, &ncl, &lds, c->eig_min_wg);
   if (P <= 0) return set_error(c, PW_ERR_ARG, "heevd_dist: order " + std::to_string(n) + " out of range");
-  // workspace: V (n x n), tau (n), pg (2n) complex; e (n), sp (4P) doubles; sync (2) + status
+  // workspace: V (n x n), T blocks, tau (n), pg (2n), cg (2n) complex; e (n), sp (4P)
+  // doubles; sync (2) + status
   const size_t vbytes = (size_t)n * n * sizeof(cplx<double>);
   const int nblk = (n - 1 + WY_NB - 1) / WY_NB;
   const size_t tbytes = (size_t)nblk * WY_NB * WY_NB * sizeof(cplx<double>);
   const size_t need =
-      vbytes + tbytes + 3 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
+      vbytes + tbytes + 5 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
   int rc = ensure(c, &c->d_eig, &c->eig_bytes, need);
   if (rc) return rc;
   char* base = reinterpret_cast<char*>(c->d_eig);
@@ -488,7 +679,8 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   cplx<double>* Tb = V + (size_t)n * n;
   cplx<double>* tau = Tb + (size_t)nblk * WY_NB * WY_NB;
   cplx<double>* pg = tau + n;
-  double* E = reinterpret_cast<double*>(pg + 2 * (size_t)n);
+  cplx<double>* cg = pg + 2 * (size_t)n;
+  double* E = reinterpret_cast<double*>(cg + 2 * (size_t)n);
   double* sp = E + n;
   unsigned* sync = reinterpret_cast<unsigned*>(sp + 4 * (size_t)P);
   int* status = reinterpret_cast<int*>(sync + 2);
@@ -501,8 +693,17 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   double* dd = d_w;
   void* args[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
                   (void*)&dd, (void*)&E, (void*)&pg, (void*)&sp, (void*)&sync, (void*)&status};
-  const void* kern = c->eig_light ? (const void*)k_hetrd_dist<true> : (const void*)k_hetrd_dist<false>;
-  e = hipLaunchCooperativeKernel(kern, dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
+  void* args1[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
+                   (void*)&dd, (void*)&E, (void*)&pg, (void*)&sp, (void*)&cg, (void*)&sync, (void*)&status};
+  // by order: the single-exchange form is faster up to n ~ 480 (n = 160: 2.33 -> 2.19 ms),
+  // slower at n = 960 (15.8 -> 16.6 ms), where each step's local work outweighs an exchange
+  const int mode = c->eig_light == 3 ? (n <= 640 ? 2 : 1) : c->eig_light;
+  if (mode == 2)
+    e = hipLaunchCooperativeKernel((const void*)k_hetrd_dist1<true>, dim3(P), dim3(HT_NT), args1, (unsigned)lds,
+                                   c->stream);
+  else
+    e = hipLaunchCooperativeKernel(mode ? (const void*)k_hetrd_dist<true> : (const void*)k_hetrd_dist<false>,
+                                   dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "k_hetrd_dist (cooperative launch)");
   // the timeout status of the reduction is folded into d_info below
   rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);

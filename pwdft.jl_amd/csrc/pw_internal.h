@@ -244,7 +244,8 @@ struct pw_ctx {
   void* d_dwork = nullptr; size_t dwork_bytes = 0;
   void* d_eig = nullptr;   size_t eig_bytes = 0;     // reflectors and exchange buffers of heevd_dist
   int eig_path = 0;                                   // 0: device reduction where it fits, 1: rocSOLVER only
-  int eig_light = 1;                                  // device reduction: exchanges as device-coherent atomics (no cache flushes)
+  int eig_min_wg = 64;                                // device reduction: minimum workgroups (0: fewest that fit LDS)
+  int eig_light = 3;                                  // device reduction: exchanges as device-coherent atomics (no cache flushes)
   void* d_lob = nullptr;   size_t lob_bytes = 0;     // LOBPCG blocks and small matrices (pw_lobpcg.hip)
   void* d_lob2 = nullptr;  size_t lob2_bytes = 0;    // LOBPCG lock-branch blocks
   int lobpcg_lock_exp = 0;                            // 0: lock at 10 eps; k: at 10 * 10^-k

@@ -10,6 +10,7 @@ pytestmark = pytest.mark.gpu
 
 PW_TUNE_EIG_PATH = 9
 PW_TUNE_EIG_SYNC = 14
+PW_TUNE_EIG_WG = 17
 
 
 @pytest.fixture(scope="module")
@@ -83,20 +84,44 @@ def test_heevd_values_only_and_rocsolver_path(ctx):
 
 @pytest.mark.parametrize("n", [33, 160, 480, 960])
 def test_heevd_fenced_exchanges(ctx, n):
-    """The reduction with its cross-workgroup exchanges ordered by agent-scope fences
-    (PW_TUNE_EIG_SYNC = 0) against numpy and against the default device-coherent exchanges."""
+    """The reduction with two exchanges per column, ordered by agent-scope fences
+    (PW_TUNE_EIG_SYNC = 0) or as device-coherent atomics (1), and with one exchange per
+    column (2), against numpy and against the default choice by order (3)."""
     rng = np.random.default_rng(n + 7)
     A = _herm(rng, n, "random")
-    print(f"case heevd n={n} fenced exchanges", flush=True)
-    ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, 0)
-    try:
-        w, X = _eig(ctx, A)
-    finally:
-        ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, 1)
     w0, _ = _eig(ctx, A)
     ref = np.linalg.eigvalsh(A)
     scale = max(1.0, np.abs(ref).max())
-    assert np.abs(w - ref).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
-    assert np.abs(w - w0).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
-    assert np.abs(X.conj().T @ X - np.eye(n)).max() < 1e-12 * max(1.0, np.sqrt(n) / 4)
-    assert np.abs(A @ X - X * w).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+    for mode in (0, 1, 2):
+        print(f"case heevd n={n} exchange mode {mode}", flush=True)
+        ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, mode)
+        try:
+            w, X = _eig(ctx, A)
+        finally:
+            ctx.call("pw_tune_set", PW_TUNE_EIG_SYNC, 3)
+        assert np.abs(w - ref).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(w - w0).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(X.conj().T @ X - np.eye(n)).max() < 1e-12 * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(A @ X - X * w).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+
+
+@pytest.mark.parametrize("n", [100, 480])
+def test_heevd_workgroup_counts(ctx, n):
+    """The reduction shared by the fewest workgroups that fit (PW_TUNE_EIG_WG = 0), by 32 and
+    by 128, against numpy and against the default (64)."""
+    rng = np.random.default_rng(n + 11)
+    A = _herm(rng, n, "degenerate")
+    w0, _ = _eig(ctx, A)
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    for wg in (0, 32, 128):
+        print(f"case heevd n={n} workgroups >= {wg}", flush=True)
+        ctx.call("pw_tune_set", PW_TUNE_EIG_WG, wg)
+        try:
+            w, X = _eig(ctx, A)
+        finally:
+            ctx.call("pw_tune_set", PW_TUNE_EIG_WG, 64)
+        assert np.abs(w - ref).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(w - w0).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(X.conj().T @ X - np.eye(n)).max() < 1e-12 * max(1.0, np.sqrt(n) / 4)
+        assert np.abs(A @ X - X * w).max() < 1e-12 * scale * max(1.0, np.sqrt(n) / 4)
