@@ -104,15 +104,18 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
         cplx<double> ph = mk(1.0, 0.0);
         if (q < n) {
           const cplx<double> g = a[p + n * q];
-          const double ag = sqrt(g.x * g.x + g.y * g.y);
+          const double ag2 = g.x * g.x + g.y * g.y;
           const double app = a[p + n * p].x, aqq = a[q + n * q].x;
-          if (ag > 1e-15 * sqrt(fabs(app * aqq)) && ag > gfloor) {
+          // the tests of |A_pq| on squares, and 1 / |A_pq| by one reciprocal square root: the
+          // rotation parameters are a serial chain on every round's critical path
+          if (ag2 > 1e-30 * fabs(app * aqq) && ag2 > gfloor * gfloor) {
             ++nrot;
-            const double tau = (aqq - app) / (2.0 * ag);
+            const double rag = rsqrt(ag2);   // 1 / |A_pq|
+            const double tau = (aqq - app) * (0.5 * rag);
             const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
+            c = rsqrt(1.0 + t * t);
             s = t * c;
-            ph = mk(g.x / ag, -g.y / ag);   // e^{-i phi}
+            ph = mk(g.x * rag, -g.y * rag);   // e^{-i phi}
           }
         } else {
           q = -1;   // p meets the dummy: not rotated this round
