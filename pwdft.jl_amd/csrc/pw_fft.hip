@@ -274,6 +274,17 @@ __device__ __attribute__((noinline)) void line_static(LDS_AS cplx<T>* lds, int b
   if constexpr (R2 > 1) spass<T, S, N, R0 * R1, R2, L>(lds, base, es, tw, q, active, vpre, false);
 }
 
+// Radix of pass p from the radices packed 4 bits each (every radix is at most 9): indexing
+// the by-value FftDim argument with a run-time p would copy the struct into per-lane scratch
+// memory at kernel entry (and the compiler turns selects between constant-index loads back
+// into such an index).
+__device__ inline int pass_radix(const FftDim& d, int p) {
+  unsigned code = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) code |= (unsigned)d.radix[i] << (4 * i);
+  return (int)((code >> (4 * p)) & 15u);
+}
+
 // One line through all radix passes.  vpre (length n, or null) multiplies the input
 // element-wise before the first pass.
 template <typename T, int S, int ME>
@@ -295,7 +306,7 @@ __device__ inline void wave_line_fft(cplx<T>* lds, int base, int es, const FftDi
   }
   int Ns = 1;
   for (int p = 0; p < d.npass; ++p) {
-    const int R = d.radix[p];
+    const int R = pass_radix(d, p);
     // global V row for the first pass only; the flag travels separately from the pointer
     const bool usev = (p == 0) && (vpre != nullptr);
     const T* vp = vpre;
