@@ -117,10 +117,17 @@ enum {
                                    thread per row; A^H B with m, n <= 32: one workgroup per
                                    product) where every product of the batch fits them; 0: the
                                    16 x 16-tile kernel for all */
-  PW_TUNE_EIG_WG = 17           /* the device tridiagonal reduction: at least this many
+  PW_TUNE_EIG_WG = 17,          /* the device tridiagonal reduction: at least this many
                                    workgroups share the matrix (default 64: n = 320 5.2 -> 4.2
                                    ms, n = 480 7.5 -> 6.7 ms for the whole eigensolve); 0: the
                                    fewest whose column shares fit in LDS, at least 8 */
+  PW_TUNE_EIG_TRI = 18          /* eigenpairs of the reduced tridiagonal matrix: 1 (default)
+                                   only the wanted lowest ones on the device (multisection
+                                   Sturm bisection, inverse iteration, orthogonalisation and
+                                   Rayleigh-Ritz within runs of close eigenvalues) in the
+                                   solvers and pw_heev_lowest_dev, rocSOLVER ZSTEDC for
+                                   pw_heevd_dev; 2 the device path for pw_heevd_dev as well;
+                                   0 ZSTEDC everywhere */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 
@@ -397,6 +404,11 @@ int pw_trtri_dev(pw_ctx* ctx, int n, void* A, int64_t lda);
 int pw_heevd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
 /* eigvals(Hermitian(A)) only (A is destroyed). */
 int pw_heev_values_dev(pw_ctx* ctx, int n, void* A, int64_t lda, double* w);
+/* The m lowest eigenpairs of Hermitian(A), as diag_LOBPCG! keeps the Nstates lowest Ritz
+ * pairs of eigen(Hermitian(T)) (diag_LOBPCG.jl:176-185): eigenvectors into A's first m
+ * columns (ascending), eigenvalues into w (host, m); the other columns of A are destroyed.
+ * 1 <= m <= n. */
+int pw_heev_lowest_dev(pw_ctx* ctx, int n, int m, void* A, int64_t lda, double* w);
 int pw_hegvd_dev(pw_ctx* ctx, int n, void* A, int64_t lda, void* B, int64_t ldb, double* w);
 /* eigen(Hermitian(A_i)) for nbatch matrices A_i = A + i*strideA (elements), each n x n with
  * leading dimension lda: the per-k-point Rayleigh-Ritz problems of diag_LOBPCG! run over

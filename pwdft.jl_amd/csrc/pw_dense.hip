@@ -170,7 +170,8 @@ int heev_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, double* d_w, int n
   if (c->eig_path == 0 && n <= small_dense_max_n()) return heev_small_batch(c, n, 1, A, lda, 0, d_w, nvec, d_info);
   int rc = ensure_solver(c);
   if (rc) return rc;
-  if (c->eig_path == 0 && n <= heevd_dist_max_n()) return heevd_dist(c, c->rb, n, A, lda, d_w, nvec, d_info);
+  if (c->eig_path == 0 && n <= heevd_dist_max_n())
+    return heevd_dist(c, c->rb, n, A, lda, d_w, nvec, d_info, c->eig_tri >= 1);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   return st_check(c, rocsolver_zheevd(h, nvec > 0 ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
                                       n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_w, E, d_info),
@@ -291,15 +292,26 @@ int pw_trtri_dev(pw_ctx* c, int n, void* A, int64_t lda) {
   return PW_OK;
 }
 
-static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool vectors);
+static int heevd_impl(pw_ctx* c, int n, int m, void* A, int64_t lda, double* w, bool vectors, bool lowest_api);
 
-int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) { return heevd_impl(c, n, A, lda, w, true); }
-
-int pw_heev_values_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
-  return heevd_impl(c, n, A, lda, w, false);
+int pw_heevd_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
+  return heevd_impl(c, n, n, A, lda, w, true, false);
 }
 
-static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool vectors) {
+int pw_heev_values_dev(pw_ctx* c, int n, void* A, int64_t lda, double* w) {
+  return heevd_impl(c, n, n, A, lda, w, false, false);
+}
+
+int pw_heev_lowest_dev(pw_ctx* c, int n, int m, void* A, int64_t lda, double* w) {
+  CTX_CHECK(c);
+  if (n < 1 || m < 1 || m > n) return set_error(c, PW_ERR_ARG, "pw_heev_lowest_dev: need 1 <= m <= n");
+  return heevd_impl(c, n, m, A, lda, w, true, true);
+}
+
+// Eigenvalues of the m lowest eigenpairs into w (host) and, when vectors, their eigenvectors
+// into A's first m columns.  lowest_api: the caller keeps only those m (the device
+// tridiagonal eigensolver may then compute just them).
+static int heevd_impl(pw_ctx* c, int n, int m, void* A, int64_t lda, double* w, bool vectors, bool lowest_api) {
   CTX_CHECK(c);
   hipSetDevice(c->device);
   if (n < 0 || !A || !w || lda < (n > 0 ? n : 1)) return set_error(c, PW_ERR_ARG, "pw_heevd_dev: bad arguments");
@@ -312,11 +324,13 @@ static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool ve
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   int* dinfo = reinterpret_cast<int*>(c->d_dinfo);
   if (c->eig_path == 0 && n <= heevd_dist_max_n()) {
+    const bool tri_dev = c->eig_tri == 2 || (lowest_api && c->eig_tri == 1);
     rc = n <= small_dense_max_n()
              ? heev_small_batch(c, n, 1, reinterpret_cast<cplx<double>*>(A), (long)lda, 0, D, vectors ? n : 0, dinfo)
-             : heevd_dist(c, c->rb, n, reinterpret_cast<cplx<double>*>(A), (long)lda, D, vectors ? n : 0, dinfo);
+             : heevd_dist(c, c->rb, n, reinterpret_cast<cplx<double>*>(A), (long)lda, D, vectors ? m : 0, dinfo,
+                          tri_dev);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipMemcpyAsync(w, D, m * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return hip_check(c, e, "eigenvalues download");
     int info = 0;
     rc = read_info(c, dinfo, &info);
@@ -329,7 +343,7 @@ static int heevd_impl(pw_ctx* c, int n, void* A, int64_t lda, double* w, bool ve
                                     n, reinterpret_cast<rocblas_double_complex*>(A), (int)lda, D, D + n, dinfo),
                 "zheevd");
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(w, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipMemcpyAsync(w, D, m * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "eigenvalues download");
   int info = 0;
   rc = read_info(c, dinfo, &info);

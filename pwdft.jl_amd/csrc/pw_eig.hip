@@ -12,7 +12,10 @@
 //     (A22 v)_j = A22[:, j]^H v needs only local columns) the p segments are gathered.  The
 //     owner of column i + 1 updates that column first and publishes v_{i+1} before it
 //     updates the rest of its columns (look-ahead).
-//  2. The eigenpairs of T through rocSOLVER's divide and conquer (ZSTEDC).
+//  2. The eigenpairs of T: for the block eigensolvers (which keep only the lowest third of
+//     a Rayleigh-Ritz problem) the wanted ones from the device kernels k_tri_bisect,
+//     k_tri_invit and k_tri_cluster below; otherwise every one through rocSOLVER's divide
+//     and conquer (ZSTEDC).
 //  3. k_larft + k_apply_q_wy: eigenvectors of A = Q (eigenvectors of T), the reflectors
 //     applied by blocks of 32 in compact WY form (I - V T V^H) to the wanted columns only,
 //     four columns per workgroup staged in LDS.
@@ -620,6 +623,426 @@ __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
   if (t < n) p[t] = v;
 }
 
+// ---- Lowest eigenpairs of the real symmetric tridiagonal T = tridiag(e, d, e) (the
+// reduction's output), for the block eigensolvers, which keep the Nst lowest of a Rayleigh-
+// Ritz problem of order 2 Nst or 3 Nst (diag_LOBPCG.jl:176-185): rocSOLVER's ZSTEDC computes
+// every eigenpair, the kernels below only the ones wanted.
+//  1. k_tri_bisect: one wavefront per eigenvalue index i.  Sturm counts (the number of
+//     negative pivots of the LDL^T factorization of T - x I, pivots below pivmin replaced by
+//     -pivmin) at 64 points of the bracket per round cut it 65-fold, from the Gershgorin
+//     interval to eps ||T|| in about 9 rounds.
+//  2. k_tri_invit: one wavefront per eigenvector: LU factorization of T - lambda I with
+//     partial pivoting (pivots below eps ||T|| raised to it) and two inverse-iteration
+//     solves from a hashed start vector, normalised.
+//  3. k_tri_cluster: one workgroup per run of eigenvalues closer than TE_CTOL ||T|| to their
+//     neighbour: the run's vectors are orthonormalised (modified Gram-Schmidt, twice), then
+//     T is diagonalised within their span (cyclic Jacobi on the projected matrix, chunks of
+//     at most TE_MAXC vectors), which gives the run's eigenvectors even when the eigenvalues
+//     are degenerate or nearly so.
+// TE_EXTRA eigenpairs past the wanted ones are computed so that a run crossing the cut is
+// resolved whole.
+constexpr int TE_EXTRA = 32;
+constexpr int TE_MAXC = 32;
+constexpr double TE_CTOL = 1e-5;
+constexpr double TE_EPS = 2.220446049250313e-16;
+constexpr double TE_SAFMIN = 2.2250738585072014e-308;
+constexpr int TE_CL_NT = 256;
+
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Gershgorin bounds and max e^2 of T, from LDS copies (one wavefront, every lane gets them).
+__device__ inline void tri_bounds(const double* sd, const double* se, int n, double& gl, double& gu,
+                                  double& emax2) {
+  const int lane = threadIdx.x & 63;
+  double lo = 1e308, hi = -1e308, em = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double el = j > 0 ? fabs(se[j - 1]) : 0.0, er = j < n - 1 ? fabs(se[j]) : 0.0;
+    lo = fmin(lo, sd[j] - el - er);
+    hi = fmax(hi, sd[j] + el + er);
+    em = fmax(em, er * er);
+  }
+  gl = wave_min(lo);
+  gu = wave_max(hi);
+  emax2 = wave_max(em);
+}
+
+// 1 / q to full double precision: the hardware reciprocal refined by two Newton steps (each
+// at least doubles the correct bits), a shorter dependency chain than IEEE division.
+__device__ inline double rcp_nr(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return r;
+}
+
+// Number of eigenvalues of T not above x (negative or zero LDL^T pivots of T - x I).
+__device__ inline int tri_sturm(const double* sd, const double* se2, int n, double x, double pivmin) {
+  double q = sd[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int cnt = q <= 0.0 ? 1 : 0;
+#pragma unroll 4
+  for (int j = 1; j < n; ++j) {
+    q = fma(-se2[j - 1], rcp_nr(q), sd[j] - x);
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q <= 0.0 ? 1 : 0;
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(64) k_tri_bisect(const double* __restrict__ d, const double* __restrict__ e,
+                                                   int n, double* __restrict__ w) {
+  extern __shared__ double tsh[];
+  double* sd = tsh;
+  double* se = tsh + n;
+  double* se2 = se + n;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  for (int j = lane; j < n; j += 64) {
+    sd[j] = d[j];
+    const double ej = j < n - 1 ? e[j] : 0.0;
+    se[j] = ej;
+    se2[j] = ej * ej;
+  }
+  __syncthreads();
+  double gl, gu, emax2;
+  tri_bounds(sd, se, n, gl, gu, emax2);
+  const double bnorm = fmax(fabs(gl), fabs(gu));
+  const double pivmin = TE_SAFMIN * fmax(1.0, emax2);
+  double lo = gl - 2.0 * TE_EPS * bnorm * n - 2.0 * pivmin;
+  double hi = gu + 2.0 * TE_EPS * bnorm * n + 2.0 * pivmin;
+  const double tol = TE_EPS * bnorm + pivmin;
+  // count(lo) <= i < count(hi): eigenvalue i (ascending, from 0) lies in (lo, hi]
+  for (int round = 0; round < 40 && hi - lo > tol; ++round) {
+    const double x = lo + (hi - lo) * (double)(lane + 1) * (1.0 / 65.0);
+    const int cnt = tri_sturm(sd, se2, n, x, pivmin);
+    const unsigned long long above = __ballot(cnt >= i + 1);
+    if (above == 0ull) {
+      lo = __shfl(x, 63, 64);
+    } else {
+      const int f = __ffsll((long long)above) - 1;
+      const double xf = __shfl(x, f, 64);
+      const double xp = __shfl(x, f > 0 ? f - 1 : 0, 64);
+      hi = xf;
+      if (f > 0) lo = xp;
+    }
+  }
+  if (lane == 0) w[i] = 0.5 * (lo + hi);
+}
+
+// 32-bit integer hash (two multiply-xorshift rounds) to a start-vector entry in [-1, 1).
+__device__ inline double tri_start(unsigned i, unsigned r) {
+  unsigned h = i * 0x9E3779B1u + r * 0x85EBCA77u + 0x165667B1u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
+__global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d, const double* __restrict__ e, int n,
+                                                  const double* __restrict__ w, cplx<double>* __restrict__ Z,
+                                                  long ldz) {
+  extern __shared__ double tsh[];
+  double* sd = tsh;          // d, then (after the factorization) 1 / u0
+  double* se = sd + n;       // e
+  double* sl = se + n;       // multipliers
+  double* su1 = sl + n;      // first superdiagonal of U
+  double* su2 = su1 + n;     // second superdiagonal of U (row swaps only)
+  double* sx = su2 + n;      // iterate
+  int* sw = reinterpret_cast<int*>(sx + n);   // row j swapped with row j + 1
+  const int i = blockIdx.x, lane = threadIdx.x;
+  for (int j = lane; j < n; j += 64) {
+    sd[j] = d[j];
+    se[j] = j < n - 1 ? e[j] : 0.0;
+    sx[j] = tri_start((unsigned)i, (unsigned)j);
+  }
+  __syncthreads();
+  double gl, gu, emax2;
+  tri_bounds(sd, se, n, gl, gu, emax2);
+  const double bnorm = fmax(fabs(gl), fabs(gu));
+  const double tiny = bnorm > 0.0 ? TE_EPS * bnorm : TE_SAFMIN;
+  const double lam = w[i];
+  if (lane == 0) {
+    // Gaussian elimination of T - lam I with partial pivoting; row j of U is
+    // (u0_j, su1_j, su2_j) on columns j, j + 1, j + 2.
+    double p = sd[0] - lam, q = n > 1 ? se[0] : 0.0;
+    for (int j = 0; j < n - 1; ++j) {
+      const double s = se[j];
+      const double a1 = sd[j + 1] - lam;
+      const double b1 = j + 1 < n - 1 ? se[j + 1] : 0.0;
+      double u0;
+      if (fabs(p) >= fabs(s)) {
+        if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
+        const double lj = s * rcp_nr(p);
+        u0 = p;
+        su1[j] = q;
+        su2[j] = 0.0;
+        sw[j] = 0;
+        sl[j] = lj;
+        p = a1 - lj * q;
+        q = b1;
+      } else {
+        const double lj = p * rcp_nr(s);
+        u0 = s;
+        su1[j] = a1;
+        su2[j] = b1;
+        sw[j] = 1;
+        sl[j] = lj;
+        p = q - lj * a1;
+        q = -lj * b1;
+      }
+      sd[j] = 1.0 / u0;
+    }
+    if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
+    sd[n - 1] = 1.0 / p;
+  }
+  __syncthreads();
+  for (int it = 0; it < 2; ++it) {
+    if (lane == 0) {
+      // forward: the row interchanges and multipliers applied to the right-hand side, the
+      // running entry carried in a register
+      double yj = sx[0];
+      for (int j = 0; j < n - 1; ++j) {
+        double yn = sx[j + 1];
+        const bool s = sw[j] != 0;
+        const double a = s ? yn : yj, b = s ? yj : yn;
+        sx[j] = a;
+        yj = fma(-sl[j], a, b);
+      }
+      sx[n - 1] = yj;
+      // back substitution through U
+      double x2 = 0.0, x1 = sx[n - 1] * sd[n - 1];
+      sx[n - 1] = x1;
+      for (int j = n - 2; j >= 0; --j) {
+        const double xj = (sx[j] - su1[j] * x1 - su2[j] * x2) * sd[j];
+        sx[j] = xj;
+        x2 = x1;
+        x1 = xj;
+      }
+    }
+    __syncthreads();
+    double s2 = 0.0, mx = 0.0;
+    for (int j = lane; j < n; j += 64) mx = fmax(mx, fabs(sx[j]));
+    mx = wave_max(mx);
+    const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+    for (int j = lane; j < n; j += 64) {
+      const double v = sx[j] * sc;
+      s2 += v * v;
+    }
+    s2 = wave_sum(s2);
+    const double f = sc / sqrt(s2);
+    __syncthreads();
+    for (int j = lane; j < n; j += 64) sx[j] *= f;
+    __syncthreads();
+  }
+  for (int j = lane; j < n; j += 64) Z[(long)i * ldz + j] = mk(sx[j], 0.0);
+}
+
+// Block-wide sum (TE_CL_NT threads); red: TE_CL_NT / 64 doubles of LDS.
+__device__ inline double block_sum_cl(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < TE_CL_NT / 64; ++q) s += red[q];
+  return s;
+}
+
+// Runs of close eigenvalues among the m2 computed ones that start below m.  kc_max: the
+// chunk size that fits LDS for this n (at most TE_MAXC).
+__global__ void __launch_bounds__(TE_CL_NT) k_tri_cluster(const double* __restrict__ d, const double* __restrict__ e,
+                                                          int n, const double* __restrict__ w, int m, int m2,
+                                                          int kc_max, cplx<double>* __restrict__ Z, long ldz) {
+  extern __shared__ double tsh[];
+  __shared__ double red[TE_CL_NT / 64];
+  __shared__ double rot_c[TE_MAXC / 2], rot_s[TE_MAXC / 2];
+  __shared__ int rot_p[TE_MAXC / 2], rot_q[TE_MAXC / 2], nrot_sh, rank_sh[TE_MAXC];
+  const int j0 = blockIdx.x, tid = threadIdx.x;
+  // max(|lower|, |upper| Gershgorin bound) of T, as the other two kernels form it
+  double bl = 0.0;
+  for (int j = tid; j < n; j += TE_CL_NT) {
+    const double el = j > 0 ? fabs(e[j - 1]) : 0.0, er = j < n - 1 ? fabs(e[j]) : 0.0;
+    bl = fmax(bl, fmax(fabs(d[j] - el - er), fabs(d[j] + el + er)));
+  }
+  bl = wave_max(bl);
+  if ((tid & 63) == 0) red[tid >> 6] = bl;
+  __syncthreads();
+  double bnorm = red[0];
+  for (int q = 1; q < TE_CL_NT / 64; ++q) bnorm = fmax(bnorm, red[q]);
+  __syncthreads();
+  const double ctol = TE_CTOL * bnorm;
+  if (j0 > 0 && w[j0] - w[j0 - 1] <= ctol) return;   // not the first of its run
+  int k = 1;
+  while (j0 + k < m2 && w[j0 + k] - w[j0 + k - 1] <= ctol) ++k;
+  if (k == 1) return;
+  // 1. modified Gram-Schmidt over the run, twice per vector
+  for (int c = j0; c < j0 + k; ++c) {
+    cplx<double>* zc = Z + (long)c * ldz;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int p = j0; p < c; ++p) {
+        const cplx<double>* zp = Z + (long)p * ldz;
+        double s = 0.0;
+        for (int r = tid; r < n; r += TE_CL_NT) s += zp[r].x * zc[r].x;
+        s = block_sum_cl(s, red);
+        for (int r = tid; r < n; r += TE_CL_NT) zc[r].x -= s * zp[r].x;
+        __syncthreads();
+      }
+    }
+    double s = 0.0;
+    for (int r = tid; r < n; r += TE_CL_NT) s += zc[r].x * zc[r].x;
+    s = block_sum_cl(s, red);
+    const double f = 1.0 / sqrt(s);
+    for (int r = tid; r < n; r += TE_CL_NT) zc[r].x *= f;
+    __syncthreads();
+  }
+  // 2. Rayleigh-Ritz within chunks of at most kc_max vectors
+  double* V = tsh;                              // [n][kc]
+  for (int c0 = j0; c0 < j0 + k; c0 += kc_max) {
+    const int kc = min(kc_max, j0 + k - c0);
+    const int kp = kc + (kc & 1);                 // even player count (one dummy when odd)
+    double* M = V + (size_t)n * kc;               // [kc][kc]
+    double* Y = M + kc * kc;                      // [kc][kc]
+    for (int t = tid; t < n * kc; t += TE_CL_NT) {
+      const int r = t / kc, q = t - r * kc;
+      V[t] = Z[(long)(c0 + q) * ldz + r].x;
+    }
+    __syncthreads();
+    for (int t = tid; t < kc * kc; t += TE_CL_NT) {
+      const int p = t / kc, q = t - p * kc;
+      Y[t] = p == q ? 1.0 : 0.0;
+      if (p > q) continue;
+      double s = 0.0;
+      for (int r = 0; r < n; ++r) {
+        double tv = d[r] * V[r * kc + q];
+        if (r > 0) tv += e[r - 1] * V[(r - 1) * kc + q];
+        if (r < n - 1) tv += e[r] * V[(r + 1) * kc + q];
+        s += V[r * kc + p] * tv;
+      }
+      M[p * kc + q] = s;
+      M[q * kc + p] = s;
+    }
+    __syncthreads();
+    double fro = 0.0;
+    for (int t = tid; t < kc * kc; t += TE_CL_NT) fro += M[t] * M[t];
+    fro = sqrt(block_sum_cl(fro, red));
+    const double thr = 2.0 * TE_EPS * fro;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+      if (tid == 0) nrot_sh = 0;
+      __syncthreads();
+      for (int rnd = 0; rnd < kp - 1; ++rnd) {
+        // round-robin pairs: player 0 fixed, the others rotate
+        if (tid < kp / 2) {
+          const int a = tid == 0 ? 0 : ((tid - 1 + rnd) % (kp - 1)) + 1;
+          const int b = ((kp - 2 - tid + rnd) % (kp - 1)) + 1;
+          int p = min(a, b), q = max(a, b);
+          double cs = 1.0, sn = 0.0;
+          if (q < kc) {
+            const double apq = M[p * kc + q];
+            if (fabs(apq) > thr) {
+              const double th = (M[q * kc + q] - M[p * kc + p]) / (2.0 * apq);
+              const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+              cs = 1.0 / sqrt(1.0 + t * t);
+              sn = t * cs;
+              atomicAdd(&nrot_sh, 1);
+            }
+          } else {
+            p = q = -1;
+          }
+          rot_p[tid] = p;
+          rot_q[tid] = q;
+          rot_c[tid] = cs;
+          rot_s[tid] = sn;
+        }
+        __syncthreads();
+        // rows: M <- J^T M
+        for (int t = tid; t < (kp / 2) * kc; t += TE_CL_NT) {
+          const int g = t / kc, col = t - g * kc, p = rot_p[g], q = rot_q[g];
+          if (p < 0) continue;
+          const double cs = rot_c[g], sn = rot_s[g], mp = M[p * kc + col], mq = M[q * kc + col];
+          M[p * kc + col] = cs * mp - sn * mq;
+          M[q * kc + col] = sn * mp + cs * mq;
+        }
+        __syncthreads();
+        // columns: M <- M J, Y <- Y J
+        for (int t = tid; t < (kp / 2) * kc; t += TE_CL_NT) {
+          const int g = t / kc, row = t - g * kc, p = rot_p[g], q = rot_q[g];
+          if (p < 0) continue;
+          const double cs = rot_c[g], sn = rot_s[g];
+          const double mp = M[row * kc + p], mq = M[row * kc + q];
+          M[row * kc + p] = cs * mp - sn * mq;
+          M[row * kc + q] = sn * mp + cs * mq;
+          const double yp = Y[row * kc + p], yq = Y[row * kc + q];
+          Y[row * kc + p] = cs * yp - sn * yq;
+          Y[row * kc + q] = sn * yp + cs * yq;
+        }
+        __syncthreads();
+      }
+      const int nr = nrot_sh;
+      __syncthreads();
+      if (nr == 0) break;
+    }
+    // ascending order of the Ritz values: rank of each (ties by index)
+    if (tid < kc) {
+      const double v = M[tid * kc + tid];
+      int rk = 0;
+      for (int q = 0; q < kc; ++q) {
+        const double u = M[q * kc + q];
+        rk += (u < v || (u == v && q < tid)) ? 1 : 0;
+      }
+      rank_sh[rk] = tid;
+    }
+    __syncthreads();
+    for (int t = tid; t < n * kc; t += TE_CL_NT) {
+      const int r = t / kc, c = t - r * kc, src = rank_sh[c];
+      double s = 0.0;
+      for (int p = 0; p < kc; ++p) s += V[r * kc + p] * Y[p * kc + src];
+      Z[(long)(c0 + c) * ldz + r] = mk(s, 0.0);
+    }
+    __syncthreads();
+  }
+}
+
+// Largest chunk of k_tri_cluster whose V, M and Y fit the dynamic LDS budget at order n.
+static int tri_cluster_chunk(int n, size_t* lds) {
+  const size_t budget = 144 * 1024;
+  int kc = TE_MAXC;
+  while (kc > 1 && ((size_t)n * kc + 2 * (size_t)kc * kc) * sizeof(double) > budget) --kc;
+  *lds = ((size_t)n * kc + 2 * (size_t)kc * kc) * sizeof(double);
+  return kc;
+}
+
+// Eigenvalues of the m lowest eigenpairs of T (d, e: n and n - 1 doubles) into w and, when
+// vectors, their eigenvectors into Z's first m columns (complex, imaginary parts zero).
+// wt: workspace of m + TE_EXTRA doubles (the computed eigenvalues).
+int tri_lowest(pw_ctx* c, int n, int m, const double* d, const double* e, double* w, double* wt, bool vectors,
+               cplx<double>* Z, long ldz) {
+  const int m2 = std::min(n, m + (vectors ? TE_EXTRA : 0));
+  hipLaunchKernelGGL(k_tri_bisect, dim3(m2), dim3(64), 3 * (size_t)n * sizeof(double), c->stream, d, e, n, wt);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return hip_check(c, err, "k_tri_bisect");
+  err = hipMemcpyAsync(w, wt, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+  if (err != hipSuccess) return hip_check(c, err, "eigenvalue copy");
+  if (!vectors) return PW_OK;
+  const size_t lds_inv = 6 * (size_t)n * sizeof(double) + (size_t)n * sizeof(int);
+  hipLaunchKernelGGL(k_tri_invit, dim3(m2), dim3(64), lds_inv, c->stream, d, e, n, wt, Z, ldz);
+  err = hipGetLastError();
+  if (err != hipSuccess) return hip_check(c, err, "k_tri_invit");
+  size_t lds_cl = 0;
+  const int kc = tri_cluster_chunk(n, &lds_cl);
+  hipLaunchKernelGGL(k_tri_cluster, dim3(m), dim3(TE_CL_NT), lds_cl, c->stream, d, e, n, wt, m, m2, kc, Z, ldz);
+  return hip_check(c, hipGetLastError(), "k_tri_cluster");
+}
+
 // Workgroups of the distributed reduction for order n: the fewest whose column share,
 // the three staging vectors and the reduction scratch fit in LDS; 0 when n is out of range.
 int hetrd_workgroups(int n, int* ncl_max, size_t* lds_bytes, int min_wg) {
@@ -659,8 +1082,11 @@ int heevd_dist_max_n() {
 // Eigenvalues (ascending, into d_w on the device) and, when nvec > 0, the eigenvectors of
 // the nvec lowest into A's first nvec columns.  rocSOLVER's handle runs on the context stream.
 // Enqueues everything on the context stream; d_info gets 0, or the ZSTEDC failure code, or
-// -1 when a cross-workgroup wait of the reduction timed out.
-int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info) {
+// -1 when a cross-workgroup wait of the reduction timed out.  lowest (with nvec > 0): only
+// the nvec lowest eigenvalues are wanted, and the tridiagonal eigenpairs come from the
+// device kernels above instead of ZSTEDC (d_w then holds nvec values; with nvec = 0 all n).
+int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info,
+               bool lowest) {
   int ncl = 0;
   size_t lds = 0;
   const int P = hetrd_workgroups(n, &ncl, &lds, c->eig_min_wg);
@@ -670,8 +1096,12 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   const size_t vbytes = (size_t)n * n * sizeof(cplx<double>);
   const int nblk = (n - 1 + WY_NB - 1) / WY_NB;
   const size_t tbytes = (size_t)nblk * WY_NB * WY_NB * sizeof(cplx<double>);
-  const size_t need =
-      vbytes + tbytes + 5 * (size_t)n * sizeof(cplx<double>) + ((size_t)n + 4 * P) * sizeof(double) + 64;
+  // (+ 2n doubles: the tridiagonal's diagonal and the computed eigenvalues of the device path;
+  // + n nvec + nvec^2 complex: its copy of the vectors and their Gram matrix)
+  const size_t lw = lowest && nvec > 0 ? ((size_t)n * nvec + (size_t)nvec * nvec) * sizeof(cplx<double>) : 0;
+  const size_t off_lw = (vbytes + tbytes + 5 * (size_t)n * sizeof(cplx<double>) +
+                         ((size_t)3 * n + 4 * P) * sizeof(double) + 64 + 255) & ~(size_t)255;
+  const size_t need = off_lw + lw;
   int rc = ensure(c, &c->d_eig, &c->eig_bytes, need);
   if (rc) return rc;
   char* base = reinterpret_cast<char*>(c->d_eig);
@@ -682,15 +1112,19 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
   cplx<double>* cg = pg + 2 * (size_t)n;
   double* E = reinterpret_cast<double*>(cg + 2 * (size_t)n);
   double* sp = E + n;
-  unsigned* sync = reinterpret_cast<unsigned*>(sp + 4 * (size_t)P);
+  double* Dt = sp + 4 * (size_t)P;
+  double* Wt = Dt + n;
+  unsigned* sync = reinterpret_cast<unsigned*>(Wt + n);
   int* status = reinterpret_cast<int*>(sync + 2);
+  cplx<double>* Zc = reinterpret_cast<cplx<double>*>(base + off_lw);
+  cplx<double>* Gz = Zc + (size_t)n * (nvec > 0 ? nvec : 0);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, sync, 3, 0u);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_check(c, e, "k_fill_u32");
   const cplx<double>* Ac = A;
   long ldal = lda, ldv = n;
   int nn = n, nclm = ncl;
-  double* dd = d_w;
+  double* dd = lowest ? Dt : d_w;
   void* args[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
                   (void*)&dd, (void*)&E, (void*)&pg, (void*)&sp, (void*)&sync, (void*)&status};
   void* args1[] = {(void*)&Ac, (void*)&ldal, (void*)&nn, (void*)&nclm, (void*)&V, (void*)&ldv, (void*)&tau,
@@ -706,13 +1140,29 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
                                    dim3(P), dim3(HT_NT), args, (unsigned)lds, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "k_hetrd_dist (cooperative launch)");
   // the timeout status of the reduction is folded into d_info below
-  rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);
-  // ZSTEDC with the tridiagonal's eigenvectors even when only values are wanted: without
-  // them rocSOLVER runs its single-thread QL iteration (ZSTERF), which costs ~13 ms at n = 160
-  rocblas_status st = rocsolver_zstedc(h, rocblas_evect_tridiagonal, n, d_w, E,
-                                       reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_info);
-  if (st != rocblas_status_success)
-    return set_error(c, PW_ERR_HIP, "zstedc: rocSOLVER status " + std::to_string((int)st));
+  if (lowest) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<unsigned*>(d_info), 1, 0u);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(c, e, "k_fill_u32");
+    int rc2 = tri_lowest(c, n, nvec > 0 ? nvec : n, Dt, E, d_w, Wt, nvec > 0, A, lda);
+    if (!rc2 && nvec > 0) {
+      // inverse-iteration vectors of eigenvalues closer than ~1e-4 ||T|| (and outside one
+      // run) overlap by ~eps ||T|| / gap: one first-order Loewdin step, Z <- Z (3 I - G) / 2
+      // with G = Z^H Z, leaves overlaps of second order and residuals at eps ||T||
+      rc2 = copy_cols(c, n, nvec, A, lda, Zc, n);
+      if (!rc2) rc2 = zgemm(c, 1, nvec, nvec, n, mk(1.0, 0.0), Zc, n, Zc, n, mk(0.0, 0.0), Gz, nvec, 0);
+      if (!rc2) rc2 = zgemm(c, 0, n, nvec, nvec, mk(-0.5, 0.0), Zc, n, Gz, nvec, mk(1.5, 0.0), A, lda, 0);
+    }
+    if (rc2) return rc2;
+  } else {
+    rocblas_handle h = reinterpret_cast<rocblas_handle>(rb);
+    // ZSTEDC with the tridiagonal's eigenvectors even when only values are wanted: without
+    // them rocSOLVER runs its single-thread QL iteration (ZSTERF), which costs ~13 ms at n = 160
+    rocblas_status st = rocsolver_zstedc(h, rocblas_evect_tridiagonal, n, d_w, E,
+                                         reinterpret_cast<rocblas_double_complex*>(A), (int)lda, d_info);
+    if (st != rocblas_status_success)
+      return set_error(c, PW_ERR_HIP, "zstedc: rocSOLVER status " + std::to_string((int)st));
+  }
   if (nvec > 0) {
     hipLaunchKernelGGL(k_larft, dim3(nblk), dim3(WY_NT_T), 0, c->stream, V, ldv, tau, n, Tb);
     e = hipGetLastError();

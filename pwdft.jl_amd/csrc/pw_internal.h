@@ -244,6 +244,7 @@ struct pw_ctx {
   void* d_dwork = nullptr; size_t dwork_bytes = 0;
   void* d_eig = nullptr;   size_t eig_bytes = 0;     // reflectors and exchange buffers of heevd_dist
   int eig_path = 0;                                   // 0: device reduction where it fits, 1: rocSOLVER only
+  int eig_tri = 1;                                    // tridiagonal eigenpairs: 0 ZSTEDC, 1 device (solver calls), 2 also pw_heevd_dev
   int eig_min_wg = 64;                                // device reduction: minimum workgroups (0: fewest that fit LDS)
   int eig_light = 3;                                  // device reduction: exchanges as device-coherent atomics (no cache flushes)
   void* d_lob = nullptr;   size_t lob_bytes = 0;     // LOBPCG blocks and small matrices (pw_lobpcg.hip)
@@ -355,7 +356,8 @@ void free_dense(pw_ctx* c);
 // Hermitian eigensolver with the device tridiagonal reduction (pw_eig.hip): eigenvalues into
 // d_w (device), eigenvectors of the nvec lowest into A's first nvec columns; enqueued on the
 // context stream, status (0, rocSOLVER's code, or -1 for a timed-out reduction) into d_info.
-int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info);
+int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_w, int nvec, int* d_info,
+               bool lowest = false);
 int heevd_dist_max_n();
 // dst[:, 0:ncols] = src[:, 0:ncols] (nrows rows) on the context stream.
 int copy_cols(pw_ctx* c, int nrows, int ncols, const cplx<double>* src, long lds, cplx<double>* dst, long ldd);

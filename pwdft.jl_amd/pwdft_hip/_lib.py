@@ -29,6 +29,8 @@ PW_TUNE_FUSED_BATCH = 11
 PW_TUNE_LOBPCG_GRAPH = 12
 PW_TUNE_FUSED_NT = 13
 PW_TUNE_BATCH_GEMM_SMALL = 16
+PW_TUNE_EIG_WG = 17
+PW_TUNE_EIG_TRI = 18
 
 PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
 
@@ -72,6 +74,7 @@ _PROTOS = {
     "pw_chol_inv_batch_async_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp],
     "pw_heevd_batch_async_dev": [_vp, _i, _i, _vp, _i64, _i64, _vp, _vp],
     "pw_heev_values_dev": [_vp, _i, _vp, _i64, _vp],
+    "pw_heev_lowest_dev": [_vp, _i, _i, _vp, _i64, _vp],
     "pw_hegvd_dev": [_vp, _i, _vp, _i64, _vp, _i64, _vp],
     "pw_diag_lobpcg_dev": [_vp, _i, _i, _i, _vp, _i64, _d, _i, _i, _c.c_uint, _vp, _vp],
     "pw_diag_davidson_dev": [_vp, _i, _i, _i, _vp, _i64, _d, _i, _i, _c.c_uint, _vp, _vp],
