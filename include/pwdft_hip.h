@@ -121,13 +121,18 @@ enum {
                                    workgroups share the matrix (default 64: n = 320 5.2 -> 4.2
                                    ms, n = 480 7.5 -> 6.7 ms for the whole eigensolve); 0: the
                                    fewest whose column shares fit in LDS, at least 8 */
-  PW_TUNE_EIG_TRI = 18          /* eigenpairs of the reduced tridiagonal matrix: 1 (default)
+  PW_TUNE_EIG_TRI = 18,         /* eigenpairs of the reduced tridiagonal matrix: 1 (default)
                                    only the wanted lowest ones on the device (multisection
                                    Sturm bisection, inverse iteration, orthogonalisation and
                                    Rayleigh-Ritz within runs of close eigenvalues) in the
                                    solvers and pw_heev_lowest_dev, rocSOLVER ZSTEDC for
                                    pw_heevd_dev; 2 the device path for pw_heevd_dev as well;
                                    0 ZSTEDC everywhere */
+  PW_TUNE_CHOL_REC = 19         /* inv(cholesky(A).U) above order 64 in the solvers and in
+                                   pw_chol_inv_batch_dev with one matrix: 1 (default) recursive
+                                   2 x 2 blocks on the device (one-workgroup kernels for the
+                                   diagonal blocks of order <= 64, complex GEMMs for the rest);
+                                   0 rocSOLVER ZPOTRF + ZTRTRI */
 };
 int pw_tune_set(pw_ctx* ctx, int key, int value);
 

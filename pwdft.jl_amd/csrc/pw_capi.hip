@@ -425,6 +425,10 @@ int pw_tune_set(pw_ctx* c, int key, int value) {
       if (value < 0 || value > 256) return set_error(c, PW_ERR_ARG, "pw_tune_set: reduction workgroups must be 0..256");
       c->eig_min_wg = value;
       return PW_OK;
+    case PW_TUNE_CHOL_REC:
+      if (value < 0 || value > 1) return set_error(c, PW_ERR_ARG, "pw_tune_set: Cholesky inverse mode must be 0..1");
+      c->chol_rec = value;
+      return PW_OK;
     case PW_TUNE_EIG_TRI:
       if (value < 0 || value > 2) return set_error(c, PW_ERR_ARG, "pw_tune_set: tridiagonal eigensolver mode must be 0..2");
       c->eig_tri = value;

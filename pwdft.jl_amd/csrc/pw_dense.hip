@@ -43,6 +43,9 @@ void free_dense(pw_ctx* c) {
   hipFree(c->d_eig);
   c->d_eig = nullptr;
   c->eig_bytes = 0;
+  hipFree(c->d_chol);
+  c->d_chol = nullptr;
+  c->chol_bytes = 0;
   hipFree(c->d_lob);
   c->d_lob = nullptr;
   c->lob_bytes = 0;
@@ -137,10 +140,86 @@ int solver_handle(pw_ctx* c, void** rb) {
   return PW_OK;
 }
 
+// ---- inv(cholesky(Hermitian(A)).U) above the one-workgroup order, by 2 x 2 blocks
+// (A = U^H U, U = [U11 U12; 0 U22]): V11 = inv(U11) by recursion on A11; T = V11^H A12 (= U12);
+// A22 -= T^H T; V22 = inv(U22) by recursion on A22; V12 = -V11 (T V22).  The recursion ends in
+// blocks of order <= small_dense_max_n() (k_chol_inv_small, one workgroup each); the
+// off-diagonal blocks are complex GEMMs.  Each leaf writes its potrf / trtri statuses to its
+// own slot, and k_chol_rec_info folds them into the caller's two info words (the failing
+// leading minor counted from the whole matrix).  Stable as blocked right-looking Cholesky;
+// the inverse's off-diagonal blocks come from the same products as a blocked triangular
+// inverse.
+constexpr int CR_MAXLEAF = 128;
+struct CholLeaves {
+  int off[CR_MAXLEAF];
+  int n;
+};
+
+__global__ void k_chol_rec_info(const int* __restrict__ linfo, CholLeaves lv, int* __restrict__ info) {
+  int p = 0, t = 0;
+  for (int l = 0; l < lv.n; ++l) {
+    if (p == 0 && linfo[2 * l] != 0) p = lv.off[l] + linfo[2 * l];
+    if (t == 0 && linfo[2 * l + 1] != 0) t = lv.off[l] + linfo[2 * l + 1];
+  }
+  info[0] = p;
+  info[1] = t;
+}
+
+struct CholRec {
+  cplx<double>* ws;
+  int* linfo;
+  CholLeaves lv;
+};
+
+static int chol_rec(pw_ctx* c, int n, cplx<double>* A, long lda, int off, CholRec& r) {
+  if (n <= small_dense_max_n()) {
+    if (r.lv.n >= CR_MAXLEAF) return set_error(c, PW_ERR_ARG, "cholesky inverse: too many blocks");
+    const int slot = r.lv.n++;
+    r.lv.off[slot] = off;
+    return chol_inv_small_batch(c, n, 1, A, lda, 0, r.linfo + 2 * slot);
+  }
+  const cplx<double> one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  const int n1 = n / 2, n2 = n - n1;
+  cplx<double>* A11 = A;
+  cplx<double>* A12 = A + (long)n1 * lda;
+  cplx<double>* A22 = A12 + n1;
+  cplx<double>* T = r.ws;
+  cplx<double>* W = T + (size_t)n1 * n2;
+  r.ws = W + (size_t)n1 * n2;
+  int rc = chol_rec(c, n1, A11, lda, off, r);
+  if (!rc) rc = zgemm(c, 1, n1, n2, n1, one, A11, lda, A12, lda, zero, T, n1, 0);
+  if (!rc) rc = zgemm(c, 1, n2, n2, n1, mone, T, n1, T, n1, one, A22, lda, 0);
+  if (!rc) rc = chol_rec(c, n2, A22, lda, off + n1, r);
+  if (!rc) rc = zgemm(c, 0, n1, n2, n2, one, T, n1, A22, lda, zero, W, n1, 0);
+  if (!rc) rc = zgemm(c, 0, n1, n2, n1, mone, A11, lda, W, n1, zero, A12, lda, 0);
+  // the block's strict lower triangle (the caller's unread entries below A11's and A22's
+  // diagonals and all of A21) zeroed, so the level above multiplies a triangular factor
+  if (!rc) rc = herm_fix_launch(c, n, 1, A, lda, 0, 1);
+  r.ws = T;
+  return rc;
+}
+
+static int chol_inv_rec(pw_ctx* c, int n, cplx<double>* A, long lda, int* d_info) {
+  // blocks T and W of every level on the current path: sum of 2 n1 n2 < n^2 complex
+  const size_t ws = (size_t)n * n * sizeof(cplx<double>);
+  int rc = ensure(c, &c->d_chol, &c->chol_bytes, ws + 2 * CR_MAXLEAF * sizeof(int));
+  if (rc) return rc;
+  CholRec r;
+  r.ws = reinterpret_cast<cplx<double>*>(c->d_chol);
+  r.linfo = reinterpret_cast<int*>(reinterpret_cast<char*>(c->d_chol) + ws);
+  r.lv.n = 0;
+  rc = chol_rec(c, n, A, lda, 0, r);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_chol_rec_info, dim3(1), dim3(1), 0, c->stream, r.linfo, r.lv, d_info);
+  return hip_check(c, hipGetLastError(), "k_chol_rec_info");
+}
+
 // U = cholesky(Hermitian(A)).U and its inverse in place (strict lower triangle zeroed);
 // potrf's info into d_info[0], trtri's into d_info[1].  No synchronisation.
 int chol_inv_enqueue(pw_ctx* c, int n, cplx<double>* A, long lda, int* d_info) {
   if (c->eig_path == 0 && n <= small_dense_max_n()) return chol_inv_small_batch(c, n, 1, A, lda, 0, d_info);
+  if (c->eig_path == 0 && c->chol_rec && n <= small_dense_max_n() * CR_MAXLEAF)
+    return chol_inv_rec(c, n, A, lda, d_info);
   int rc = ensure_solver(c);
   if (rc) return rc;
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
@@ -206,6 +285,8 @@ int pw_herm_fix_batch_dev(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, in
 static int chol_inv_batch_enqueue(pw_ctx* c, int n, int nbatch, void* A, int64_t lda, int64_t strideA, int* dinfo) {
   if (c->eig_path == 0 && n <= small_dense_max_n())
     return chol_inv_small_batch(c, n, nbatch, reinterpret_cast<cplx<double>*>(A), (long)lda, (long)strideA, dinfo);
+  if (nbatch == 1 && c->eig_path == 0 && c->chol_rec && n <= small_dense_max_n() * CR_MAXLEAF)
+    return chol_inv_rec(c, n, reinterpret_cast<cplx<double>*>(A), (long)lda, dinfo);
   rocblas_handle h = reinterpret_cast<rocblas_handle>(c->rb);
   rocblas_double_complex* a = reinterpret_cast<rocblas_double_complex*>(A);
   int rc = st_check(c, rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, a, (int)lda, (rocblas_stride)strideA,

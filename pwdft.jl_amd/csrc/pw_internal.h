@@ -243,6 +243,8 @@ struct pw_ctx {
   void* d_dinfo = nullptr; size_t dinfo_bytes = 0;
   void* d_dwork = nullptr; size_t dwork_bytes = 0;
   void* d_eig = nullptr;   size_t eig_bytes = 0;     // reflectors and exchange buffers of heevd_dist
+  void* d_chol = nullptr;  size_t chol_bytes = 0;    // blocks and leaf statuses of the recursive Cholesky inverse
+  int chol_rec = 1;                                   // Cholesky inverse above order 64: 1 recursive on the device, 0 rocSOLVER
   int eig_path = 0;                                   // 0: device reduction where it fits, 1: rocSOLVER only
   int eig_tri = 1;                                    // tridiagonal eigenpairs: 0 ZSTEDC, 1 device (solver calls), 2 also pw_heevd_dev
   int eig_min_wg = 64;                                // device reduction: minimum workgroups (0: fewest that fit LDS)

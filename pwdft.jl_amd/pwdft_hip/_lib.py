@@ -31,6 +31,7 @@ PW_TUNE_FUSED_NT = 13
 PW_TUNE_BATCH_GEMM_SMALL = 16
 PW_TUNE_EIG_WG = 17
 PW_TUNE_EIG_TRI = 18
+PW_TUNE_CHOL_REC = 19
 
 PROF_SLOTS = {"col_ifft": 0, "plane_vloc": 1, "col_fft": 2, "gemm_cn": 3, "gemm_nn": 4, "plane_rhoe": 5}
 

@@ -197,11 +197,12 @@ def test_heevd_batch_graded_spectrum():
     c.close()
 
 
-@pytest.mark.parametrize("n,nbatch", [(1, 3), (8, 182), (24, 182), (40, 5)])
+@pytest.mark.parametrize("n,nbatch", [(1, 3), (8, 182), (24, 182), (40, 5), (100, 1), (160, 1), (480, 1), (1000, 1)])
 def test_chol_inv_batch_vs_numpy(n, nbatch):
     """pw_chol_inv_batch_dev: every block becomes inv(cholesky(Hermitian(A)).U) of numpy's
-    factor (upper triangle read, strict lower triangle zeroed); a batch holding an indefinite
-    matrix raises PosDefException naming it.  pw_herm_fix_batch_dev modes 0 and 2 against
+    factor (upper triangle read, strict lower triangle zeroed; one matrix above order 64 by
+    the recursive 2 x 2 block form); a batch holding an indefinite matrix raises
+    PosDefException naming it.  pw_herm_fix_batch_dev modes 0 and 2 against
     numpy's Hermitian() and (A + A^H)/2."""
     from pwdft_hip._lib import Context, DeviceMatrix, PosDefException
     c = Context(0)
@@ -496,3 +497,37 @@ def test_op_H_batch_fused_vs_three_stage(al, mixed):
     finally:
         H.ctx.call("pw_tune_set", PW_TUNE_FUSED_BATCH, 1)
         H.ctx.call("pw_tune_set", PW_TUNE_FUSED_NT, 0)
+
+
+@pytest.mark.parametrize("n,bad", [(480, 300), (160, 70), (100, 0)])
+def test_chol_inv_recursive_failure_index_and_rocsolver(n, bad):
+    """The recursive Cholesky inverse reports the failing leading minor counted over the
+    whole matrix (a leaf deep in the recursion), as rocSOLVER's ZPOTRF does
+    (PW_TUNE_CHOL_REC = 0), and both give the same inverse factor of a definite matrix."""
+    from pwdft_hip._lib import Context, DeviceMatrix, PosDefException, PW_TUNE_CHOL_REC
+    c = Context(0)
+    rng = np.random.default_rng(n + bad)
+    Z = rng.standard_normal((n, 2 * n)) + 1j * rng.standard_normal((n, 2 * n))
+    A = Z @ Z.conj().T / (2 * n) + 0.1 * np.eye(n)
+    d = DeviceMatrix.from_host(c, A)
+    out = []
+    for mode in (1, 0):
+        c.call("pw_tune_set", PW_TUNE_CHOL_REC, mode)
+        try:
+            d.upload(A)
+            c.call("pw_chol_inv_batch_dev", n, 1, d.p, n, n * n)
+            out.append(d.download())
+            B = np.array(A)
+            B[bad, bad] = -1.0
+            d.upload(B)
+            print(f"case chol_inv recursive failure n={n} bad={bad} mode={mode}", flush=True)
+            with pytest.raises(PosDefException, match=f"leading minor {bad + 1} "):
+                c.call("pw_chol_inv_batch_dev", n, 1, d.p, n, n * n)
+        finally:
+            c.call("pw_tune_set", PW_TUNE_CHOL_REC, 1)
+    ref = np.linalg.inv(np.linalg.cholesky(A).conj().T)
+    scale = np.abs(ref).max()
+    assert np.abs(out[0] - ref).max() < 1e-11 * scale
+    assert np.abs(out[0] - out[1]).max() < 1e-11 * scale
+    d.free()
+    c.close()
