@@ -96,8 +96,12 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
           p = m - 1;
           q = r;
         } else {
-          p = (r + lane) % (m - 1);
-          q = (r - lane + (m - 1)) % (m - 1);
+          // (r + lane) mod (m - 1) and (r - lane) mod (m - 1) by one conditional subtraction
+          // each (both sums lie in [0, 2 (m - 1))): this runs on every round's critical path
+          p = r + lane;
+          if (p >= m - 1) p -= m - 1;
+          q = r - lane + (m - 1);
+          if (q >= m - 1) q -= m - 1;
         }
         if (p > q) { const int t = p; p = q; q = t; }
         double c = 1.0, s = 0.0;
@@ -127,9 +131,10 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
         rph[lane] = ph;
       }
       __syncthreads();
-      // A <- J^H A J, block (rows of pair k, columns of pair k2)
-      for (int t = lane; t < npair * npair; t += SM_NT) {
-        const int k = t % npair, k2 = t / npair;
+      // A <- J^H A J, block (rows of pair k, columns of pair k2); k = lane mod 32 and k2 stepping
+      // by SM_NT / 32 (npair <= 32), no run-time division
+      static_assert(SM_NMAX / 2 <= 32 && SM_NMAX <= 64, "lane maps of the round updates");
+      for (int k2 = lane >> 5, k = lane & 31; k < npair && k2 < npair; k2 += SM_NT / 32) {
         const int p = rp[k], q = rq[k], p2 = rp[k2], q2 = rq[k2];
         const double s = rs[k], s2 = rs[k2];
         if (s == 0.0 && s2 == 0.0) continue;
@@ -155,9 +160,8 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
         if (qv) a[q + n * p2] = b10;
         if (qv && q2v) a[q + n * q2] = b11;
       }
-      // V <- V J, columns of pair k2
-      for (int t = lane; t < npair * n; t += SM_NT) {
-        const int k2 = t / n, i = t % n;
+      // V <- V J, columns of pair k2; row i = lane mod 64 (n <= 64)
+      for (int k2 = lane >> 6, i = lane & 63; i < n && k2 < npair; k2 += SM_NT / 64) {
         const int p2 = rp[k2], q2 = rq[k2];
         const double s2 = rs[k2];
         if (q2 < 0 || s2 == 0.0) continue;
@@ -221,15 +225,15 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
       __syncthreads();
       break;
     }
-    const double d = sqrt(dkk);
+    const double d = sqrt(dkk), rd = 1.0 / d;
     __syncthreads();
-    for (int j = k + tid; j < n; j += SM_NT) u[k + n * j] = j == k ? mk(d, 0.0) : scal(1.0 / d, u[k + n * j]);
+    for (int j = k + tid; j < n; j += SM_NT) u[k + n * j] = j == k ? mk(d, 0.0) : scal(rd, u[k + n * j]);
     __syncthreads();
-    // A_ij -= conj(U_ki) U_kj for k < i <= j
-    const int m = n - k - 1;
-    for (int t = tid; t < m * m; t += SM_NT) {
-      const int i = k + 1 + t % m, j = k + 1 + t / m;
-      if (i <= j) u[i + n * j] = u[i + n * j] - cmulc(u[k + n * j], u[k + n * i]);
+    // A_ij -= conj(U_ki) U_kj for k < i <= j, threads on a fixed 16 x 16 grid of (i, j)
+    static_assert(SM_NT == 256, "16 x 16 update grid");
+    for (int j = k + 1 + (tid >> 4); j < n; j += 16) {
+      const cplx<double> ukj = u[k + n * j];
+      for (int i = k + 1 + (tid & 15); i <= j; i += 16) u[i + n * j] = u[i + n * j] - cmulc(ukj, u[k + n * i]);
     }
     __syncthreads();
   }
@@ -241,15 +245,23 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
     }
     return;
   }
-  // X = U^-1, one column per thread: U X[:, j] = e_j by back substitution
-  for (int j = tid; j < n; j += SM_NT) {
+  // X = U^-1: U X[:, j] = e_j by back substitution, column j on the four lanes 4 j .. 4 j + 3
+  // of one wavefront (n <= SM_NT / 4): each lane sums every fourth term of a row's dot
+  // product, the four partial sums are combined by lane shuffles, and every lane of the
+  // group writes the (equal) result, so each lane's later reads follow its own store
+  static_assert(SM_NMAX * 4 <= SM_NT, "four lanes per column");
+  {
+    const int j = tid >> 2, q = tid & 3;
+    const bool col = j < n;
     for (int i = n - 1; i >= 0; --i) {
-      cplx<double> s = mk(i == j ? 1.0 : 0.0, 0.0);
-      if (i <= j)
-        for (int k = i + 1; k <= j; ++k) s = s - cmul(u[i + n * k], x[k + n * j]);
-      else
-        s = mk(0.0, 0.0);
-      x[i + n * j] = i <= j ? scal(1.0 / u[i + n * i].x, s) : mk(0.0, 0.0);
+      cplx<double> s = mk(0.0, 0.0);
+      if (col && i < j)
+        for (int k = i + 1 + q; k <= j; k += 4) s = s + cmul(u[i + n * k], x[k + n * j]);
+      s.x += __shfl_xor(s.x, 1, 64);
+      s.y += __shfl_xor(s.y, 1, 64);
+      s.x += __shfl_xor(s.x, 2, 64);
+      s.y += __shfl_xor(s.y, 2, 64);
+      if (col) x[i + n * j] = i <= j ? scal(1.0 / u[i + n * i].x, mk((i == j ? 1.0 : 0.0) - s.x, -s.y)) : mk(0.0, 0.0);
     }
   }
   __syncthreads();
