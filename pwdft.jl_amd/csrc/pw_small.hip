@@ -13,8 +13,9 @@
 //    real by diag(1, e^{-i phi}) and diagonalised by the classical rotation
 //    t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = (A_qq - A_pp) / (2 |g|).
 //  - k_chol_inv_small: U = chol(Hermitian(A)).U by right-looking elimination, then U^-1 by
-//    column-parallel back substitution; the strict lower triangle zeroed.  Status: 0, or
-//    the index (1-based) of the first non-positive pivot.
+//    back substitution over rows, the matrix held in registers (a 4 x 4 tile per thread) and
+//    one barrier per step; the strict lower triangle zeroed.  Status: 0, or the index
+//    (1-based) of the first non-positive pivot.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -202,76 +203,161 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
 
 // Matrix b of the stack: inv(chol(Hermitian(A)).U) in place, strict lower triangle zeroed;
 // info[b] (potrf) and info[nbatch + b] (trtri) as rocSOLVER would set them.
+// Thread (ti, tj) of a 16 x 16 grid holds the entries (ti + 16 a, tj + 16 b), a, b < 4, in
+// registers from the load to the store.  Factorization (right-looking): at step k the holders
+// of row k publish it to an LDS row (two rows alternate), one barrier, then every thread
+// scales what it needs of row k by 1 / sqrt(A_kk) and updates its own trailing entries.
+// Inverse (X = U^-1, rows from the last): the holders of row i finish X_ij = (delta_ij -
+// S_ij) / U_ii and publish it, one barrier, then every thread adds U_ri X_ij to its sums S_rj
+// of the rows r < i (U's column i read from LDS, where U was stored after the factorization).
+// One barrier per step in both phases.
 __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restrict__ A, long lda, long stride, int n,
                                                           int* __restrict__ info, int nbatch) {
   extern __shared__ char smem_raw[];
-  cplx<double>* u = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n
-  cplx<double>* x = u + n * n;                                     // n x n
-  __shared__ int bad;
+  cplx<double>* u = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n: U after the factorization
+  __shared__ cplx<double> rowbuf[2][SM_NMAX];
+  __shared__ double rdiag[SM_NMAX];
+  static_assert(SM_NT == 256 && SM_NMAX <= 64, "16 x 16 grid, 4 x 4 entries per thread");
   cplx<double>* M = A + (long)blockIdx.x * stride;
-  const int tid = threadIdx.x;
-  const int nn = n * n;
-  for (int t = tid; t < nn; t += SM_NT) {
-    const int i = t % n, j = t / n;
-    u[i + n * j] = i <= j ? (i == j ? mk(M[(long)j * lda + i].x, 0.0) : M[(long)j * lda + i]) : mk(0.0, 0.0);
-  }
-  if (tid == 0) bad = 0;
-  __syncthreads();
-  // right-looking: row k of U, then the trailing upper triangle
+  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+  cplx<double> r[4][4];   // r[b][a]: entry (ti + 16 a, tj + 16 b)
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ti + 16 * a, j = tj + 16 * b;
+      cplx<double> v = mk(0.0, 0.0);
+      if (j < n && i <= j) {
+        v = M[(long)j * lda + i];
+        if (i == j) v.y = 0.0;
+      }
+      r[b][a] = v;
+    }
+  int bad = 0;
   for (int k = 0; k < n; ++k) {
-    const double dkk = u[k + n * k].x;
-    if (!(dkk > 0.0)) {
-      if (tid == 0 && bad == 0) bad = k + 1;
-      __syncthreads();
+    const int buf = k & 1, ak = k >> 4;
+    if (ti == (k & 15)) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == ak) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int j = tj + 16 * b;
+            if (j >= k && j < n) rowbuf[buf][j] = r[b][a];
+          }
+        }
+    }
+    __syncthreads();
+    const double dkk = rowbuf[buf][k].x;
+    if (!(dkk > 0.0)) {   // the same value on every thread: a uniform exit
+      bad = k + 1;
       break;
     }
-    const double d = sqrt(dkk), rd = 1.0 / d;
-    __syncthreads();
-    for (int j = k + tid; j < n; j += SM_NT) u[k + n * j] = j == k ? mk(d, 0.0) : scal(rd, u[k + n * j]);
-    __syncthreads();
-    // A_ij -= conj(U_ki) U_kj for k < i <= j, threads on a fixed 16 x 16 grid of (i, j)
-    static_assert(SM_NT == 256, "16 x 16 update grid");
-    for (int j = k + 1 + (tid >> 4); j < n; j += 16) {
-      const cplx<double> ukj = u[k + n * j];
-      for (int i = k + 1 + (tid & 15); i <= j; i += 16) u[i + n * j] = u[i + n * j] - cmulc(ukj, u[k + n * i]);
+    // 1 / sqrt and sqrt from one reciprocal square root; the row reads are unconditional
+    // (every index < SM_NMAX is a valid LDS slot) so they issue together and wait once
+    const double rd = rsqrt(dkk), d = dkk * rd;
+    cplx<double> uki[4], ukj[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ti + 16 * a;
+      const cplx<double> v = rowbuf[buf][i];
+      uki[a] = (i > k && i < n) ? scal(rd, v) : mk(0.0, 0.0);
     }
-    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = tj + 16 * b;
+      const cplx<double> v = rowbuf[buf][j];
+      ukj[b] = (j > k && j < n) ? scal(rd, v) : mk(0.0, 0.0);
+    }
+    // tile rows a and columns b wholly before k are finished: skipped on every thread alike
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (16 * b + 15 < k || 16 * b >= n) continue;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (16 * a + 15 < k || 16 * a > 16 * b + 15) continue;
+        const int i = ti + 16 * a, j = tj + 16 * b;
+        if (j < n && i <= j) {
+          if (i == k) r[b][a] = j == k ? mk(d, 0.0) : scal(rd, r[b][a]);            // row k of U
+          else if (i > k) r[b][a] = r[b][a] - cmulc(ukj[b], uki[a]);               // A_ij -= conj(U_ki) U_kj
+        }
+      }
+    }
   }
-  const int b = blockIdx.x;
+  const int bb = blockIdx.x;
   if (bad) {
     if (tid == 0) {
-      info[b] = bad;
-      info[nbatch + b] = 0;
+      info[bb] = bad;
+      info[nbatch + bb] = 0;
     }
     return;
   }
-  // X = U^-1: U X[:, j] = e_j by back substitution, column j on the four lanes 4 j .. 4 j + 3
-  // of one wavefront (n <= SM_NT / 4): each lane sums every fourth term of a row's dot
-  // product, the four partial sums are combined by lane shuffles, and every lane of the
-  // group writes the (equal) result, so each lane's later reads follow its own store
-  static_assert(SM_NMAX * 4 <= SM_NT, "four lanes per column");
-  {
-    const int j = tid >> 2, q = tid & 3;
-    const bool col = j < n;
-    for (int i = n - 1; i >= 0; --i) {
-      cplx<double> s = mk(0.0, 0.0);
-      if (col && i < j)
-        for (int k = i + 1 + q; k <= j; k += 4) s = s + cmul(u[i + n * k], x[k + n * j]);
-      s.x += __shfl_xor(s.x, 1, 64);
-      s.y += __shfl_xor(s.y, 1, 64);
-      s.x += __shfl_xor(s.x, 2, 64);
-      s.y += __shfl_xor(s.y, 2, 64);
-      if (col) x[i + n * j] = i <= j ? scal(1.0 / u[i + n * i].x, mk((i == j ? 1.0 : 0.0) - s.x, -s.y)) : mk(0.0, 0.0);
+  // U into LDS (its columns feed the inverse), 1 / U_ii
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ti + 16 * a, j = tj + 16 * b;
+      if (j < n && i <= j) {
+        u[i + n * j] = r[b][a];
+        if (i == j) rdiag[i] = 1.0 / r[b][a].x;
+      }
+      r[b][a] = mk(0.0, 0.0);   // from here: the sums S, then X
+    }
+  __syncthreads();
+  for (int i = n - 1; i >= 0; --i) {
+    const int buf = i & 1, ai = i >> 4;
+    if (ti == (i & 15)) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (a == ai) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int j = tj + 16 * b;
+            if (j >= i && j < n) {
+              const cplx<double> x = scal(rdiag[i], mk((j == i ? 1.0 : 0.0) - r[b][a].x, -r[b][a].y));
+              r[b][a] = x;
+              rowbuf[buf][j] = x;
+            }
+          }
+        }
+    }
+    __syncthreads();
+    cplx<double> uri[4], xij[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int rr = ti + 16 * a;
+      const cplx<double> v = u[min(rr, n - 1) + n * i];   // unconditional (clamped) read
+      uri[a] = rr < i ? v : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = tj + 16 * b;
+      const cplx<double> v = rowbuf[buf][j];
+      xij[b] = (j >= i && j < n) ? v : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (16 * b + 15 < i || 16 * b >= n) continue;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (16 * a >= i) continue;
+        const int rr = ti + 16 * a, j = tj + 16 * b;
+        if (rr < i && j >= i && j < n) r[b][a] = r[b][a] + cmul(uri[a], xij[b]);
+      }
     }
   }
-  __syncthreads();
-  for (int t = tid; t < nn; t += SM_NT) {
-    const int i = t % n, j = t / n;
-    M[(long)j * lda + i] = x[i + n * j];
-  }
+  // X (upper triangle) and zeros below the diagonal
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ti + 16 * a, j = tj + 16 * b;
+      if (i < n && j < n) M[(long)j * lda + i] = i <= j ? cplx<double>(r[b][a]) : mk(0.0, 0.0);
+    }
   if (tid == 0) {
-    info[b] = 0;
-    info[nbatch + b] = 0;
+    info[bb] = 0;
+    info[nbatch + bb] = 0;
   }
 }
 
@@ -285,7 +371,7 @@ int heev_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, lo
 
 int chol_inv_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, int* d_info) {
   if (n < 1 || n > SM_NMAX || nbatch < 1) return set_error(c, PW_ERR_ARG, "chol_inv_small_batch: order out of range");
-  const size_t lds = 2 * (size_t)n * n * sizeof(cplx<double>);
+  const size_t lds = (size_t)n * n * sizeof(cplx<double>);
   hipLaunchKernelGGL(k_chol_inv_small, dim3(nbatch), dim3(SM_NT), lds, c->stream, A, lda, stride, n, d_info, nbatch);
   return hip_check(c, hipGetLastError(), "k_chol_inv_small");
 }
