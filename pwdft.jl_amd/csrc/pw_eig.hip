@@ -545,66 +545,114 @@ __global__ void __launch_bounds__(WY_NT_T) k_larft(const cplx<double>* __restric
 }
 
 // C[:, c] <- Q C[:, c] for the columns c < ncols, Q = B_0 B_1 ... (B_b = I - V_b T_b V_b^H):
-// the blocks are applied last to first to WY_CW columns per workgroup staged in LDS.
+// the blocks are applied last to first to CW columns per workgroup staged in LDS.  Each
+// block's reflectors are read from HBM / L2 in chunks of WY_RC rows staged in LDS with
+// coalesced loads; from a chunk every thread accumulates its share of W = V_b^H C (thread:
+// reflector l, column q, half of the chunk's rows), and after W2 = T_b W the same chunks give
+// C -= V_b W2 (thread: row, column).  V_b's implicit entries: v_l[i0 + l + 1] = 1, zero above.
+constexpr int WY_RC = 64;      // rows of a staged reflector chunk
+template <int CW>
 __global__ void __launch_bounds__(HT_NT) k_apply_q_wy(const cplx<double>* __restrict__ V, long ldv,
                                                      const cplx<double>* __restrict__ Tb, int n,
                                                      cplx<double>* __restrict__ C, long ldc, int ncols) {
+  static_assert(HT_NT == 256 && WY_NB == 32 && WY_RC == 64 && (CW == 4 || CW == 1), "thread maps below");
   extern __shared__ char smem_raw[];
-  cplx<double>* Cs = reinterpret_cast<cplx<double>*>(smem_raw);   // [WY_CW][n]
+  cplx<double>* Cs = reinterpret_cast<cplx<double>*>(smem_raw);   // [CW][n]
   __shared__ cplx<double> Ts[WY_NB][WY_NB + 1];
-  __shared__ cplx<double> W[WY_NB][WY_CW], W2[WY_NB][WY_CW];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = HT_NT / 64;
-  const int c0 = blockIdx.x * WY_CW;
-  const int nc = min(WY_CW, ncols - c0);
-  for (int t = tid; t < WY_CW * n; t += HT_NT) {
+  __shared__ cplx<double> Vs[WY_RC][WY_NB + 1];
+  __shared__ cplx<double> W[WY_NB][CW], W2[WY_NB][CW];
+  __shared__ cplx<double> Wh[2][WY_NB][CW];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CW;
+  const int nc = min(CW, ncols - c0);
+  for (int t = tid; t < CW * n; t += HT_NT) {
     const int q = t / n, r = t % n;
     Cs[t] = q < nc ? C[(long)(c0 + q) * ldc + r] : mk(0.0, 0.0);
   }
+  // phase-1 map: reflector l, column q, row half h (CW = 1: four row quarters)
+  const int l1 = tid & 31;
+  const int q1 = CW == 4 ? (tid >> 5) & 3 : 0;
+  const int h1 = CW == 4 ? tid >> 7 : tid >> 5;            // 0..1 or 0..7
+  constexpr int NH = CW == 4 ? 2 : 8;                         // row groups of a chunk
+  constexpr int RH = WY_RC / NH;                              // rows per group
   const int nblk = (n - 1 + WY_NB - 1) / WY_NB;
   for (int b = nblk - 1; b >= 0; --b) {
     const int i0 = b * WY_NB;
     const int nr = min(WY_NB, n - 1 - i0);
     for (int t = tid; t < WY_NB * WY_NB; t += HT_NT) Ts[t % WY_NB][t / WY_NB] = Tb[(long)b * WY_NB * WY_NB + t];
-    __syncthreads();
-    // W = V_b^H C
-    for (int p = wave; p < nr * WY_CW; p += nwave) {
-      const int l = p % nr, q = p / nr;
-      const cplx<double>* vl = V + (long)(i0 + l) * ldv;
-      const cplx<double>* cq = Cs + (long)q * n;
-      const int r0 = i0 + l + 1;
-      double sr = 0.0, si = 0.0;
-      for (int r = r0 + lane; r < n; r += 64) {
-        const cplx<double> a = r == r0 ? mk(1.0, 0.0) : vl[r];
-        const cplx<double> x = cq[r];
-        sr += a.x * x.x + a.y * x.y;
-        si += a.x * x.y - a.y * x.x;
+    // W = V_b^H C over the rows i0 + 1 .. n - 1, chunk by chunk
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int rb = i0 + 1; rb < n; rb += WY_RC) {
+      __syncthreads();   // the previous chunk's readers are done
+      for (int e = tid; e < WY_RC * WY_NB; e += HT_NT) {
+        const int rr = e & (WY_RC - 1), l = e >> 6, r = rb + rr;
+        cplx<double> v = mk(0.0, 0.0);
+        if (r < n && l < nr) {
+          const int r1 = i0 + l + 1;
+          v = r == r1 ? mk(1.0, 0.0) : (r > r1 ? V[(long)(i0 + l) * ldv + r] : mk(0.0, 0.0));
+        }
+        Vs[rr][l] = v;
       }
-      sr = wave_sum(sr);
-      si = wave_sum(si);
-      if (lane == 0) W[l][q] = mk(sr, si);
+      __syncthreads();
+      if (l1 < nr && q1 < nc) {
+        for (int k = 0; k < RH; ++k) {
+          const int rr = h1 * RH + k, r = rb + rr;
+          if (r < n) {
+            const cplx<double> a = Vs[rr][l1], x = Cs[q1 * n + r];
+            acc.x += a.x * x.x + a.y * x.y;
+            acc.y += a.x * x.y - a.y * x.x;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // combine the row groups
+    if (CW == 4) {
+      Wh[h1][l1][q1] = acc;
+      __syncthreads();
+      if (tid < WY_NB * CW) {
+        const int l = tid & 31, q = tid >> 5;
+        W[l][q] = Wh[0][l][q] + Wh[1][l][q];
+      }
+    } else {
+      // CW = 1: eight row groups of 8 rows; reduce through Wh[.][l][0] in two halves
+      if (h1 < 2) Wh[h1][l1][0] = acc;
+      __syncthreads();
+      for (int g = 2; g < 8; g += 2) {
+        if (h1 == g || h1 == g + 1) Wh[h1 - g][l1][0] = Wh[h1 - g][l1][0] + acc;
+        __syncthreads();
+      }
+      if (tid < WY_NB) W[tid][0] = Wh[0][tid][0] + Wh[1][tid][0];
     }
     __syncthreads();
     // W2 = T_b W
-    for (int t = tid; t < nr * WY_CW; t += HT_NT) {
+    for (int t = tid; t < nr * CW; t += HT_NT) {
       const int k = t % nr, q = t / nr;
-      cplx<double> acc = mk(0.0, 0.0);
-      for (int l = k; l < nr; ++l) acc = acc + cmul(Ts[k][l], W[l][q]);
-      W2[k][q] = acc;
+      cplx<double> s = mk(0.0, 0.0);
+      for (int l = k; l < nr; ++l) s = s + cmul(Ts[k][l], W[l][q]);
+      W2[k][q] = s;
     }
-    __syncthreads();
-    // C -= V_b W2 (rows from i0 + 1; v_l contributes from row i0 + l + 1)
-    for (int r = i0 + 1 + tid; r < n; r += HT_NT) {
-      cplx<double> acc[WY_CW];
-#pragma unroll
-      for (int q = 0; q < WY_CW; ++q) acc[q] = mk(0.0, 0.0);
-      const int lmax = min(nr - 1, r - i0 - 1);
-      for (int l = 0; l <= lmax; ++l) {
-        const cplx<double> a = r == i0 + l + 1 ? mk(1.0, 0.0) : V[(long)(i0 + l) * ldv + r];
-#pragma unroll
-        for (int q = 0; q < WY_CW; ++q) acc[q] = acc[q] + cmul(a, W2[l][q]);
+    // C -= V_b W2, chunk by chunk (thread: row of the chunk, column)
+    for (int rb = i0 + 1; rb < n; rb += WY_RC) {
+      __syncthreads();   // W2 written / the previous chunk's readers are done
+      for (int e = tid; e < WY_RC * WY_NB; e += HT_NT) {
+        const int rr = e & (WY_RC - 1), l = e >> 6, r = rb + rr;
+        cplx<double> v = mk(0.0, 0.0);
+        if (r < n && l < nr) {
+          const int r1 = i0 + l + 1;
+          v = r == r1 ? mk(1.0, 0.0) : (r > r1 ? V[(long)(i0 + l) * ldv + r] : mk(0.0, 0.0));
+        }
+        Vs[rr][l] = v;
       }
-#pragma unroll
-      for (int q = 0; q < WY_CW; ++q) Cs[(long)q * n + r] = Cs[(long)q * n + r] - acc[q];
+      __syncthreads();
+      for (int t = tid; t < WY_RC * CW; t += HT_NT) {
+        const int rr = t & (WY_RC - 1), q = t >> 6, r = rb + rr;
+        if (r < n && q < nc) {
+          cplx<double> s = mk(0.0, 0.0);
+          for (int l = 0; l < nr; ++l) s = s + cmul(Vs[rr][l], W2[l][q]);
+          Cs[q * n + r] = Cs[q * n + r] - s;
+        }
+      }
     }
     __syncthreads();
   }
@@ -1167,9 +1215,15 @@ int heevd_dist(pw_ctx* c, void* rb, int n, cplx<double>* A, long lda, double* d_
     hipLaunchKernelGGL(k_larft, dim3(nblk), dim3(WY_NT_T), 0, c->stream, V, ldv, tau, n, Tb);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_check(c, e, "k_larft");
-    const int blocks = (nvec + WY_CW - 1) / WY_CW;
-    const size_t sh = (size_t)WY_CW * n * sizeof(cplx<double>);
-    hipLaunchKernelGGL(k_apply_q_wy, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, Tb, n, A, lda, nvec);
+    // four columns per workgroup while they fit LDS beside the static staging (~60 KiB)
+    const bool cw4 = (size_t)WY_CW * n * sizeof(cplx<double>) <= 96 * 1024;
+    const int cw = cw4 ? WY_CW : 1;
+    const int blocks = (nvec + cw - 1) / cw;
+    const size_t sh = (size_t)cw * n * sizeof(cplx<double>);
+    if (cw4)
+      hipLaunchKernelGGL(k_apply_q_wy<WY_CW>, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, Tb, n, A, lda, nvec);
+    else
+      hipLaunchKernelGGL(k_apply_q_wy<1>, dim3(blocks), dim3(HT_NT), sh, c->stream, V, ldv, Tb, n, A, lda, nvec);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_check(c, e, "k_apply_q_wy");
   }
