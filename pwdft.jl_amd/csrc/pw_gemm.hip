@@ -122,10 +122,11 @@ k_zgemm(int M, int N, int K, const cplx<TA>* __restrict__ A, long lda, const cpl
   };
   auto b_idx = [&](int s, int& kb, int& nn) { kb = tid % SBK; nn = tid / SBK + (NT / SBK) * s; };
   // a stage wholly inside the matrices (every stage of an interior tile but the last of its
-  // K chunk) is loaded and stored without per-element checks; FP32-MFMA A^H B keeps the
-  // masked form only (measured 6-10% slower with the extra path at the projector and Gram
-  // shapes, 2-4% faster for the other three forms)
-  constexpr bool FAST = !(sizeof(TC) == sizeof(float) && TRANSA == 1);
+  // K chunk) is loaded and stored without per-element checks, in workgroups of at most four
+  // wavefronts except FP32-MFMA A^H B: measured 2-5 % faster there at the Si64 projector
+  // shapes, but 6-10 % slower for FP32-MFMA A^H B and ~10 % slower in the 8-wavefront tiles
+  // of the Si216 shapes (the extra path's registers)
+  constexpr bool FAST = WM * WN <= 4 && !(sizeof(TC) == sizeof(float) && TRANSA == 1);
   bool sfull = false;
   auto load_stage = [&](int k0) {
     sfull = FAST && m0 + BMC <= M && n0 + BNC <= N && k0 + SBK <= kend;
