@@ -132,10 +132,10 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
         rph[lane] = ph;
       }
       __syncthreads();
-      // A <- J^H A J, block (rows of pair k, columns of pair k2); k = lane mod 32 and k2 stepping
-      // by SM_NT / 32 (npair <= 32), no run-time division
-      static_assert(SM_NMAX / 2 <= 32 && SM_NMAX <= 64, "lane maps of the round updates");
-      for (int k2 = lane >> 5, k = lane & 31; k < npair && k2 < npair; k2 += SM_NT / 32) {
+      // A <- J^H A J, block (rows of pair k, columns of pair k2), one block per thread (a
+      // lane map on 32-wide rows of blocks idles most lanes at the batched orders 16 and 24)
+      for (int t = lane; t < npair * npair; t += SM_NT) {
+        const int k = t % npair, k2 = t / npair;
         const int p = rp[k], q = rq[k], p2 = rp[k2], q2 = rq[k2];
         const double s = rs[k], s2 = rs[k2];
         if (s == 0.0 && s2 == 0.0) continue;
@@ -161,8 +161,9 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
         if (qv) a[q + n * p2] = b10;
         if (qv && q2v) a[q + n * q2] = b11;
       }
-      // V <- V J, columns of pair k2; row i = lane mod 64 (n <= 64)
-      for (int k2 = lane >> 6, i = lane & 63; i < n && k2 < npair; k2 += SM_NT / 64) {
+      // V <- V J, columns of pair k2
+      for (int t = lane; t < npair * n; t += SM_NT) {
+        const int k2 = t / n, i = t % n;
         const int p2 = rp[k2], q2 = rq[k2];
         const double s2 = rs[k2];
         if (q2 < 0 || s2 == 0.0) continue;
