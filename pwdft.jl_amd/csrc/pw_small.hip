@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
 
 // Matrix b of the stack: inv(chol(Hermitian(A)).U) in place, strict lower triangle zeroed;
 // info[b] (potrf) and info[nbatch + b] (trtri) as rocSOLVER would set them.
-// Thread (ti, tj) of a 16 x 16 grid holds the entries (ti + 16 a, tj + 16 b), a, b < 4, in
+// Thread (ti, tj) of a G x G grid (G = 16 or 32) holds the entries (ti + G a, tj + G b), a, b < 64 / G, in
 // registers from the load to the store.  Factorization (right-looking): at step k the holders
 // of row k publish it to an LDS row (two rows alternate), one barrier, then every thread
 // scales what it needs of row k by 1 / sqrt(A_kk) and updates its own trailing entries.
@@ -212,21 +212,23 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
 // S_ij) / U_ii and publish it, one barrier, then every thread adds U_ri X_ij to its sums S_rj
 // of the rows r < i (U's column i read from LDS, where U was stored after the factorization).
 // One barrier per step in both phases.
-__global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restrict__ A, long lda, long stride, int n,
-                                                          int* __restrict__ info, int nbatch) {
+template <int G>
+__global__ void __launch_bounds__(G * G) k_chol_inv_small(cplx<double>* __restrict__ A, long lda, long stride, int n,
+                                                           int* __restrict__ info, int nbatch) {
+  constexpr int E = SM_NMAX / G;   // entries per thread along each axis
   extern __shared__ char smem_raw[];
   cplx<double>* u = reinterpret_cast<cplx<double>*>(smem_raw);   // n x n: U after the factorization
   __shared__ cplx<double> rowbuf[2][SM_NMAX];
   __shared__ double rdiag[SM_NMAX];
-  static_assert(SM_NT == 256 && SM_NMAX <= 64, "16 x 16 grid, 4 x 4 entries per thread");
+  static_assert((G == 16 || G == 32) && SM_NMAX == 64, "G x G grid, E x E entries per thread");
   cplx<double>* M = A + (long)blockIdx.x * stride;
-  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-  cplx<double> r[4][4];   // r[b][a]: entry (ti + 16 a, tj + 16 b)
+  const int tid = threadIdx.x, ti = tid % G, tj = tid / G;
+  cplx<double> r[E][E];   // r[b][a]: entry (ti + G a, tj + G b)
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+  for (int b = 0; b < E; ++b)
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = ti + 16 * a, j = tj + 16 * b;
+    for (int a = 0; a < E; ++a) {
+      const int i = ti + G * a, j = tj + G * b;
       cplx<double> v = mk(0.0, 0.0);
       if (j < n && i <= j) {
         v = M[(long)j * lda + i];
@@ -236,14 +238,14 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
     }
   int bad = 0;
   for (int k = 0; k < n; ++k) {
-    const int buf = k & 1, ak = k >> 4;
-    if (ti == (k & 15)) {
+    const int buf = k & 1, ak = k / G;
+    if (ti == k % G) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < E; ++a)
         if (a == ak) {
 #pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            const int j = tj + 16 * b;
+          for (int b = 0; b < E; ++b) {
+            const int j = tj + G * b;
             if (j >= k && j < n) rowbuf[buf][j] = r[b][a];
           }
         }
@@ -257,27 +259,27 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
     // 1 / sqrt and sqrt from one reciprocal square root; the row reads are unconditional
     // (every index < SM_NMAX is a valid LDS slot) so they issue together and wait once
     const double rd = rsqrt(dkk), d = dkk * rd;
-    cplx<double> uki[4], ukj[4];
+    cplx<double> uki[E], ukj[E];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = ti + 16 * a;
+    for (int a = 0; a < E; ++a) {
+      const int i = ti + G * a;
       const cplx<double> v = rowbuf[buf][i];
       uki[a] = (i > k && i < n) ? scal(rd, v) : mk(0.0, 0.0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int j = tj + 16 * b;
+    for (int b = 0; b < E; ++b) {
+      const int j = tj + G * b;
       const cplx<double> v = rowbuf[buf][j];
       ukj[b] = (j > k && j < n) ? scal(rd, v) : mk(0.0, 0.0);
     }
     // tile rows a and columns b wholly before k are finished: skipped on every thread alike
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (16 * b + 15 < k || 16 * b >= n) continue;
+    for (int b = 0; b < E; ++b) {
+      if (G * b + G - 1 < k || G * b >= n) continue;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (16 * a + 15 < k || 16 * a > 16 * b + 15) continue;
-        const int i = ti + 16 * a, j = tj + 16 * b;
+      for (int a = 0; a < E; ++a) {
+        if (G * a + G - 1 < k || G * a > G * b + G - 1) continue;
+        const int i = ti + G * a, j = tj + G * b;
         if (j < n && i <= j) {
           if (i == k) r[b][a] = j == k ? mk(d, 0.0) : scal(rd, r[b][a]);            // row k of U
           else if (i > k) r[b][a] = r[b][a] - cmulc(ukj[b], uki[a]);               // A_ij -= conj(U_ki) U_kj
@@ -295,10 +297,10 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
   }
   // U into LDS (its columns feed the inverse), 1 / U_ii
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+  for (int b = 0; b < E; ++b)
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = ti + 16 * a, j = tj + 16 * b;
+    for (int a = 0; a < E; ++a) {
+      const int i = ti + G * a, j = tj + G * b;
       if (j < n && i <= j) {
         u[i + n * j] = r[b][a];
         if (i == j) rdiag[i] = 1.0 / r[b][a].x;
@@ -307,14 +309,14 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
     }
   __syncthreads();
   for (int i = n - 1; i >= 0; --i) {
-    const int buf = i & 1, ai = i >> 4;
-    if (ti == (i & 15)) {
+    const int buf = i & 1, ai = i / G;
+    if (ti == i % G) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < E; ++a)
         if (a == ai) {
 #pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            const int j = tj + 16 * b;
+          for (int b = 0; b < E; ++b) {
+            const int j = tj + G * b;
             if (j >= i && j < n) {
               const cplx<double> x = scal(rdiag[i], mk((j == i ? 1.0 : 0.0) - r[b][a].x, -r[b][a].y));
               r[b][a] = x;
@@ -324,36 +326,36 @@ __global__ void __launch_bounds__(SM_NT) k_chol_inv_small(cplx<double>* __restri
         }
     }
     __syncthreads();
-    cplx<double> uri[4], xij[4];
+    cplx<double> uri[E], xij[E];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int rr = ti + 16 * a;
+    for (int a = 0; a < E; ++a) {
+      const int rr = ti + G * a;
       const cplx<double> v = u[min(rr, n - 1) + n * i];   // unconditional (clamped) read
       uri[a] = rr < i ? v : mk(0.0, 0.0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int j = tj + 16 * b;
+    for (int b = 0; b < E; ++b) {
+      const int j = tj + G * b;
       const cplx<double> v = rowbuf[buf][j];
       xij[b] = (j >= i && j < n) ? v : mk(0.0, 0.0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (16 * b + 15 < i || 16 * b >= n) continue;
+    for (int b = 0; b < E; ++b) {
+      if (G * b + G - 1 < i || G * b >= n) continue;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (16 * a >= i) continue;
-        const int rr = ti + 16 * a, j = tj + 16 * b;
+      for (int a = 0; a < E; ++a) {
+        if (G * a >= i) continue;
+        const int rr = ti + G * a, j = tj + G * b;
         if (rr < i && j >= i && j < n) r[b][a] = r[b][a] + cmul(uri[a], xij[b]);
       }
     }
   }
   // X (upper triangle) and zeros below the diagonal
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+  for (int b = 0; b < E; ++b)
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = ti + 16 * a, j = tj + 16 * b;
+    for (int a = 0; a < E; ++a) {
+      const int i = ti + G * a, j = tj + G * b;
       if (i < n && j < n) M[(long)j * lda + i] = i <= j ? cplx<double>(r[b][a]) : mk(0.0, 0.0);
     }
   if (tid == 0) {
@@ -373,7 +375,12 @@ int heev_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, lo
 int chol_inv_small_batch(pw_ctx* c, int n, int nbatch, cplx<double>* A, long lda, long stride, int* d_info) {
   if (n < 1 || n > SM_NMAX || nbatch < 1) return set_error(c, PW_ERR_ARG, "chol_inv_small_batch: order out of range");
   const size_t lds = (size_t)n * n * sizeof(cplx<double>);
-  hipLaunchKernelGGL(k_chol_inv_small, dim3(nbatch), dim3(SM_NT), lds, c->stream, A, lda, stride, n, d_info, nbatch);
+  // one matrix above order 32 (the recursive inverse's blocks): a 32 x 32 grid of 2 x 2 tiles,
+  // a quarter of the arithmetic per thread and step; stacks of small matrices keep 16 x 16
+  if (nbatch == 1 && n > 32)
+    hipLaunchKernelGGL(k_chol_inv_small<32>, dim3(1), dim3(1024), lds, c->stream, A, lda, stride, n, d_info, nbatch);
+  else
+    hipLaunchKernelGGL(k_chol_inv_small<16>, dim3(nbatch), dim3(256), lds, c->stream, A, lda, stride, n, d_info, nbatch);
   return hip_check(c, hipGetLastError(), "k_chol_inv_small");
 }
 
