@@ -735,8 +735,23 @@ __device__ inline int tri_sturm(const double* sd, const double* se2, int n, doub
   double q = sd[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   int cnt = q <= 0.0 ? 1 : 0;
-#pragma unroll 4
-  for (int j = 1; j < n; ++j) {
+  // the serial chain runs on registers: each block's LDS operands are read ahead of it
+  int j = 1;
+  for (; j + 8 <= n; j += 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = sd[j + u] - x;
+      ev[u] = se2[j + u - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      q = fma(-ev[u], rcp_nr(q), dv[u]);
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q <= 0.0 ? 1 : 0;
+    }
+  }
+  for (; j < n; ++j) {
     q = fma(-se2[j - 1], rcp_nr(q), sd[j] - x);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q <= 0.0 ? 1 : 0;
@@ -821,32 +836,42 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d, 
     // Gaussian elimination of T - lam I with partial pivoting; row j of U is
     // (u0_j, su1_j, su2_j) on columns j, j + 1, j + 2.
     double p = sd[0] - lam, q = n > 1 ? se[0] : 0.0;
-    for (int j = 0; j < n - 1; ++j) {
-      const double s = se[j];
-      const double a1 = sd[j + 1] - lam;
-      const double b1 = j + 1 < n - 1 ? se[j + 1] : 0.0;
-      double u0;
-      if (fabs(p) >= fabs(s)) {
-        if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
-        const double lj = s * rcp_nr(p);
-        u0 = p;
-        su1[j] = q;
-        su2[j] = 0.0;
-        sw[j] = 0;
-        sl[j] = lj;
-        p = a1 - lj * q;
-        q = b1;
-      } else {
-        const double lj = p * rcp_nr(s);
-        u0 = s;
-        su1[j] = a1;
-        su2[j] = b1;
-        sw[j] = 1;
-        sl[j] = lj;
-        p = q - lj * a1;
-        q = -lj * b1;
+    // blocks of 8 steps whose operands (d, e of the rows ahead, not yet overwritten) are read
+    // into registers before the serial chain; se[n - 1] is 0, so b1 needs no bound test
+    for (int j0 = 0; j0 < n - 1; j0 += 8) {
+      double dblk[8], eblk[9];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) dblk[t] = sd[min(j0 + 1 + t, n - 1)];
+#pragma unroll
+      for (int t = 0; t < 9; ++t) eblk[t] = se[min(j0 + t, n - 1)];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 + t;
+        if (j >= n - 1) break;
+        const double s = eblk[t], a1 = dblk[t] - lam, b1 = eblk[t + 1];
+        double u0;
+        if (fabs(p) >= fabs(s)) {
+          if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
+          const double lj = s * rcp_nr(p);
+          u0 = p;
+          su1[j] = q;
+          su2[j] = 0.0;
+          sw[j] = 0;
+          sl[j] = lj;
+          p = a1 - lj * q;
+          q = b1;
+        } else {
+          const double lj = p * rcp_nr(s);
+          u0 = s;
+          su1[j] = a1;
+          su2[j] = b1;
+          sw[j] = 1;
+          sl[j] = lj;
+          p = q - lj * a1;
+          q = -lj * b1;
+        }
+        sd[j] = 1.0 / u0;
       }
-      sd[j] = 1.0 / u0;
     }
     if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
     sd[n - 1] = 1.0 / p;
@@ -857,22 +882,49 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d, 
       // forward: the row interchanges and multipliers applied to the right-hand side, the
       // running entry carried in a register
       double yj = sx[0];
-      for (int j = 0; j < n - 1; ++j) {
-        double yn = sx[j + 1];
-        const bool s = sw[j] != 0;
-        const double a = s ? yn : yj, b = s ? yj : yn;
-        sx[j] = a;
-        yj = fma(-sl[j], a, b);
+      for (int j0 = 0; j0 < n - 1; j0 += 8) {
+        double yb[8], lb[8];
+        int wb[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = min(j0 + t, n - 2);
+          yb[t] = sx[j + 1];
+          lb[t] = sl[j];
+          wb[t] = sw[j];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = j0 + t;
+          if (j >= n - 1) break;
+          const bool s = wb[t] != 0;
+          const double a = s ? yb[t] : yj, b = s ? yj : yb[t];
+          sx[j] = a;
+          yj = fma(-lb[t], a, b);
+        }
       }
       sx[n - 1] = yj;
       // back substitution through U
       double x2 = 0.0, x1 = sx[n - 1] * sd[n - 1];
       sx[n - 1] = x1;
-      for (int j = n - 2; j >= 0; --j) {
-        const double xj = (sx[j] - su1[j] * x1 - su2[j] * x2) * sd[j];
-        sx[j] = xj;
-        x2 = x1;
-        x1 = xj;
+      for (int j0 = n - 2; j0 >= 0; j0 -= 8) {
+        double yb[8], ub1[8], ub2[8], rb[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = max(j0 - t, 0);
+          yb[t] = sx[j];
+          ub1[t] = su1[j];
+          ub2[t] = su2[j];
+          rb[t] = sd[j];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int j = j0 - t;
+          if (j < 0) break;
+          const double xj = (yb[t] - ub1[t] * x1 - ub2[t] * x2) * rb[t];
+          sx[j] = xj;
+          x2 = x1;
+          x1 = xj;
+        }
       }
     }
     __syncthreads();
