@@ -2,30 +2,33 @@
 // diag_davidson.jl; ortho_sqrt.jl:6-9), built for the sizes the block eigensolvers meet
 // (n = Nst .. 3 Nst, a few hundred to about a thousand):
 //
-//  1. k_hetrd_dist: Householder reduction to real symmetric tridiagonal form,
-//     T = Q^H A Q, Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^H (LAPACK ZHETD2,
+//  1. k_hetrd_dist / k_hetrd_dist1: Householder reduction to real symmetric tridiagonal
+//     form, T = Q^H A Q, Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^H (LAPACK ZHETD2,
 //     lower variant).  One persistent cooperative launch: the matrix lives in the LDS of P
 //     workgroups (column j in workgroup j mod P), so the n - 1 Householder steps never
-//     re-read it from HBM and never pay a kernel launch.  Each step has two cross-workgroup
-//     synchronisation points: the owner of column i publishes v_i, and after every
+//     re-read it from HBM and never pay a kernel launch.  k_hetrd_dist has two cross-workgroup
+//     synchronisation points per step: the owner of column i publishes v_i, and after every
 //     workgroup has formed p = tau A22 v for its own columns (A22 is Hermitian, so
-//     (A22 v)_j = A22[:, j]^H v needs only local columns) the p segments are gathered.  The
+//     (A22 v)_j = A22[:, j]^H v needs only local columns) the p segments are gathered; the
 //     owner of column i + 1 updates that column first and publishes v_{i+1} before it
-//     updates the rest of its columns (look-ahead).
+//     updates the rest of its columns (look-ahead).  k_hetrd_dist1 has one: with its p
+//     segments the owner of the next column publishes that column, and every workgroup
+//     forms the next reflector itself.
 //  2. The eigenpairs of T: for the block eigensolvers (which keep only the lowest third of
 //     a Rayleigh-Ritz problem) the wanted ones from the device kernels k_tri_bisect,
 //     k_tri_invit and k_tri_cluster below; otherwise every one through rocSOLVER's divide
 //     and conquer (ZSTEDC).
 //  3. k_larft + k_apply_q_wy: eigenvectors of A = Q (eigenvectors of T), the reflectors
 //     applied by blocks of 32 in compact WY form (I - V T V^H) to the wanted columns only,
-//     four columns per workgroup staged in LDS.
+//     four columns per workgroup staged in LDS (one beyond n ~ 1500), each block's
+//     reflectors staged by 64-row chunks.
 //
 // Only the upper triangle of A is read, as Julia's Hermitian(A) does.  Cross-workgroup
 // waits are bounded: a wait that exceeds its spin limit sets a status word, every other
-// wait then ends at once, and the host reports the failure.  A step costs ~17 us at
-// n = 480 (26 workgroups): about eight round trips to device-coherent memory on its
-// critical path (flag, v_i, p segments, partial sums, the next reflector), which bounds
-// this one-dimensional distribution.
+// wait then ends at once, and the host reports the failure.  A step costs ~7.4 us at
+// n = 480 with 64 workgroups and one exchange (17 us with 26 workgroups and two): the round
+// trips to device-coherent memory on its critical path bound this one-dimensional
+// distribution.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
