@@ -8,8 +8,9 @@
 //
 //  - k_heev_jacobi: Hermitian eigenpairs by cyclic Jacobi rotations in tournament order
 //    (n/2 disjoint rotations per round, all applied at once), a pair
-//    rotated while |A_pq| > max(1e-15 sqrt|A_pp A_qq|, 2 eps ||A||_F), until a sweep rotates
-//    no pair; eigenvalues sorted ascending with their vectors.  Complex 2x2 step: the pair (p, q) with A_pq = |g| e^{i phi} is made
+//    rotated while |A_pq| > max(1e-15 sqrt|A_pp A_qq|, 2 eps ||A||_F), until no pair passes
+//    that test (checked on every pair after each sweep, in place of a sweep that would only
+//    confirm it); eigenvalues sorted ascending with their vectors.  Complex 2x2 step: the pair (p, q) with A_pq = |g| e^{i phi} is made
 //    real by diag(1, e^{-i phi}) and diagonalised by the classical rotation
 //    t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = (A_qq - A_pp) / (2 |g|).
 //  - k_chol_inv_small: U = chol(Hermitian(A)).U by right-looking elimination, then U^-1 by
@@ -65,7 +66,7 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
   const int lane = threadIdx.x;
   const int nn = n * n;
   __shared__ double red[SM_NT / 64];
-  __shared__ int flag;
+  __shared__ int flag, flag2;
   // Hermitian(M): upper triangle, real diagonal
   double fro = 0.0;
   for (int t = lane; t < nn; t += SM_NT) {
@@ -175,10 +176,28 @@ __global__ void __launch_bounds__(SM_NT) k_heev_jacobi(cplx<double>* __restrict_
       __syncthreads();
     }
     if (nrot) flag = 1;   // any thread that rotated this sweep
+    if (lane == 0) flag2 = 0;
     __syncthreads();
     const int more = flag;
-    __syncthreads();
     if (!more) {
+      converged = true;
+      break;
+    }
+    // would the next sweep rotate any pair?  The same test as the rotations', on every pair
+    // of the matrix as it stands: when none passes, that sweep would change nothing, so the
+    // sweep that only confirms convergence is skipped (the result is the same to the bit)
+    for (int t = lane; t < nn; t += SM_NT) {
+      const int i = t % n, j = t / n;
+      if (i < j) {
+        const cplx<double> g = a[i + n * j];
+        const double ag2 = g.x * g.x + g.y * g.y;
+        if (ag2 > 1e-30 * fabs(a[i + n * i].x * a[j + n * j].x) && ag2 > gfloor * gfloor) flag2 = 1;
+      }
+    }
+    __syncthreads();
+    const int again = flag2;
+    __syncthreads();
+    if (!again) {
       converged = true;
       break;
     }
